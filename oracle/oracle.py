@@ -1,0 +1,440 @@
+"""ctypes wrapper around the FP64 C oracle and the compiled reference driver.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` as the checker.  The product
+package ``midas_amd`` never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_SO = HERE / "_build" / "libmidasoracle.so"
+REF_SO = HERE / "_ref" / "libmidasref.so"
+REFERENCE_SRC = Path("/root/reference/FF_HEDM/src")
+
+# Map.bin record, MapperCore.h:19-25 (24 bytes).
+ENTRY_DTYPE = np.dtype([("y", "<f4"), ("z", "<f4"), ("frac", "<f8"),
+                        ("deltaR", "<f4"), ("areaWeight", "<f4")])
+
+
+class Params(C.Structure):
+    """Mirror of orc_params (midas_oracle.h)."""
+    _fields_ = [
+        ("NrPixelsY", C.c_int), ("NrPixelsZ", C.c_int),
+        ("pxY", C.c_double), ("pxZ", C.c_double), ("yCen", C.c_double),
+        ("zCen", C.c_double), ("Lsd", C.c_double), ("RhoD", C.c_double),
+        ("tx", C.c_double), ("ty", C.c_double), ("tz", C.c_double),
+        ("p", C.c_double * 15),
+        ("RMin", C.c_double), ("RMax", C.c_double), ("RBinSize", C.c_double),
+        ("EtaMin", C.c_double), ("EtaMax", C.c_double), ("EtaBinSize", C.c_double),
+        ("QMin", C.c_double), ("QMax", C.c_double), ("QBinSize", C.c_double),
+        ("Wavelength", C.c_double),
+        ("NrTransOpt", C.c_int), ("TransOpt", C.c_int * 10),
+        ("SubPixelLevel", C.c_int), ("SubPixelCardinalWidth", C.c_double),
+        ("Parallax", C.c_double),
+        ("SolidAngleCorrection", C.c_int), ("PolarizationCorrection", C.c_int),
+        ("PolarizationFraction", C.c_double),
+        ("NPanelsY", C.c_int), ("NPanelsZ", C.c_int),
+        ("PanelSizeY", C.c_int), ("PanelSizeZ", C.c_int),
+        ("PanelGapsY", C.c_int * 64), ("PanelGapsZ", C.c_int * 64),
+        ("hasMask", C.c_int), ("hasDistortion", C.c_int), ("hasResidual", C.c_int),
+        ("MaskFile", C.c_char * 1024), ("DistortionFile", C.c_char * 1024),
+        ("ResidualCorrectionMap", C.c_char * 1024),
+        ("PanelShiftsFile", C.c_char * 1024),
+    ]
+
+
+class Panel(C.Structure):
+    _fields_ = [("id", C.c_int), ("yMin", C.c_int), ("yMax", C.c_int),
+                ("zMin", C.c_int), ("zMax", C.c_int), ("dY", C.c_double),
+                ("dZ", C.c_double), ("dTheta", C.c_double), ("dLsd", C.c_double),
+                ("dP2", C.c_double), ("centerY", C.c_double), ("centerZ", C.c_double)]
+
+
+def build(with_ref: bool | None = None) -> None:
+    """Compile the oracle (and, when /root/reference exists, the reference
+    driver) with oracle/Makefile."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "all"], check=True)
+    if with_ref is None:
+        with_ref = REFERENCE_SRC.is_dir()
+    if with_ref:
+        subprocess.run(["make", "-s", "-C", str(HERE), "ref"], check=True)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not ORACLE_SO.exists():
+            build(with_ref=False)
+        L = C.CDLL(str(ORACLE_SO))
+        P = C.POINTER
+        d = C.c_double
+        L.orc_params_default.argtypes = [P(Params)]
+        L.orc_parse_mapper_params.argtypes = [C.c_char_p, P(Params)]
+        L.orc_bin_counts.argtypes = [P(Params), P(C.c_int), P(C.c_int)]
+        L.orc_bin_edges.argtypes = [P(Params)] + [P(d)] * 4
+        L.orc_tilt_matrix.argtypes = [d, d, d, P(d)]
+        L.orc_pixel_to_reta.argtypes = [P(Params), P(d), d, d, d, d, P(d), P(d), P(d)]
+        L.orc_generate_panels.argtypes = [P(Params), P(Panel), C.c_int]
+        L.orc_generate_panels.restype = C.c_int
+        L.orc_image_transform.argtypes = [C.c_int, P(C.c_int), P(d), C.c_int, C.c_int]
+        L.orc_header_string.argtypes = [P(Params), C.c_int, C.c_char_p, C.c_int]
+        L.orc_map_build.argtypes = [P(Params), P(d), P(d), P(d), P(d), P(Panel),
+                                    C.c_int, C.c_int]
+        L.orc_map_build.restype = C.c_void_p
+        L.orc_map_nnz.argtypes = [C.c_void_p]
+        L.orc_map_nnz.restype = C.c_int64
+        L.orc_map_nbins.argtypes = [C.c_void_p]
+        L.orc_map_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_map_counters.argtypes = [C.c_void_p, P(C.c_int64)]
+        L.orc_map_free.argtypes = [C.c_void_p]
+        L.orc_integrate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                    C.c_int, P(d), C.c_int, C.c_int, d, d,
+                                    C.c_int, C.c_int, C.c_int, C.c_int, P(d), P(d)]
+        L.orc_area_inbounds.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                        C.c_int, P(d)]
+        L.orc_lineouts.argtypes = [P(d), P(d), C.c_int, C.c_int, P(d), P(d), P(d)]
+        L.orc_reta_map.argtypes = [P(d)] * 4 + [C.c_int, C.c_int, P(d), d, d, d, P(d)]
+        L.orc_tch_eta_fwhm.argtypes = [d, d, P(d), P(d)]
+        L.orc_tch_model.argtypes = [C.c_int, P(d), C.c_int, P(d), d, d, P(d)]
+        L.orc_pv_model.argtypes = [P(d), C.c_int, P(d), P(d)]
+        L.orc_lm_fit.argtypes = [C.c_int, C.c_int, P(d), P(d), P(d), P(d), P(d),
+                                 C.c_int, P(d)]
+        L.orc_lm_fit.restype = C.c_int
+        L.orc_window_init.argtypes = [C.c_int, C.c_int, P(d), P(d), d, d, P(d),
+                                      P(d), P(d)]
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The compiled reference driver, or None where it is not built."""
+    global _ref
+    if _ref is None:
+        if not REF_SO.exists():
+            if not REFERENCE_SRC.is_dir():
+                return None
+            build(with_ref=True)
+        L = C.CDLL(str(REF_SO))
+        P = C.POINTER
+        d = C.c_double
+        L.ref_build_map.argtypes = [P(Params), P(d), P(d), P(d), P(d), P(d), C.c_int]
+        L.ref_build_map.restype = C.c_void_p
+        L.ref_map_nnz.argtypes = [C.c_void_p]
+        L.ref_map_nnz.restype = C.c_int64
+        L.ref_map_nbins.argtypes = [C.c_void_p]
+        L.ref_map_copy.argtypes = [C.c_void_p] * 4
+        L.ref_map_free.argtypes = [C.c_void_p]
+        L.ref_integrate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                    C.c_int, P(d), C.c_int, C.c_int, d, d,
+                                    C.c_int, C.c_int, P(d), P(d)]
+        L.ref_pixel_to_reta.argtypes = [P(Params), C.c_int, P(d), P(d), P(d), P(d)]
+        assert L.ref_entry_size() == 24
+        _ref = L
+    return _ref
+
+
+def _dp(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_params(**kw) -> Params:
+    P = Params()
+    lib().orc_params_default(C.byref(P))
+    for k, v in kw.items():
+        if k == "p":
+            for i, x in enumerate(v):
+                P.p[i] = x
+        elif k == "TransOpt":
+            P.NrTransOpt = len(v)
+            for i, x in enumerate(v):
+                P.TransOpt[i] = x
+        elif k in ("PanelGapsY", "PanelGapsZ"):
+            arr = getattr(P, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        elif k == "BC":
+            P.yCen, P.zCen = v
+        elif k == "px":
+            P.pxY = P.pxZ = v
+        else:
+            setattr(P, k, v)
+    return P
+
+
+def parse_params(path) -> Params:
+    P = Params()
+    L = lib()
+    L.orc_params_default(C.byref(P))
+    if L.orc_parse_mapper_params(str(path).encode(), C.byref(P)) != 0:
+        raise FileNotFoundError(path)
+    return P
+
+
+def bin_edges(P: Params):
+    L = lib()
+    Q = Params.from_buffer_copy(P)
+    nR, nE = C.c_int(), C.c_int()
+    L.orc_bin_counts(C.byref(Q), C.byref(nR), C.byref(nE))
+    RLo, RHi = np.zeros(nR.value), np.zeros(nR.value)
+    ELo, EHi = np.zeros(nE.value), np.zeros(nE.value)
+    L.orc_bin_edges(C.byref(Q), _dp(RLo), _dp(RHi), _dp(ELo), _dp(EHi))
+    return RLo, RHi, ELo, EHi, Q
+
+
+def header_string(P: Params) -> str:
+    RLo, RHi, ELo, EHi, Q = bin_edges(P)
+    q = 1 if (P.QBinSize > 0 and P.Wavelength > 0) else 0
+    buf = C.create_string_buffer(4096)
+    lib().orc_header_string(C.byref(Q), q, buf, 4096)
+    return buf.value.decode()
+
+
+def generate_panels(P: Params, shifts=None):
+    arr = (Panel * 4096)()
+    n = lib().orc_generate_panels(C.byref(P), arr, 4096)
+    panels = arr if n > 0 else None
+    if shifts is not None:
+        for row in shifts:
+            i = int(row[0])
+            if 0 <= i < n:
+                arr[i].dY, arr[i].dZ = row[1], row[2]
+                if len(row) > 3:
+                    arr[i].dTheta = row[3]
+                if len(row) > 4:
+                    arr[i].dLsd = row[4]
+                if len(row) > 5:
+                    arr[i].dP2 = row[5]
+    return panels, n
+
+
+def transform(P: Params, img):
+    out = np.ascontiguousarray(img, dtype=np.float64).copy()
+    opts = (C.c_int * 10)(*P.TransOpt)
+    lib().orc_image_transform(P.NrTransOpt, opts, _dp(out), P.NrPixelsY, P.NrPixelsZ)
+    return out
+
+
+class MapArrays:
+    """Flattened Map.bin / nMap.bin / maskMap.bin content."""
+
+    def __init__(self, entries, pairs, flags, n_r, n_eta, skipped=None):
+        self.entries = entries
+        self.pairs = pairs
+        self.flags = flags
+        self.n_r = n_r
+        self.n_eta = n_eta
+        self.skipped = skipped
+
+    @property
+    def counts(self):
+        return self.pairs[:, 0]
+
+    @property
+    def offsets(self):
+        return self.pairs[:, 1]
+
+
+def build_map(P: Params, mask=None, dist_y=None, dist_z=None, residual=None,
+              panel_shifts=None, threads=0) -> MapArrays:
+    """Oracle mapper.  ``mask``/``dist_*`` are given in raw detector
+    orientation and transformed here like DetectorMapper.c:640-676."""
+    L = lib()
+    m_t = transform(P, mask) if mask is not None else None
+    dy_t = transform(P, dist_y) if dist_y is not None else None
+    dz_t = transform(P, dist_z) if dist_z is not None else None
+    res = np.ascontiguousarray(residual, dtype=np.float64) if residual is not None else None
+    panels, npan = generate_panels(P, panel_shifts)
+    h = L.orc_map_build(C.byref(P), _dp(m_t), _dp(dy_t), _dp(dz_t), _dp(res),
+                        panels, npan if panels is not None else 0, threads)
+    try:
+        nnz = L.orc_map_nnz(h)
+        nb = L.orc_map_nbins(h)
+        entries = np.zeros(nnz, dtype=ENTRY_DTYPE)
+        pairs = np.zeros((nb, 2), dtype=np.int32)
+        flags = np.zeros(nb, dtype=np.int32)
+        L.orc_map_copy(h, entries.ctypes.data, pairs.ctypes.data, flags.ctypes.data)
+        sk = (C.c_int64 * 3)()
+        L.orc_map_counters(h, sk)
+    finally:
+        L.orc_map_free(h)
+    nR, nE = C.c_int(), C.c_int()
+    L.orc_bin_counts(C.byref(P), C.byref(nR), C.byref(nE))
+    return MapArrays(entries, pairs, flags, nR.value, nE.value, tuple(sk))
+
+
+def ref_build_map(P: Params, mask=None, dist_y=None, dist_z=None, residual=None,
+                  panel_shifts=None) -> MapArrays | None:
+    L = ref_lib()
+    if L is None:
+        return None
+    res = np.ascontiguousarray(residual, dtype=np.float64) if residual is not None else None
+    sh = None
+    nsh = 0
+    if panel_shifts is not None:
+        sh = np.zeros((len(panel_shifts), 6))
+        for i, row in enumerate(panel_shifts):
+            sh[i, :len(row)] = row
+        nsh = len(panel_shifts)
+    mk = np.ascontiguousarray(mask, dtype=np.float64) if mask is not None else None
+    dy = np.ascontiguousarray(dist_y, dtype=np.float64) if dist_y is not None else None
+    dz = np.ascontiguousarray(dist_z, dtype=np.float64) if dist_z is not None else None
+    h = L.ref_build_map(C.byref(P), _dp(mk), _dp(dy), _dp(dz), _dp(res), _dp(sh), nsh)
+    try:
+        nnz = L.ref_map_nnz(h)
+        nb = L.ref_map_nbins(h)
+        entries = np.zeros(nnz, dtype=ENTRY_DTYPE)
+        pairs = np.zeros((nb, 2), dtype=np.int32)
+        flags = np.zeros(nb, dtype=np.int32)
+        L.ref_map_copy(h, entries.ctypes.data, pairs.ctypes.data, flags.ctypes.data)
+    finally:
+        L.ref_map_free(h)
+    RLo, RHi, ELo, EHi, Q = bin_edges(P)
+    return MapArrays(entries, pairs, flags, len(RLo), len(ELo))
+
+
+def integrate(m: MapArrays, image, NY, NZ, *, bc=(0.0, 0.0), gradient=False,
+              normalize=True, semantics=0, threads=0):
+    """FP64 cake of an already dark-subtracted image."""
+    img = np.ascontiguousarray(image, dtype=np.float64).reshape(-1)
+    nb = m.n_r * m.n_eta
+    cake = np.zeros(nb)
+    area = np.zeros(nb)
+    lib().orc_integrate(m.entries.ctypes.data, m.pairs.ctypes.data,
+                        m.flags.ctypes.data if m.flags is not None else None,
+                        m.n_r, m.n_eta, _dp(img), NY, NZ, bc[0], bc[1],
+                        int(gradient), int(normalize), semantics, threads,
+                        _dp(cake), _dp(area))
+    return cake.reshape(m.n_r, m.n_eta), area.reshape(m.n_r, m.n_eta)
+
+
+def ref_integrate(m: MapArrays, image, NY, NZ, *, bc=(0.0, 0.0), gradient=False,
+                  normalize=True):
+    L = ref_lib()
+    if L is None:
+        return None
+    img = np.ascontiguousarray(image, dtype=np.float64).reshape(-1)
+    nb = m.n_r * m.n_eta
+    cake = np.zeros(nb)
+    area = np.zeros(nb)
+    L.ref_integrate(m.entries.ctypes.data, m.pairs.ctypes.data, m.flags.ctypes.data,
+                    m.n_r, m.n_eta, _dp(img), NY, NZ, bc[0], bc[1], int(gradient),
+                    int(normalize), _dp(cake), _dp(area))
+    return cake.reshape(m.n_r, m.n_eta), area.reshape(m.n_r, m.n_eta)
+
+
+def area_inbounds(m: MapArrays, NY, NZ):
+    nb = m.n_r * m.n_eta
+    a = np.zeros(nb)
+    lib().orc_area_inbounds(m.entries.ctypes.data, m.pairs.ctypes.data, nb, NY, NZ, _dp(a))
+    return a.reshape(m.n_r, m.n_eta)
+
+
+def lineouts(cake, area_inb):
+    cake = np.ascontiguousarray(cake, dtype=np.float64)
+    a = np.ascontiguousarray(area_inb, dtype=np.float64)
+    nR, nE = cake.shape
+    lc, la, lm = np.zeros(nR), np.zeros(nR), np.zeros(nR)
+    lib().orc_lineouts(_dp(cake), _dp(a), nR, nE, _dp(lc), _dp(la), _dp(lm))
+    return lc, la, lm
+
+
+def reta_map(P: Params, area, lam):
+    RLo, RHi, ELo, EHi, Q = bin_edges(P)
+    nR, nE = len(RLo), len(ELo)
+    out = np.zeros((5, nR, nE))
+    a = np.ascontiguousarray(area, dtype=np.float64)
+    lib().orc_reta_map(_dp(RLo), _dp(RHi), _dp(ELo), _dp(EHi), nR, nE, _dp(a),
+                       P.pxY, P.Lsd, lam, _dp(out))
+    return out
+
+
+def pixel_to_reta(P: Params, Y, Z):
+    L = lib()
+    T = np.zeros(9)
+    L.orc_tilt_matrix(P.tx, P.ty, P.tz, _dp(T))
+    Y = np.asarray(Y, dtype=np.float64).ravel()
+    Z = np.asarray(Z, dtype=np.float64).ravel()
+    R = np.zeros_like(Y)
+    E = np.zeros_like(Y)
+    r, e = C.c_double(), C.c_double()
+    for i in range(Y.size):
+        L.orc_pixel_to_reta(C.byref(P), _dp(T), Y[i], Z[i], 0.0, 0.0, None,
+                            C.byref(r), C.byref(e))
+        R[i], E[i] = r.value, e.value
+    return R, E
+
+
+def ref_pixel_to_reta(P: Params, Y, Z):
+    L = ref_lib()
+    if L is None:
+        return None
+    Y = np.ascontiguousarray(Y, dtype=np.float64).ravel()
+    Z = np.ascontiguousarray(Z, dtype=np.float64).ravel()
+    R = np.zeros_like(Y)
+    E = np.zeros_like(Y)
+    L.ref_pixel_to_reta(C.byref(P), Y.size, _dp(Y), _dp(Z), _dp(R), _dp(E))
+    return R, E
+
+
+# ---------------------------------------------------------------- fits
+def tch_eta_fwhm(sig, gam):
+    f, e = C.c_double(), C.c_double()
+    lib().orc_tch_eta_fwhm(sig, gam, C.byref(f), C.byref(e))
+    return f.value, e.value
+
+
+def tch_model(params, x, xlo=None, xhi=None):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    n_peaks = (p.size - 2) // 4
+    out = np.zeros_like(x)
+    lib().orc_tch_model(n_peaks, _dp(p), x.size, _dp(x),
+                        x[0] if xlo is None else xlo, x[-1] if xhi is None else xhi,
+                        _dp(out))
+    return out
+
+
+def pv_model(params, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    out = np.zeros_like(x)
+    lib().orc_pv_model(_dp(p), x.size, _dp(x), _dp(out))
+    return out
+
+
+def window_init(model, x, y, rstep, rmean):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    n = 5 if model == 0 else 6
+    p, lo, hi = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().orc_window_init(model, x.size, _dp(x), _dp(y), rstep, rmean, _dp(p), _dp(lo), _dp(hi))
+    return p, lo, hi
+
+
+def lm_fit(model, x, y, p0, lo, hi, max_iter=200):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    p = np.array(p0, dtype=np.float64)
+    lo = np.ascontiguousarray(lo, dtype=np.float64)
+    hi = np.ascontiguousarray(hi, dtype=np.float64)
+    chi = C.c_double()
+    it = lib().orc_lm_fit(model, x.size, _dp(x), _dp(y), _dp(p), _dp(lo), _dp(hi),
+                          max_iter, C.byref(chi))
+    return p, chi.value, it
+
+
+def n_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
