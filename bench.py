@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Throughput of the caking path on one MI355X.
+
+Workload (batch_2048): 64 frames of 2048x2048 uint16 (12 Gaussian rings up
+to 1000 px, dark 100 + Poisson(5), Poisson background 50), R 10-1000 px by
+0.25 px x eta 1 deg (3960 x 360 bins), solid-angle + polarisation (0.99)
+weights in the map, bilinear mode, Normalize=1, mask NaN, fused lineouts.
+The map is built on the GPU by the FP64 mapper, frames are generated on the
+GPU and stay resident: the timed region is integration only.
+
+A step is one pass of the batched kernel over the 64 frames (one launch at
+batch 64).  `value` = frames / s of the batched kernel; the one-frame kernel
+(64 launches per step) is reported beside it, with per-frame latency
+percentiles from HIP events, the HBM roofline of the batched kernel and the
+FP64 CPU oracle (all host cores) on a bounded sample of the same frames.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_frame(n_px, n_bins, n_r, nnz, batch, frame_bytes=2):
+    """bytes/frame = S_frame + S_out + (S_map + S_static)/B with
+    S_frame = N_px * sizeof(pixel), S_out = n_bins*4 + 2*n_r*4,
+    S_map = nnz*8 + (n_bins+1)*4 + n_bins*4 + n_bins, S_static = N_px*4."""
+    s_frame = n_px * frame_bytes
+    s_out = n_bins * 4 + 2 * n_r * 4
+    s_map = nnz * 8 + (n_bins + 1) * 4 + n_bins * 4 + n_bins
+    s_static = n_px * 4
+    return s_frame + s_out + (s_map + s_static) / batch
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--frames", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--lds", type=int, default=0)
+    ap.add_argument("--mode", default="bilinear")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(frames shard by frame with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import midas_amd as ma
+    from midas_amd import synth
+
+    P = synth.batch_2048()
+    dev = torch.device("cuda:0")
+    log("building the 2048^2 map on the GPU (FP64 mapper)")
+    t0 = time.perf_counter()
+    g = ma.build_map(P)
+    t_map = time.perf_counter() - t0
+    info = g.info()
+    log(f"map: {info.nnz} entries, {P.n_r_bins}x{P.n_eta_bins} bins, {t_map:.2f} s")
+    frames, dark = synth.torch_frames(P, args.frames, seed=0, n_rings=12, device=dev)
+    g.set_dark(dark)
+    g.set_tuning(batch=args.batch, rows_per_group=args.rows, lds_bytes=args.lds)
+    B, H, L = g.tuning()
+    n_bins = P.n_r_bins * P.n_eta_bins
+    npx = P.NrPixelsY * P.NrPixelsZ
+    cakes = torch.empty((args.frames, P.n_r_bins, P.n_eta_bins), dtype=torch.float32,
+                        device=dev)
+    lo = torch.empty((args.frames, 3, P.n_r_bins), dtype=torch.float32, device=dev)
+    log("first integration (layout build)")
+    ma.integrate_batch(frames[:2], g, mode=args.mode, mask_nan=True)
+    ma.integrate_batch(frames[:1], g, mode=args.mode, mask_nan=True)
+
+    # ---- batched: the timed steps ----
+    log(f"warmup {args.warmup} + {args.steps} steps of {args.frames} frames (batch {B})")
+    ma.time_integrate(g, frames, kind="batched", mode=args.mode, iters=1, warmup=args.warmup,
+                      cakes_dev=cakes, lineouts_dev=lo)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms_launch_b, ms_iter_b = ma.time_integrate(g, frames, kind="batched", mode=args.mode,
+                                               iters=args.steps, warmup=0, cakes_dev=cakes,
+                                               lineouts_dev=lo)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fps_b = args.frames * args.steps / wall
+    log(f"batched: {fps_b:,.0f} frames/s wall, kernel {ms_launch_b*1e3:.1f} us/launch")
+
+    # ---- one-frame kernel: latency distribution ----
+    log("one-frame kernel")
+    ms_launch_p, ms_iter_p = ma.time_integrate(g, frames, kind="plain", mode=args.mode,
+                                               iters=args.plain_steps, warmup=1,
+                                               cakes_dev=cakes, lineouts_dev=lo)
+    fps_p = args.frames / (ms_iter_p / 1e3)
+    lat = []
+    for i in range(min(args.frames, 64)):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t_a = time.perf_counter()
+        ma.integrate_batch(frames[i], g, mode=args.mode, mask_nan=True, lineouts=True)
+        lat.append((time.perf_counter() - t_a) * 1e3)
+    p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+
+    # ---- roofline of the batched kernel ----
+    bpf = algorithmic_bytes_per_frame(npx, n_bins, P.n_r_bins, info.nnz, B)
+    nf_launch = min(B, args.frames)
+    achieved = bpf * nf_launch / (ms_launch_b / 1e3) / 1e9
+    bpf_plain = algorithmic_bytes_per_frame(npx, n_bins, P.n_r_bins, info.nnz, 1)
+    achieved_plain = bpf_plain / (ms_launch_p / 1e3) / 1e9
+
+    # ---- CPU baseline: FP64 C oracle, all host cores ----
+    cpu = None
+    if not args.no_cpu:
+        try:
+            from oracle import oracle as orc
+            log("CPU baseline (FP64 oracle, all cores)")
+            pm = g.download()
+            om = orc.MapArrays(pm.pxList, pm.pairs(), pm.mask_flags, P.n_r_bins, P.n_eta_bins)
+            cores = os.cpu_count() or 1
+            fr_h = frames[:4].cpu().numpy()
+            dk = dark.cpu().numpy().astype(np.float64)
+            done, t_cpu = 0, 0.0
+            while t_cpu < args.cpu_seconds and done < 64:
+                img = fr_h[done % len(fr_h)].astype(np.float64) - dk
+                t_a = time.perf_counter()
+                orc.integrate(om, img, P.NrPixelsY, P.NrPixelsZ, bc=(P.BC_y, P.BC_z),
+                              threads=cores)
+                t_cpu += time.perf_counter() - t_a
+                done += 1
+            cpu = {"value": done / t_cpu, "unit": "frames/s", "cores": cores, "kind": "port",
+                   "sample": f"{done} frames of the same workload, FP64 C restatement of "
+                             "IntegratorZarrOMP.c:1710-1778 (OpenMP dynamic,64), "
+                             "dark-subtraction excluded like t_integration"}
+        except Exception as exc:  # the checker is optional on the box
+            log(f"CPU baseline skipped: {exc}")
+
+    out = {
+        "metric": "frames/s (2048^2 uint16 -> RxEta cake) batched vs plain; p50/p99 frame latency",
+        "value": fps_b,
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian rings + Poisson noise, generated on the GPU)",
+        "config": {"workload": f"batch_2048: {args.frames} x 2048x2048 uint16 frames, "
+                               f"{P.n_r_bins}x{P.n_eta_bins} bins (R 0.25 px, eta 1 deg), "
+                               f"solid angle + polarisation, {args.mode}, map nnz {info.nnz}",
+                   "batch": B, "rows_per_group": H, "lds_bytes": L,
+                   "frames_per_step": args.frames},
+        "plain_fps": fps_p,
+        "batched_over_plain": fps_b / fps_p,
+        "latency_ms": {"p50": p50, "p99": p99, "what": "one frame through mc_integrate "
+                       "(device-resident frame, host-synchronous call)"},
+        "kernel_us": {"batched_per_launch": ms_launch_b * 1e3, "plain_per_launch": ms_launch_p * 1e3},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_frame": bpf, "plain_achieved": achieved_plain},
+        "cpu_baseline": cpu,
+        "map_build_s": t_map,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,272 @@
+/*
+ * midas_caking.h -- C ABI of the MI355X caking library (libmidas_caking.so).
+ *
+ * Plain C types only: pointers, sizes, status codes.  Every function returns
+ * an int status (MC_OK = 0, negative on error) and never exits the process;
+ * mc_last_error() returns a thread-local message for the last failure.
+ *
+ * Ownership: the caller owns host arrays; the library owns device memory and
+ * handles, which are freed explicitly (mc_*_destroy).  A context serialises
+ * its GPU work on one HIP stream; map handles are immutable after they are
+ * built, except for the dark frame and the cached kernel layouts, which are
+ * guarded by the map's own lock.
+ *
+ * Reference interfaces replaced (paths relative to the MIDAS repository):
+ *   mc_map_build        mapper_build_map, FF_HEDM/src/MapperCore.h:52-84, and
+ *                       the DetectorMapper executable, DetectorMapper.c:53-888
+ *   mc_map_write        Map.bin/nMap.bin/maskMap.bin writer,
+ *                       DetectorMapper.c:813-881 + MapHeader.h:200-240
+ *   mc_map_load         ReadBins, IntegratorZarrOMP.c:97-191 (and the GPU
+ *                       stream's mmap, IntegratorFitPeaksGPUStream.cu:742-797)
+ *   mc_map_from_host    load_map/build_csr, packages/midas_integrate/
+ *                       midas_integrate/bin_io.py:206, kernels.py:242-375
+ *   mc_map_set_dark     dark averaging, IntegratorZarrOMP.c:1416-1454 and
+ *                       IntegratorFitPeaksGPUStream.cu:2192-2238
+ *   mc_integrate        per-bin loop IntegratorZarrOMP.c:1710-1778 (==
+ *                       integration_apply_map, IntegrationCore.h:28-36),
+ *                       integrate_noMapMask .cu:975-1050, the 1-D lineouts
+ *                       IntegratorZarrOMP.c:1829-1863 / .cu:1162-1334, and
+ *                       kernels.py integrate/profile_1d (:416-601)
+ *   mc_fit_windows      per-eta peak fits, IntegratorZarrOMP.c:1903-2070
+ *                       (pf_fit_single_peak PeakFit.h:158-160, fitPeaks
+ *                       PeakFit.h:114-116)
+ *   mc_server_*         the frame server of IntegratorFitPeaksGPUStream.cu
+ *                       :567-734, 2567-2883 and server.py FrameServer
+ */
+#ifndef MIDAS_CAKING_H
+#define MIDAS_CAKING_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define MC_OK 0
+#define MC_ERR_ARG (-1)
+#define MC_ERR_HIP (-2)
+#define MC_ERR_IO (-3)
+#define MC_ERR_FORMAT (-4)
+#define MC_ERR_BUDGET (-5)
+#define MC_ERR_STATE (-6)
+#define MC_ERR_NOMEM (-7)
+#define MC_ERR_TIMEOUT (-8)
+
+/* ---- frame dtype codes: the wire codes of the reference frame server
+ * (IntegratorFitPeaksGPUStream.cu:568-585) plus an int32 extension ---- */
+#define MC_DTYPE_U8 0
+#define MC_DTYPE_U16 1
+#define MC_DTYPE_U32 2
+#define MC_DTYPE_I64 3
+#define MC_DTYPE_F32 4
+#define MC_DTYPE_F64 5
+#define MC_DTYPE_HYBRID 6 /* u16 base + (i32 index, i64 value) patches */
+#define MC_DTYPE_I32 7
+
+/* ---- integration modes (kernels.py:1-30 vocabulary) ---- */
+#define MC_MODE_FLOOR 0    /* integer pixel (float offset, .cu:916) */
+#define MC_MODE_BILINEAR 1 /* bilinear at stored y,z (IntegratorZarrOMP) */
+#define MC_MODE_GRADIENT 2 /* bilinear at y,z - deltaR*(dy,dz)/R */
+
+/* ---- integration flags ---- */
+#define MC_F_NORMALIZE 1u /* divide by sum areaWeight (Normalize=1) */
+#define MC_F_MASK_NAN 2u  /* NaN for bins flagged in maskMap.bin */
+
+/* ---- lineout rows written per frame when requested ---- */
+#define MC_LINEOUT_CPU_MEAN 0 /* mean of values != 0 and not NaN */
+#define MC_LINEOUT_AREA 1     /* sum(I*A)/sum(A) over A > 1e-9 */
+#define MC_LINEOUT_MEAN 2     /* mean of I over A > 1e-9 */
+#define MC_N_LINEOUTS 3
+
+/* ---- library / context ---- */
+int mc_version(int *major, int *minor, int *patch);
+const char *mc_last_error(void);
+int mc_device_count(int *n);
+
+typedef struct mc_ctx mc_ctx;
+int mc_ctx_create(int device, mc_ctx **out);
+int mc_ctx_destroy(mc_ctx *ctx);
+int mc_ctx_sync(mc_ctx *ctx);
+/* The context's HIP stream, as an opaque pointer (hipStream_t). */
+void *mc_ctx_stream(mc_ctx *ctx);
+/* Device buffers owned by the library (for callers without an allocator). */
+int mc_dev_alloc(mc_ctx *ctx, uint64_t bytes, void **dptr);
+int mc_dev_free(mc_ctx *ctx, void *dptr);
+int mc_memcpy_h2d(mc_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int mc_memcpy_d2h(mc_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+
+/* ---- geometry: the DetectorMapper parameter set (DetectorMapper.c:84-114) */
+typedef struct mc_geometry {
+  int32_t NrPixelsY, NrPixelsZ;
+  double pxY, pxZ, yCen, zCen, Lsd, RhoD, tx, ty, tz;
+  double p[15];
+  double RMin, RMax, RBinSize, EtaMin, EtaMax, EtaBinSize;
+  double QMin, QMax, QBinSize, Wavelength;
+  int32_t NrTransOpt;
+  int32_t TransOpt[10];
+  int32_t SubPixelLevel;
+  double SubPixelCardinalWidth;
+  double Parallax;
+  int32_t SolidAngleCorrection, PolarizationCorrection;
+  double PolarizationFraction;
+} mc_geometry;
+
+/* Panel.h:9-22. */
+typedef struct mc_panel {
+  int32_t yMin, yMax, zMin, zMax;
+  double dY, dZ, dTheta, dLsd, dP2, centerY, centerZ;
+} mc_panel;
+
+typedef struct mc_map mc_map;
+
+typedef struct mc_map_info {
+  int64_t nnz;          /* Map.bin records */
+  int32_t n_r, n_eta;   /* bins */
+  int32_t ny, nz;       /* detector pixels */
+  int32_t n_flagged;    /* bins flagged by masked pixels */
+  int32_t q_mode;
+  int64_t hbm_bytes;    /* device memory held by this map */
+  int64_t skipped[3];   /* mapper rejections: <3 vertices, <3 unique, area */
+} mc_map_info;
+
+/* GPU FP64 DetectorMapper.  mask/distY/distZ are NrPixelsZ x NrPixelsY
+ * doubles in raw detector orientation (ImTransOpt is applied inside, as in
+ * DetectorMapper.c:640-676); residual is not transformed.  Any of them may
+ * be NULL.  Records come out bin-major, within a bin in the reference's
+ * single-thread loop order. */
+int mc_map_build(mc_ctx *ctx, const mc_geometry *g, const double *mask,
+                 const double *distY, const double *distZ,
+                 const double *residual, const mc_panel *panels,
+                 int32_t n_panels, mc_map **out);
+/* Map.bin / nMap.bin (+ maskMap.bin when any bin is flagged) into dir,
+ * each with the given 64-byte header (NULL: no header, legacy layout). */
+int mc_map_write(mc_map *m, const char *dir, const uint8_t *header64);
+/* Reads Map.bin/nMap.bin (header auto-detected) and optional maskMap.bin. */
+int mc_map_load(mc_ctx *ctx, const char *map_path, const char *nmap_path,
+                const char *maskmap_path, int32_t ny, int32_t nz, int32_t n_r,
+                int32_t n_eta, mc_map **out);
+/* From host arrays: records are 24-byte MapPixelData, pairs are nMap
+ * (count, offset) int32 pairs, flags may be NULL. */
+int mc_map_from_host(mc_ctx *ctx, const void *records, int64_t nnz,
+                     const int32_t *pairs, const int32_t *flags, int32_t ny,
+                     int32_t nz, int32_t n_r, int32_t n_eta, mc_map **out);
+int mc_map_info_get(mc_map *m, mc_map_info *info);
+/* Copies the map back: records (nnz*24 B), pairs (2*n_bins), flags. */
+int mc_map_download(mc_map *m, void *records, int32_t *pairs,
+                    int32_t *flags);
+/* Static per-bin axes (REtaMap, IntegratorZarrOMP.c:1754-1763): 5 rows
+ * R, 2theta, eta, sum areaWeight, Q; needs the bin edges. */
+int mc_map_reta(mc_map *m, const double *r_lo, const double *r_hi,
+                const double *eta_lo, const double *eta_hi, double px,
+                double lsd, double lambda, double *out5);
+int mc_map_destroy(mc_map *m);
+
+/* Dark: mean of n_frames frames (FP64 on the GPU) times scale (the
+ * reference scales by framesToSum, IntegratorZarrOMP.c:887-892).  NULL
+ * clears the dark.  bc_y/bc_z are only used by MC_MODE_GRADIENT. */
+int mc_map_set_dark(mc_map *m, const void *dark, int32_t dtype,
+                    int32_t n_frames, double scale, int32_t on_device);
+int mc_map_set_beam_center(mc_map *m, double bc_y, double bc_z);
+
+/* Tuning of the batched kernel: frames per pass (8..128), R rows per
+ * workgroup, LDS budget in bytes.  0 keeps the current value. */
+int mc_map_set_tuning(mc_map *m, int32_t batch, int32_t rows_per_group,
+                      int32_t lds_bytes);
+int mc_map_get_tuning(mc_map *m, int32_t *batch, int32_t *rows_per_group,
+                      int32_t *lds_bytes);
+/* Times candidate tunings on synthetic frames of the map's shape and keeps
+ * the fastest; writes the winner's figures. */
+int mc_map_autotune(mc_map *m, int32_t dtype, double *best_fps);
+
+/* Integrates n_frames frames (each NrPixelsZ x NrPixelsY, row-major) into
+ * cakes [n_frames][n_r][n_eta] float32 and, when lineouts != NULL,
+ * [n_frames][3][n_r] float32 (rows MC_LINEOUT_*).  *_on_device selects
+ * whether each pointer is a device or host address.  n_frames == 1 uses
+ * the one-frame kernel; larger counts use the batched kernel in passes of
+ * the tuned batch size.  Synchronous. */
+int mc_integrate(mc_map *m, const void *frames, int32_t dtype,
+                 int32_t n_frames, int32_t frames_on_device, int32_t mode,
+                 uint32_t flags, float *cakes, int32_t cakes_on_device,
+                 float *lineouts, int32_t lineouts_on_device);
+
+/* 1-D lineouts of cakes already computed ([n_frames][n_r][n_eta] float32)
+ * into [n_frames][3][n_r] (rows MC_LINEOUT_*); the profile kernels of
+ * IntegratorFitPeaksGPUStream.cu:1162-1334 and the CPU lineout of
+ * IntegratorZarrOMP.c:1829-1843.  NaN bins (mask) count as missing in all
+ * three definitions, as in the lineouts mc_integrate fuses. */
+int mc_profile(mc_map *m, const float *cakes, int32_t n_frames,
+               int32_t cakes_on_device, float *lineouts,
+               int32_t lineouts_on_device);
+
+/* Launch timing of the kernels on device-resident frames, with HIP events
+ * on the context's stream.  kind 0 = one-frame kernel (n_frames launches),
+ * 1 = batched kernel (n_frames/batch launches per iteration).
+ * ms_per_launch = mean kernel time, ms_per_iter = wall time per pass over
+ * all n_frames. */
+int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype,
+                      int32_t n_frames, int32_t mode, uint32_t flags,
+                      int32_t kind, int32_t iters, int32_t warmup,
+                      float *cakes_dev, float *lineouts_dev,
+                      double *ms_per_launch, double *ms_per_iter);
+
+/* ---- batched peak fits (per eta sector and ring window) ---- */
+#define MC_FIT_PV 0  /* height-normalised pV, 5 params [R, mu, G, Imax, BG] */
+#define MC_FIT_TCH 1 /* area-normalised TCH pV, 6 free params
+                        [area, center, sig, gam, bg0, bg1] */
+#define MC_FIT_OUT 8 /* per window: 7 PF_PARAMS_PER_PEAK values + status */
+
+/* Fits n_windows windows of n_points samples each.  x, y are
+ * [n_windows][n_points] float32 and peak_guess [n_windows] (the window's
+ * expected centre); rstep is the sample spacing.  out is
+ * [n_windows][MC_FIT_OUT] double: area, center, sig, gam, FWHM, eta, chi2
+ * (chi2/dof for TCH, SNR for pV, as the reference reports them), status
+ * (iterations, negative on failure).  Pointers are host or device. */
+int mc_fit_windows(mc_ctx *ctx, int32_t model, int32_t n_windows,
+                   int32_t n_points, const float *x, const float *y,
+                   const float *peak_guess, double rstep, int32_t max_iter,
+                   double *out, int32_t on_device);
+/* Cuts windows out of cakes and fits them in one launch: for every frame,
+ * every eta bin and every ring centre (in R bin units), the ROI of
+ * 2*pad+1 R bins around the ring (IntegratorZarrOMP.c:1966-1988).
+ * out is [n_frames][n_eta][n_rings][MC_FIT_OUT]. */
+int mc_fit_cake_rings(mc_map *m, const float *cakes_dev, int32_t n_frames,
+                      const double *ring_r_bins, int32_t n_rings, int32_t pad,
+                      int32_t model, int32_t max_iter, double *out,
+                      int32_t out_on_device);
+
+/* ---- continuous-batching frame server ---- */
+typedef struct mc_server mc_server;
+typedef struct mc_server_stats {
+  int64_t frames_done;
+  int64_t batches;
+  double p50_ms, p99_ms, mean_ms, max_ms;
+  double frames_per_s;
+  int64_t admitted_bytes, peak_bytes;
+  int64_t rejected;
+} mc_server_stats;
+/* Result callback: called from the server's completion thread with the
+ * request id, its geometry id and host pointers to the cake / lineouts
+ * (valid during the call). */
+typedef void (*mc_result_cb)(void *user, int64_t request_id, int32_t geometry,
+                             const float *cake, const float *lineouts,
+                             int32_t status);
+int mc_server_create(mc_ctx *ctx, int64_t hbm_budget_bytes,
+                     int32_t max_batch, int32_t queue_depth, mc_result_cb cb,
+                     void *user, mc_server **out);
+/* Registers a geometry (map, dtype, mode, flags); returns its id.  Fails
+ * with MC_ERR_BUDGET when its resident footprint does not fit. */
+int mc_server_add_geometry(mc_server *s, mc_map *m, int32_t dtype,
+                           int32_t mode, uint32_t flags, int32_t *geometry);
+/* Submits one frame (host memory, copied before return).  Thread-safe.
+ * Blocks while the queue is full (backpressure) up to timeout_ms. */
+int mc_server_submit(mc_server *s, int32_t geometry, const void *frame,
+                     int64_t request_id, int32_t timeout_ms);
+int mc_server_drain(mc_server *s, int32_t timeout_ms);
+int mc_server_stats_get(mc_server *s, mc_server_stats *st);
+int mc_server_destroy(mc_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,960 @@
+// mc_core.cpp -- host runtime of the MI355X caking library: contexts, map
+// residency, layouts for the kernels, dark handling and the integrate
+// entry points of the C ABI (include/midas_caking.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "mc_internal.h"
+
+namespace mc {
+
+static thread_local std::string g_err;
+
+void set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+// ------------------------------------------------------------------
+// Derived per-bin arrays + upload of the canonical map.
+// ------------------------------------------------------------------
+int map_finish_host(Map *m) {
+  const int64_t nb = m->nb();
+  MC_CHECK((int64_t)m->h_pairs.size() == 2 * nb, MC_ERR_FORMAT,
+           "nMap has %lld pairs, expected %lld", (long long)m->h_pairs.size() / 2,
+           (long long)nb);
+  if (m->h_flags.empty()) m->h_flags.assign(nb, 0);
+  for (int64_t b = 0; b < nb; b++) {
+    int64_t c = m->h_pairs[2 * b], o = m->h_pairs[2 * b + 1];
+    MC_CHECK(c >= 0 && o >= 0 && o + c <= m->nnz, MC_ERR_FORMAT,
+             "bin %lld references records [%lld, %lld) past %lld", (long long)b,
+             (long long)o, (long long)(o + c), (long long)m->nnz);
+  }
+  m->h_area_all.assign(nb, 0.0);
+  m->h_area_inb.assign(nb, 0.0);
+  std::vector<float> inv(nb), ainb(nb);
+  std::vector<float4> rs(m->n_r, make_float4(0.f, 0.f, 0.f, 0.f));
+  for (int64_t b = 0; b < nb; b++) {
+    double A = 0, Ai = 0;
+    const int64_t o = m->h_pairs[2 * b + 1], c = m->h_pairs[2 * b];
+    for (int64_t k = o; k < o + c; k++) {
+      const Rec &r = m->h_rec[k];
+      A += r.areaWeight;
+      if (!(r.y < 0 || r.y >= m->ny || r.z < 0 || r.z >= m->nz)) Ai += r.areaWeight;
+    }
+    m->h_area_all[b] = A;
+    m->h_area_inb[b] = Ai;
+    inv[b] = A > 0 ? (float)(1.0 / A) : 0.f;
+    ainb[b] = (float)Ai;
+    if (Ai > 1e-9) {
+      float4 &q = rs[b / m->n_eta];
+      q.x += (float)Ai;
+      q.y += 1.f;
+      if (!m->h_flags[b]) {
+        q.z += (float)Ai;
+        q.w += 1.f;
+      }
+    }
+  }
+  int rc;
+  if ((rc = m->d_rec.alloc(m->nnz * sizeof(Rec))) || (rc = m->d_pairs.alloc(2 * nb * 4)) ||
+      (rc = m->d_flags.alloc(nb * 4)) || (rc = m->d_area_inb.alloc(nb * 4)) ||
+      (rc = m->d_row_static.alloc(m->n_r * sizeof(float4))))
+    return rc;
+  static_assert(sizeof(float2) == 8, "float2");
+  MC_HIP(hipMemcpy(m->d_rec.p, m->h_rec.data(), m->nnz * sizeof(Rec), hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(m->d_pairs.p, m->h_pairs.data(), 2 * nb * 4, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(m->d_flags.p, m->h_flags.data(), nb * 4, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(m->d_area_inb.p, ainb.data(), nb * 4, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(m->d_row_static.p, rs.data(), m->n_r * sizeof(float4), hipMemcpyHostToDevice));
+  if ((rc = m->d_binfo.alloc(nb * sizeof(float2)))) return rc;
+  m->binfo_valid = false;
+  if ((rc = m->inv_area.alloc(nb * 4))) return rc;
+  MC_HIP(hipMemcpy(m->inv_area.p, inv.data(), nb * 4, hipMemcpyHostToDevice));
+  return MC_OK;
+}
+
+// ------------------------------------------------------------------
+// CSR of one integration mode (host build, then upload).
+// ------------------------------------------------------------------
+namespace {
+
+struct PW {
+  int32_t pix;
+  double w;
+};
+
+// Appends the (pixel, weight) contributions of record r for mode.
+inline void expand(const Map &m, const Rec &r, int mode, double bcy, double bcz,
+                   std::vector<PW> &out) {
+  const int NY = m.ny, NZ = m.nz;
+  const int64_t npx = m.npx();
+  if (mode == MC_MODE_FLOOR) {
+    // (long long)z * NY + y in float arithmetic, IntegratorFitPeaksGPUStream.cu:916.
+    long long zz = (long long)r.z * NY;
+    float o = (float)zz + r.y;
+    long long p = (long long)o;
+    if (p < 0) p = 0;
+    if (p >= npx) p = npx - 1;
+    out.push_back({(int32_t)p, r.frac});
+    return;
+  }
+  double ry = r.y, rz = r.z;
+  if (mode == MC_MODE_GRADIENT && r.deltaR != 0.0f) {
+    double dy = r.y - bcy, dz = r.z - bcz;
+    double R = sqrt(dy * dy + dz * dz);
+    if (R > 1.0) {
+      ry -= r.deltaR * dy / R;
+      rz -= r.deltaR * dz / R;
+    }
+  }
+  // Bilinear sampler of IntegratorZarrOMP.c:1733-1744 (floorf on the
+  // coordinate converted to float, edge clamps).
+  int iy = (int)floorf((float)ry), iz = (int)floorf((float)rz);
+  double fy = ry - iy, fz = rz - iz;
+  if (iy < 0) { iy = 0; fy = 0; }
+  if (iy >= NY - 1) { iy = NY - 2; fy = 1; }
+  if (iz < 0) { iz = 0; fz = 0; }
+  if (iz >= NZ - 1) { iz = NZ - 2; fz = 1; }
+  const int64_t b00 = (int64_t)iz * NY + iy;
+  const double w00 = (1 - fy) * (1 - fz), w10 = fy * (1 - fz), w01 = (1 - fy) * fz,
+               w11 = fy * fz;
+  if (w00 != 0) out.push_back({(int32_t)b00, w00 * r.frac});
+  if (w10 != 0) out.push_back({(int32_t)(b00 + 1), w10 * r.frac});
+  if (w01 != 0) out.push_back({(int32_t)(b00 + NY), w01 * r.frac});
+  if (w11 != 0) out.push_back({(int32_t)(b00 + NY + 1), w11 * r.frac});
+}
+
+template <class F> void parallel_for(int64_t n, F &&f) {
+  int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 4096) nt = 1;
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++)
+    th.emplace_back([&, t] {
+      int64_t a = n * t / nt, b = n * (t + 1) / nt;
+      f(a, b);
+    });
+  for (auto &x : th) x.join();
+}
+
+}  // namespace
+
+int ensure_csr(Map *m, int mode) {
+  MC_CHECK(mode >= 0 && mode <= 2, MC_ERR_ARG, "mode %d", mode);
+  Csr &c = m->csr[mode];
+  if (c.built && (mode != MC_MODE_GRADIENT || (c.bc_y == m->bc_y && c.bc_z == m->bc_z)))
+    return MC_OK;
+  const int64_t nb = m->nb();
+  // Pass 1: per-bin coalesced entry counts, built per thread range.
+  std::vector<std::vector<PW>> per_bin_tmp;  // not kept: two passes instead
+  std::vector<int32_t> cnt(nb, 0);
+  auto bin_entries = [&](int64_t b, std::vector<PW> &v) {
+    v.clear();
+    const int64_t o = m->h_pairs[2 * b + 1], n = m->h_pairs[2 * b];
+    for (int64_t k = o; k < o + n; k++) expand(*m, m->h_rec[k], mode, m->bc_y, m->bc_z, v);
+    std::sort(v.begin(), v.end(), [](const PW &a, const PW &b) { return a.pix < b.pix; });
+    size_t w = 0;
+    for (size_t i = 0; i < v.size(); i++) {
+      if (w > 0 && v[w - 1].pix == v[i].pix) v[w - 1].w += v[i].w;
+      else v[w++] = v[i];
+    }
+    v.resize(w);
+  };
+  parallel_for(nb, [&](int64_t a, int64_t b) {
+    std::vector<PW> v;
+    for (int64_t q = a; q < b; q++) {
+      bin_entries(q, v);
+      cnt[q] = (int32_t)v.size();
+    }
+  });
+  c.h_row.assign(nb + 1, 0);
+  for (int64_t b = 0; b < nb; b++) c.h_row[b + 1] = c.h_row[b] + cnt[b];
+  const int64_t nnz = c.h_row[nb];
+  c.h_pix.assign(nnz, 0);
+  c.h_w.assign(nnz, 0.f);
+  parallel_for(nb, [&](int64_t a, int64_t b) {
+    std::vector<PW> v;
+    for (int64_t q = a; q < b; q++) {
+      bin_entries(q, v);
+      int64_t o = c.h_row[q];
+      for (size_t i = 0; i < v.size(); i++) {
+        c.h_pix[o + i] = v[i].pix;
+        c.h_w[o + i] = (float)v[i].w;
+      }
+    }
+  });
+  c.nnz = nnz;
+  int rc;
+  if ((rc = c.row.alloc((nb + 1) * 4)) || (rc = c.pix.alloc(nnz * 4)) ||
+      (rc = c.w.alloc(nnz * 4)) || (rc = c.dterm.alloc(nb * 4)))
+    return rc;
+  MC_HIP(hipMemcpy(c.row.p, c.h_row.data(), (nb + 1) * 4, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(c.pix.p, c.h_pix.data(), nnz * 4, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(c.w.p, c.h_w.data(), nnz * 4, hipMemcpyHostToDevice));
+  c.bc_y = m->bc_y;
+  c.bc_z = m->bc_z;
+  c.built = true;
+  c.dterm_valid = false;
+  m->tiles[mode].clear();
+  if (m->binfo_mode == mode) m->binfo_valid = false;
+  return MC_OK;
+}
+
+// ------------------------------------------------------------------
+// Tile planner of the batched kernel.
+// ------------------------------------------------------------------
+namespace {
+struct BBox {
+  int y0 = INT32_MAX, z0 = INT32_MAX, y1 = -1, z1 = -1;  // inclusive
+  bool empty() const { return y1 < 0; }
+  void add(int y, int z) {
+    y0 = std::min(y0, y);
+    y1 = std::max(y1, y);
+    z0 = std::min(z0, z);
+    z1 = std::max(z1, z);
+  }
+  void add(const BBox &o) {
+    if (o.empty()) return;
+    y0 = std::min(y0, o.y0);
+    y1 = std::max(y1, o.y1);
+    z0 = std::min(z0, o.z0);
+    z1 = std::max(z1, o.z1);
+  }
+  int64_t area() const { return empty() ? 0 : (int64_t)(y1 - y0 + 1) * (z1 - z0 + 1); }
+};
+}  // namespace
+
+int ensure_tiles(Map *m, int mode, int esize, Tiles **out) {
+  int rc = ensure_csr(m, mode);
+  if (rc) return rc;
+  const int B = m->batch, H = m->rows_per_group;
+  for (auto &t : m->tiles[mode])
+    if (t->batch == B && t->rows == H && t->lds_bytes == m->lds_bytes && t->esize == esize) {
+      *out = t.get();
+      return MC_OK;
+    }
+  const Csr &c = m->csr[mode];
+  const int NY = m->ny, nE = m->n_eta, nR = m->n_r;
+  const int64_t nb = m->nb();
+  const int pitch = batched_lds_pitch(B, esize == 2 ? MC_DTYPE_U16 : MC_DTYPE_F32);
+  const int64_t lo_bytes = (int64_t)H * 4 * B * 4;
+  const int64_t cap_px = (m->lds_bytes - lo_bytes) / ((int64_t)pitch * esize);
+  MC_CHECK(cap_px >= 16, MC_ERR_ARG, "LDS budget %d too small for batch %d", m->lds_bytes, B);
+  // Per-bin pixel bounding boxes.
+  std::vector<BBox> bb(nb);
+  parallel_for(nb, [&](int64_t a, int64_t b) {
+    for (int64_t q = a; q < b; q++)
+      for (int32_t k = c.h_row[q]; k < c.h_row[q + 1]; k++)
+        bb[q].add(c.h_pix[k] % NY, c.h_pix[k] / NY);
+  });
+  struct TileH {
+    BBox box;
+    std::vector<int64_t> bins;
+    std::vector<char> direct;
+  };
+  struct GroupH {
+    int row0;
+    int64_t ents = 0;
+    std::vector<TileH> tiles;
+  };
+  const int n_groups = (nR + H - 1) / H;
+  std::vector<GroupH> groups(n_groups);
+  parallel_for(n_groups, [&](int64_t a, int64_t bq) {
+    for (int64_t gi = a; gi < bq; gi++) {
+      GroupH &G = groups[gi];
+      G.row0 = (int)gi * H;
+      const int r1 = std::min(nR, G.row0 + H);
+      TileH cur;
+      auto flush = [&]() {
+        if (!cur.bins.empty()) G.tiles.push_back(std::move(cur));
+        cur = TileH();
+      };
+      auto fits = [&](const BBox &b) { return b.area() <= cap_px; };
+      for (int e = 0; e < nE; e++) {
+        BBox col;
+        for (int r = G.row0; r < r1; r++) col.add(bb[(int64_t)r * nE + e]);
+        BBox uni = cur.box;
+        uni.add(col);
+        if (fits(uni)) {
+          cur.box = uni;
+          for (int r = G.row0; r < r1; r++) {
+            cur.bins.push_back((int64_t)r * nE + e);
+            cur.direct.push_back(0);
+          }
+          continue;
+        }
+        flush();
+        if (fits(col)) {
+          cur.box = col;
+          for (int r = G.row0; r < r1; r++) {
+            cur.bins.push_back((int64_t)r * nE + e);
+            cur.direct.push_back(0);
+          }
+          continue;
+        }
+        // Column too large: bin by bin.
+        for (int r = G.row0; r < r1; r++) {
+          const int64_t b = (int64_t)r * nE + e;
+          if (!fits(bb[b])) {
+            cur.bins.push_back(b);
+            cur.direct.push_back(1);
+            continue;
+          }
+          BBox u2 = cur.box;
+          u2.add(bb[b]);
+          if (!fits(u2)) {
+            flush();
+            u2 = bb[b];
+          }
+          cur.box = u2;
+          cur.bins.push_back(b);
+          cur.direct.push_back(0);
+        }
+      }
+      flush();
+      for (auto &T : G.tiles)
+        for (int64_t b : T.bins) G.ents += c.h_row[b + 1] - c.h_row[b];
+    }
+  });
+  // Launch order: heaviest row blocks first.
+  std::vector<int> order(n_groups);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return groups[a].ents > groups[b].ents; });
+  std::vector<int32_t> g_tile(n_groups + 1, 0), g_row0(n_groups);
+  std::vector<int32_t> tiles;
+  std::vector<int4> tbins;
+  std::vector<uint2> ents;
+  int64_t n_direct = 0;
+  int64_t max_tile_px = 0;
+  for (int oi = 0; oi < n_groups; oi++) {
+    GroupH &G = groups[order[oi]];
+    g_row0[oi] = G.row0;
+    for (auto &T : G.tiles) {
+      const int y0 = T.box.empty() ? 0 : T.box.y0, z0 = T.box.empty() ? 0 : T.box.z0;
+      const int bw = T.box.empty() ? 0 : T.box.y1 - T.box.y0 + 1;
+      const int bh = T.box.empty() ? 0 : T.box.z1 - T.box.z0 + 1;
+      max_tile_px = std::max<int64_t>(max_tile_px, (int64_t)bw * bh);
+      const int32_t b_beg = (int32_t)tbins.size();
+      for (size_t q = 0; q < T.bins.size(); q++) {
+        const int64_t b = T.bins[q];
+        const bool dir = T.direct[q];
+        const int32_t e_beg = (int32_t)ents.size();
+        for (int32_t k = c.h_row[b]; k < c.h_row[b + 1]; k++) {
+          const int p = c.h_pix[k];
+          uint32_t idx = dir ? (uint32_t)p : (uint32_t)((p / NY - z0) * bw + (p % NY - y0));
+          uint32_t wb;
+          memcpy(&wb, &c.h_w[k], 4);
+          ents.push_back(make_uint2(idx, wb));
+        }
+        const int rib = (int)(b / nE) - G.row0;
+        tbins.push_back(make_int4((int32_t)b, e_beg, (int32_t)ents.size(),
+                                  rib | (dir ? (1 << 30) : 0)));
+        n_direct += dir;
+      }
+      tiles.insert(tiles.end(), {y0, z0, bw, bh, b_beg, (int32_t)tbins.size()});
+    }
+    g_tile[oi + 1] = (int32_t)(tiles.size() / 6);
+  }
+  auto t = std::make_unique<Tiles>();
+  t->batch = B;
+  t->rows = H;
+  t->lds_bytes = m->lds_bytes;
+  t->esize = esize;
+  t->pitch = pitch;
+  t->n_groups = n_groups;
+  t->lds_needed = (size_t)lo_bytes + (size_t)max_tile_px * pitch * esize;
+  t->lds_needed = (t->lds_needed + 15) & ~(size_t)15;
+  t->n_tiles = (int64_t)tiles.size() / 6;
+  t->n_tbins = (int64_t)tbins.size();
+  t->n_ents = (int64_t)ents.size();
+  t->n_direct = n_direct;
+  MC_CHECK(t->n_tbins == nb, MC_ERR_STATE, "tile planner covered %lld of %lld bins",
+           (long long)t->n_tbins, (long long)nb);
+  if ((rc = t->group_tile.alloc(g_tile.size() * 4)) || (rc = t->group_row0.alloc(g_row0.size() * 4)) ||
+      (rc = t->tiles.alloc(tiles.size() * 4)) || (rc = t->tbins.alloc(tbins.size() * 16)) ||
+      (rc = t->ents.alloc(ents.size() * 8)))
+    return rc;
+  MC_HIP(hipMemcpy(t->group_tile.p, g_tile.data(), g_tile.size() * 4, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(t->group_row0.p, g_row0.data(), g_row0.size() * 4, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(t->tiles.p, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(t->tbins.p, tbins.data(), tbins.size() * 16, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(t->ents.p, ents.data(), ents.size() * 8, hipMemcpyHostToDevice));
+  t->built = true;
+  *out = t.get();
+  m->tiles[mode].push_back(std::move(t));
+  return MC_OK;
+}
+
+int ensure_binfo(Map *m, int mode, uint32_t flags) {
+  int rc = ensure_csr(m, mode);
+  if (rc) return rc;
+  Csr &c = m->csr[mode];
+  hipStream_t s = m->ctx->stream;
+  if (!c.dterm_valid) {
+    if ((rc = launch_dark_term(s, c.row.as<int32_t>(), c.pix.as<int32_t>(), c.w.as<float>(),
+                               m->has_dark ? m->d_dark.as<double>() : nullptr, m->nb(),
+                               c.dterm.as<float>())))
+      return rc;
+    c.dterm_valid = true;
+    m->binfo_valid = false;
+  }
+  if (m->binfo_valid && m->binfo_mode == mode && m->binfo_flags == flags) return MC_OK;
+  if ((rc = launch_binfo(s, m->inv_area.as<float>(), m->d_flags.as<int32_t>(),
+                         c.dterm.as<float>(), m->nb(), (flags & MC_F_NORMALIZE) ? 1 : 0,
+                         (flags & MC_F_MASK_NAN) ? 1 : 0, m->d_binfo.as<float2>())))
+    return rc;
+  m->binfo_mode = mode;
+  m->binfo_flags = flags;
+  m->binfo_valid = true;
+  return MC_OK;
+}
+
+// Identity frame slots 0..127 for contiguous batches.
+static int identity_slots(Ctx *ctx, int32_t **out) {
+  static thread_local std::vector<std::pair<Ctx *, void *>> cache;
+  for (auto &e : cache)
+    if (e.first == ctx) {
+      *out = (int32_t *)e.second;
+      return MC_OK;
+    }
+  void *p = nullptr;
+  MC_HIP(hipMalloc(&p, 128 * 4));
+  int32_t h[128];
+  for (int i = 0; i < 128; i++) h[i] = i;
+  MC_HIP(hipMemcpy(p, h, sizeof(h), hipMemcpyHostToDevice));
+  cache.push_back({ctx, p});
+  *out = (int32_t *)p;
+  return MC_OK;
+}
+
+int run_integrate(Map *m, const void *frames_dev, int dtype, int n_frames, int mode,
+                  uint32_t flags, float *cakes_dev, float *lineouts_dev, bool force_plain,
+                  bool force_batched) {
+  int rc;
+  if ((rc = ensure_binfo(m, mode, flags))) return rc;
+  hipStream_t s = m->ctx->stream;
+  const size_t es = dtype_size(dtype);
+  MC_CHECK(es > 0, MC_ERR_ARG, "unsupported dtype %d", dtype);
+  if ((n_frames == 1 && !force_batched) || force_plain) {
+    return launch_plain(s, *m, m->csr[mode], frames_dev, dtype, cakes_dev, lineouts_dev,
+                        n_frames, m->npx());
+  }
+  Tiles *t = nullptr;
+  if ((rc = ensure_tiles(m, mode, batched_elem_size(dtype), &t))) return rc;
+  int32_t *slots;
+  if ((rc = identity_slots(m->ctx, &slots))) return rc;
+  for (int f0 = 0; f0 < n_frames; f0 += t->batch) {
+    const int nf = std::min(t->batch, n_frames - f0);
+    const char *fb = (const char *)frames_dev + (size_t)f0 * m->npx() * es;
+    float *cb = cakes_dev + (size_t)f0 * m->nb();
+    float *lb = lineouts_dev ? lineouts_dev + (size_t)f0 * 3 * m->n_r : nullptr;
+    if ((rc = launch_batched(s, *m, *t, fb, dtype, nf, slots, cb, lb))) return rc;
+  }
+  return MC_OK;
+}
+
+}  // namespace mc
+
+using namespace mc;
+
+// ====================================================================
+// C ABI
+// ====================================================================
+extern "C" {
+
+int mc_version(int *major, int *minor, int *patch) {
+  if (major) *major = 0;
+  if (minor) *minor = 1;
+  if (patch) *patch = 0;
+  return MC_OK;
+}
+
+const char *mc_last_error(void) { return g_err.c_str(); }
+
+int mc_device_count(int *n) {
+  MC_CHECK(n, MC_ERR_ARG, "null");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *n = c;
+  return MC_OK;
+}
+
+int mc_ctx_create(int device, mc_ctx **out) {
+  MC_CHECK(out, MC_ERR_ARG, "null out");
+  int n = 0;
+  MC_HIP(hipGetDeviceCount(&n));
+  MC_CHECK(device >= 0 && device < n, MC_ERR_ARG, "device %d not present (%d devices)",
+           device, n);
+  MC_HIP(hipSetDevice(device));
+  auto *c = new mc_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    set_error("hipStreamCreate: %s", hipGetErrorString(e));
+    return MC_ERR_HIP;
+  }
+  *out = c;
+  return MC_OK;
+}
+
+int mc_ctx_destroy(mc_ctx *ctx) {
+  if (!ctx) return MC_OK;
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return MC_OK;
+}
+
+int mc_ctx_sync(mc_ctx *ctx) {
+  MC_CHECK(ctx, MC_ERR_ARG, "null ctx");
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  return MC_OK;
+}
+
+void *mc_ctx_stream(mc_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int mc_dev_alloc(mc_ctx *ctx, uint64_t bytes, void **dptr) {
+  MC_CHECK(ctx && dptr, MC_ERR_ARG, "null");
+  MC_HIP(hipSetDevice(ctx->device));
+  hipError_t e = hipMalloc(dptr, bytes ? bytes : 16);
+  if (e != hipSuccess) {
+    set_error("hipMalloc(%llu): %s", (unsigned long long)bytes, hipGetErrorString(e));
+    return MC_ERR_NOMEM;
+  }
+  return MC_OK;
+}
+
+int mc_dev_free(mc_ctx *ctx, void *dptr) {
+  (void)ctx;
+  if (dptr) MC_HIP(hipFree(dptr));
+  return MC_OK;
+}
+
+int mc_memcpy_h2d(mc_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+  MC_CHECK(ctx, MC_ERR_ARG, "null ctx");
+  MC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  return MC_OK;
+}
+
+int mc_memcpy_d2h(mc_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+  MC_CHECK(ctx, MC_ERR_ARG, "null ctx");
+  MC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  return MC_OK;
+}
+
+int mc_map_build(mc_ctx *ctx, const mc_geometry *g, const double *mask, const double *distY,
+                 const double *distZ, const double *residual, const mc_panel *panels,
+                 int32_t n_panels, mc_map **out) {
+  MC_CHECK(ctx && g && out, MC_ERR_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  MC_HIP(hipSetDevice(ctx->device));
+  auto *m = new mc_map();
+  m->ctx = ctx;
+  m->bc_y = g->yCen;
+  m->bc_z = g->zCen;
+  int rc = map_build_gpu(ctx, g, mask, distY, distZ, residual, panels, n_panels, m);
+  if (!rc) rc = map_finish_host(m);
+  if (rc) {
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return MC_OK;
+}
+
+static int read_file(const char *path, std::vector<char> &buf) {
+  FILE *f = fopen(path, "rb");
+  MC_CHECK(f, MC_ERR_IO, "cannot open %s", path);
+  fseek(f, 0, SEEK_END);
+  long n = ftell(f);
+  fseek(f, 0, SEEK_SET);
+  buf.resize(n);
+  size_t got = n > 0 ? fread(buf.data(), 1, n, f) : 0;
+  fclose(f);
+  MC_CHECK((long)got == n, MC_ERR_IO, "short read of %s", path);
+  return MC_OK;
+}
+
+static size_t header_offset(const std::vector<char> &b) {
+  if (b.size() >= 64) {
+    uint32_t magic;
+    memcpy(&magic, b.data(), 4);
+    if (magic == 0x3050414Du) return 64;  // "MAP0", MapHeader.h:40
+  }
+  return 0;
+}
+
+int mc_map_from_host(mc_ctx *ctx, const void *records, int64_t nnz, const int32_t *pairs,
+                     const int32_t *flags, int32_t ny, int32_t nz, int32_t n_r, int32_t n_eta,
+                     mc_map **out) {
+  MC_CHECK(ctx && pairs && out && (records || nnz == 0), MC_ERR_ARG, "null argument");
+  MC_CHECK(ny > 1 && nz > 1 && n_r > 0 && n_eta > 0 && nnz >= 0, MC_ERR_ARG,
+           "bad sizes ny=%d nz=%d n_r=%d n_eta=%d", ny, nz, n_r, n_eta);
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  MC_HIP(hipSetDevice(ctx->device));
+  auto *m = new mc_map();
+  m->ctx = ctx;
+  m->ny = ny;
+  m->nz = nz;
+  m->n_r = n_r;
+  m->n_eta = n_eta;
+  m->nnz = nnz;
+  m->bc_y = 0;
+  m->bc_z = 0;
+  const int64_t nb = (int64_t)n_r * n_eta;
+  m->h_rec.resize(nnz);
+  if (nnz) memcpy(m->h_rec.data(), records, nnz * sizeof(Rec));
+  m->h_pairs.assign(pairs, pairs + 2 * nb);
+  if (flags) m->h_flags.assign(flags, flags + nb);
+  int rc = map_finish_host(m);
+  if (rc) {
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return MC_OK;
+}
+
+int mc_map_load(mc_ctx *ctx, const char *map_path, const char *nmap_path,
+                const char *maskmap_path, int32_t ny, int32_t nz, int32_t n_r, int32_t n_eta,
+                mc_map **out) {
+  MC_CHECK(ctx && map_path && nmap_path && out, MC_ERR_ARG, "null argument");
+  std::vector<char> a, b, c;
+  int rc;
+  if ((rc = read_file(map_path, a)) || (rc = read_file(nmap_path, b))) return rc;
+  size_t oa = header_offset(a), ob = header_offset(b);
+  MC_CHECK((a.size() - oa) % 24 == 0, MC_ERR_FORMAT,
+           "%s: %zu data bytes is not a multiple of the 24-byte record (legacy map?)",
+           map_path, a.size() - oa);
+  const int64_t nb = (int64_t)n_r * n_eta;
+  MC_CHECK((int64_t)(b.size() - ob) == nb * 8, MC_ERR_FORMAT,
+           "%s holds %zu pairs, expected %lld bins", nmap_path, (b.size() - ob) / 8,
+           (long long)nb);
+  const int32_t *flags = nullptr;
+  if (maskmap_path && maskmap_path[0]) {
+    FILE *f = fopen(maskmap_path, "rb");
+    if (f) {
+      fclose(f);
+      if ((rc = read_file(maskmap_path, c))) return rc;
+      size_t oc = header_offset(c);
+      MC_CHECK((int64_t)(c.size() - oc) == nb * 4, MC_ERR_FORMAT,
+               "%s holds %zu flags, expected %lld", maskmap_path, (c.size() - oc) / 4,
+               (long long)nb);
+      flags = (const int32_t *)(c.data() + oc);
+    }
+  }
+  rc = mc_map_from_host(ctx, a.data() + oa, (int64_t)((a.size() - oa) / 24),
+                        (const int32_t *)(b.data() + ob), flags, ny, nz, n_r, n_eta, out);
+  if (!rc && oa == 64) (*out)->q_mode = (unsigned char)a[40];
+  return rc;
+}
+
+int mc_map_write(mc_map *m, const char *dir, const uint8_t *header64) {
+  MC_CHECK(m && dir, MC_ERR_ARG, "null argument");
+  std::string d(dir);
+  auto wr = [&](const std::string &name, const void *p, size_t n) -> int {
+    std::string path = d + "/" + name;
+    FILE *f = fopen(path.c_str(), "wb");
+    MC_CHECK(f, MC_ERR_IO, "cannot create %s", path.c_str());
+    if (header64) fwrite(header64, 1, 64, f);
+    size_t w = n ? fwrite(p, 1, n, f) : 0;
+    fclose(f);
+    MC_CHECK(w == n, MC_ERR_IO, "short write to %s", path.c_str());
+    return MC_OK;
+  };
+  int rc;
+  if ((rc = wr("Map.bin", m->h_rec.data(), m->nnz * sizeof(Rec)))) return rc;
+  if ((rc = wr("nMap.bin", m->h_pairs.data(), m->h_pairs.size() * 4))) return rc;
+  bool any = false;
+  for (int32_t f : m->h_flags) any |= f != 0;
+  if (any && (rc = wr("maskMap.bin", m->h_flags.data(), m->h_flags.size() * 4))) return rc;
+  return MC_OK;
+}
+
+int mc_map_info_get(mc_map *m, mc_map_info *info) {
+  MC_CHECK(m && info, MC_ERR_ARG, "null argument");
+  memset(info, 0, sizeof(*info));
+  info->nnz = m->nnz;
+  info->n_r = m->n_r;
+  info->n_eta = m->n_eta;
+  info->ny = m->ny;
+  info->nz = m->nz;
+  int nf = 0;
+  for (int32_t f : m->h_flags) nf += f != 0;
+  info->n_flagged = nf;
+  info->q_mode = m->q_mode;
+  int64_t hb = m->d_rec.bytes + m->d_pairs.bytes + m->d_flags.bytes + m->d_area_inb.bytes +
+               m->d_row_static.bytes + m->d_binfo.bytes + m->inv_area.bytes + m->d_dark.bytes;
+  for (int q = 0; q < 3; q++) {
+    const Csr &c = m->csr[q];
+    hb += c.row.bytes + c.pix.bytes + c.w.bytes + c.dterm.bytes;
+    for (auto &t : m->tiles[q])
+      hb += t->group_tile.bytes + t->group_row0.bytes + t->tiles.bytes + t->tbins.bytes +
+            t->ents.bytes;
+  }
+  info->hbm_bytes = hb;
+  for (int q = 0; q < 3; q++) info->skipped[q] = m->skipped[q];
+  return MC_OK;
+}
+
+int mc_map_download(mc_map *m, void *records, int32_t *pairs, int32_t *flags) {
+  MC_CHECK(m, MC_ERR_ARG, "null map");
+  if (records) memcpy(records, m->h_rec.data(), m->nnz * sizeof(Rec));
+  if (pairs) memcpy(pairs, m->h_pairs.data(), m->h_pairs.size() * 4);
+  if (flags) memcpy(flags, m->h_flags.data(), m->h_flags.size() * 4);
+  return MC_OK;
+}
+
+int mc_map_reta(mc_map *m, const double *r_lo, const double *r_hi, const double *eta_lo,
+                const double *eta_hi, double px, double lsd, double lambda, double *out5) {
+  MC_CHECK(m && r_lo && r_hi && eta_lo && eta_hi && out5, MC_ERR_ARG, "null argument");
+  const int64_t nb = m->nb();
+  for (int r = 0; r < m->n_r; r++) {
+    const double Rm = (r_lo[r] + r_hi[r]) / 2;
+    const double tth = atan(Rm * px / lsd);
+    for (int e = 0; e < m->n_eta; e++) {
+      const int64_t b = (int64_t)r * m->n_eta + e;
+      out5[b] = Rm;
+      out5[nb + b] = (180.0 / M_PI) * tth;
+      out5[2 * nb + b] = (eta_lo[e] + eta_hi[e]) / 2;
+      out5[3 * nb + b] = m->h_area_all[b];
+      out5[4 * nb + b] = lambda > 0 ? (4.0 * M_PI / lambda) * sin(tth / 2.0) : 0.0;
+    }
+  }
+  return MC_OK;
+}
+
+int mc_map_destroy(mc_map *m) {
+  if (!m) return MC_OK;
+  {
+    std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+    (void)hipStreamSynchronize(m->ctx->stream);
+  }
+  delete m;
+  return MC_OK;
+}
+
+int mc_map_set_dark(mc_map *m, const void *dark, int32_t dtype, int32_t n_frames, double scale,
+                    int32_t on_device) {
+  MC_CHECK(m, MC_ERR_ARG, "null map");
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  std::lock_guard<std::mutex> lk2(m->mu);
+  MC_HIP(hipSetDevice(m->ctx->device));
+  for (auto &c : m->csr) c.dterm_valid = false;
+  m->binfo_valid = false;
+  if (!dark) {
+    m->has_dark = false;
+    return MC_OK;
+  }
+  MC_CHECK(n_frames >= 1, MC_ERR_ARG, "n_frames %d", n_frames);
+  const size_t es = dtype_size(dtype);
+  MC_CHECK(es > 0, MC_ERR_ARG, "unsupported dark dtype %d", dtype);
+  int rc;
+  if ((rc = m->d_dark.alloc(m->npx() * 8))) return rc;
+  const void *src = dark;
+  DBuf tmp;
+  const size_t bytes = (size_t)n_frames * m->npx() * es;
+  if (!on_device) {
+    if ((rc = tmp.alloc(bytes))) return rc;
+    MC_HIP(hipMemcpyAsync(tmp.p, dark, bytes, hipMemcpyHostToDevice, m->ctx->stream));
+    src = tmp.p;
+  }
+  if ((rc = launch_dark_mean(m->ctx->stream, src, dtype, n_frames, m->npx(), scale,
+                             m->d_dark.as<double>())))
+    return rc;
+  MC_HIP(hipStreamSynchronize(m->ctx->stream));
+  m->has_dark = true;
+  return MC_OK;
+}
+
+int mc_map_set_beam_center(mc_map *m, double bc_y, double bc_z) {
+  MC_CHECK(m, MC_ERR_ARG, "null map");
+  std::lock_guard<std::mutex> lk(m->mu);
+  m->bc_y = bc_y;
+  m->bc_z = bc_z;
+  return MC_OK;
+}
+
+int mc_map_set_tuning(mc_map *m, int32_t batch, int32_t rows_per_group, int32_t lds_bytes) {
+  MC_CHECK(m, MC_ERR_ARG, "null map");
+  std::lock_guard<std::mutex> lk(m->mu);
+  if (batch) {
+    MC_CHECK(batch == 8 || batch == 16 || batch == 32 || batch == 64 || batch == 128,
+             MC_ERR_ARG, "batch %d not in {8,16,32,64,128}", batch);
+    m->batch = batch;
+  }
+  if (rows_per_group) {
+    MC_CHECK(rows_per_group >= 1 && rows_per_group <= 64, MC_ERR_ARG, "rows_per_group %d",
+             rows_per_group);
+    m->rows_per_group = rows_per_group;
+  }
+  if (lds_bytes) {
+    MC_CHECK(lds_bytes >= 8192 && lds_bytes <= 160 * 1024, MC_ERR_ARG, "lds_bytes %d",
+             lds_bytes);
+    m->lds_bytes = lds_bytes;
+  }
+  return MC_OK;
+}
+
+int mc_map_get_tuning(mc_map *m, int32_t *batch, int32_t *rows_per_group, int32_t *lds_bytes) {
+  MC_CHECK(m, MC_ERR_ARG, "null map");
+  if (batch) *batch = m->batch;
+  if (rows_per_group) *rows_per_group = m->rows_per_group;
+  if (lds_bytes) *lds_bytes = m->lds_bytes;
+  return MC_OK;
+}
+
+int mc_integrate(mc_map *m, const void *frames, int32_t dtype, int32_t n_frames,
+                 int32_t frames_on_device, int32_t mode, uint32_t flags, float *cakes,
+                 int32_t cakes_on_device, float *lineouts, int32_t lineouts_on_device) {
+  MC_CHECK(m && frames && cakes && n_frames >= 1, MC_ERR_ARG, "bad arguments");
+  MC_CHECK(mode >= 0 && mode <= 2, MC_ERR_ARG, "mode %d", mode);
+  const size_t es = dtype_size(dtype);
+  MC_CHECK(es > 0, MC_ERR_ARG, "unsupported frame dtype %d", dtype);
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  std::lock_guard<std::mutex> lk2(m->mu);
+  MC_HIP(hipSetDevice(m->ctx->device));
+  hipStream_t s = m->ctx->stream;
+  const size_t fbytes = (size_t)n_frames * m->npx() * es;
+  const size_t cbytes = (size_t)n_frames * m->nb() * 4;
+  const size_t lbytes = (size_t)n_frames * 3 * m->n_r * 4;
+  DBuf df, dc, dl;
+  int rc;
+  const void *fd = frames;
+  if (!frames_on_device) {
+    if ((rc = df.alloc(fbytes))) return rc;
+    MC_HIP(hipMemcpyAsync(df.p, frames, fbytes, hipMemcpyHostToDevice, s));
+    fd = df.p;
+  }
+  float *cd = cakes;
+  if (!cakes_on_device) {
+    if ((rc = dc.alloc(cbytes))) return rc;
+    cd = dc.as<float>();
+  }
+  float *ld = lineouts;
+  if (lineouts && !lineouts_on_device) {
+    if ((rc = dl.alloc(lbytes))) return rc;
+    ld = dl.as<float>();
+  }
+  if ((rc = run_integrate(m, fd, dtype, n_frames, mode, flags, cd, ld, false, false))) return rc;
+  if (!cakes_on_device) MC_HIP(hipMemcpyAsync(cakes, cd, cbytes, hipMemcpyDeviceToHost, s));
+  if (lineouts && !lineouts_on_device)
+    MC_HIP(hipMemcpyAsync(lineouts, ld, lbytes, hipMemcpyDeviceToHost, s));
+  MC_HIP(hipStreamSynchronize(s));
+  return MC_OK;
+}
+
+int mc_profile(mc_map *m, const float *cakes, int32_t n_frames, int32_t cakes_on_device,
+               float *lineouts, int32_t lineouts_on_device) {
+  MC_CHECK(m && cakes && lineouts && n_frames >= 1, MC_ERR_ARG, "bad arguments");
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  MC_HIP(hipSetDevice(m->ctx->device));
+  hipStream_t s = m->ctx->stream;
+  const size_t cbytes = (size_t)n_frames * m->nb() * 4, lbytes = (size_t)n_frames * 3 * m->n_r * 4;
+  DBuf dc, dl;
+  int rc;
+  const float *cd = cakes;
+  if (!cakes_on_device) {
+    if ((rc = dc.alloc(cbytes))) return rc;
+    MC_HIP(hipMemcpyAsync(dc.p, cakes, cbytes, hipMemcpyHostToDevice, s));
+    cd = dc.as<float>();
+  }
+  float *ld = lineouts;
+  if (!lineouts_on_device) {
+    if ((rc = dl.alloc(lbytes))) return rc;
+    ld = dl.as<float>();
+  }
+  // NaN bins are missing values: the profile uses the unflagged row sums
+  // when any NaN can be present, which is what the fused path does.
+  if ((rc = launch_profile(s, *m, cd, n_frames, 1, ld))) return rc;
+  if (!lineouts_on_device) MC_HIP(hipMemcpyAsync(lineouts, ld, lbytes, hipMemcpyDeviceToHost, s));
+  MC_HIP(hipStreamSynchronize(s));
+  return MC_OK;
+}
+
+int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype, int32_t n_frames,
+                      int32_t mode, uint32_t flags, int32_t kind, int32_t iters, int32_t warmup,
+                      float *cakes_dev, float *lineouts_dev, double *ms_per_launch,
+                      double *ms_per_iter) {
+  MC_CHECK(m && frames_dev && cakes_dev && n_frames >= 1 && iters >= 1, MC_ERR_ARG,
+           "bad arguments");
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  std::lock_guard<std::mutex> lk2(m->mu);
+  MC_HIP(hipSetDevice(m->ctx->device));
+  hipStream_t s = m->ctx->stream;
+  int rc;
+  if ((rc = ensure_binfo(m, mode, flags))) return rc;
+  Tiles *t = nullptr;
+  if (kind == 1 && (rc = ensure_tiles(m, mode, batched_elem_size(dtype), &t))) return rc;
+  int32_t *slots;
+  if ((rc = identity_slots(m->ctx, &slots))) return rc;
+  const size_t es = dtype_size(dtype);
+  auto one_iter = [&](std::vector<hipEvent_t> *ev) -> int {
+    const int step = kind == 1 ? t->batch : 1;
+    for (int f0 = 0; f0 < n_frames; f0 += step) {
+      const int nf = std::min(step, n_frames - f0);
+      const char *fb = (const char *)frames_dev + (size_t)f0 * m->npx() * es;
+      float *cb = cakes_dev + (size_t)f0 * m->nb();
+      float *lb = lineouts_dev ? lineouts_dev + (size_t)f0 * 3 * m->n_r : nullptr;
+      if (ev) MC_HIP(hipEventRecord((*ev)[2 * (f0 / step)], s));
+      int r2 = kind == 1 ? launch_batched(s, *m, *t, fb, dtype, nf, slots, cb, lb)
+                         : launch_plain(s, *m, m->csr[mode], fb, dtype, cb, lb, 1, m->npx());
+      if (r2) return r2;
+      if (ev) MC_HIP(hipEventRecord((*ev)[2 * (f0 / step) + 1], s));
+    }
+    return MC_OK;
+  };
+  for (int w = 0; w < warmup; w++)
+    if ((rc = one_iter(nullptr))) return rc;
+  MC_HIP(hipStreamSynchronize(s));
+  const int step = kind == 1 ? t->batch : 1;
+  const int launches = (n_frames + step - 1) / step;
+  std::vector<hipEvent_t> ev(2 * (size_t)launches * iters);
+  for (auto &e : ev) MC_HIP(hipEventCreate(&e));
+  hipEvent_t e0, e1;
+  MC_HIP(hipEventCreate(&e0));
+  MC_HIP(hipEventCreate(&e1));
+  MC_HIP(hipEventRecord(e0, s));
+  for (int it = 0; it < iters; it++) {
+    std::vector<hipEvent_t> sub(ev.begin() + 2 * (size_t)launches * it,
+                                ev.begin() + 2 * (size_t)launches * (it + 1));
+    if ((rc = one_iter(&sub))) return rc;
+  }
+  MC_HIP(hipEventRecord(e1, s));
+  MC_HIP(hipEventSynchronize(e1));
+  double kern_ms = 0;
+  for (size_t l = 0; l < ev.size() / 2; l++) {
+    float ms = 0;
+    MC_HIP(hipEventElapsedTime(&ms, ev[2 * l], ev[2 * l + 1]));
+    kern_ms += ms;
+  }
+  float tot = 0;
+  MC_HIP(hipEventElapsedTime(&tot, e0, e1));
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms_per_launch) *ms_per_launch = kern_ms / ((double)iters * launches);
+  if (ms_per_iter) *ms_per_iter = tot / iters;
+  return MC_OK;
+}
+
+}  // extern "C"

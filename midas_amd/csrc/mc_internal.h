@@ -1,0 +1,188 @@
+// mc_internal.h -- shared types of the MI355X caking library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/midas_caking.h"
+
+namespace mc {
+
+void set_error(const char *fmt, ...);
+
+#define MC_HIP(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      ::mc::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,            \
+                      hipGetErrorString(e_));                               \
+      return MC_ERR_HIP;                                                    \
+    }                                                                       \
+  } while (0)
+
+#define MC_CHECK(cond, code, ...)                                           \
+  do {                                                                      \
+    if (!(cond)) {                                                          \
+      ::mc::set_error(__VA_ARGS__);                                         \
+      return (code);                                                        \
+    }                                                                       \
+  } while (0)
+
+// One Map.bin record, MapperCore.h:19-25 (24 bytes, 4-byte packed so the
+// device copy has no padding either).
+#pragma pack(push, 4)
+struct Rec {
+  float y, z;
+  double frac;
+  float deltaR, areaWeight;
+};
+#pragma pack(pop)
+static_assert(sizeof(Rec) == 24, "Map.bin record must be 24 bytes");
+
+// Device buffer with RAII free.
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int alloc(size_t n) {
+    reset();
+    if (n == 0) n = 16;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      set_error("hipMalloc(%zu): %s", n, hipGetErrorString(e));
+      p = nullptr;
+      return MC_ERR_NOMEM;
+    }
+    bytes = n;
+    return MC_OK;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::recursive_mutex mu;  // one submitter on the stream at a time
+};
+
+// Bin-major CSR of one integration mode: the one-frame kernel's layout.
+struct Csr {
+  bool built = false;
+  double bc_y = 0, bc_z = 0;  // beam centre it was built for (gradient)
+  int64_t nnz = 0;
+  DBuf row;   // int32 [nb+1]
+  DBuf pix;   // int32 [nnz]
+  DBuf w;     // float [nnz]
+  std::vector<int32_t> h_row, h_pix;  // host copies for the tile builder
+  std::vector<float> h_w;
+  DBuf dterm;                         // float [nb], dark term of this CSR
+  bool dterm_valid = false;
+};
+
+// Row-block/tile layout of the batched kernel for one (mode, batch, rows,
+// LDS budget).
+struct Tiles {
+  bool built = false;
+  int batch = 0, rows = 0, lds_bytes = 0, pitch = 0;  // pitch = LDS frames stride
+  int esize = 0;  // bytes per staged LDS element (2: u16, 4: f32)
+  int n_groups = 0;
+  size_t lds_needed = 0;
+  DBuf group_tile;  // int32 [n_groups+1] tile ranges, in launch order
+  DBuf group_row0;  // int32 [n_groups]
+  DBuf tiles;       // int32 x6 per tile: y0 z0 bw bh bin_begin bin_end
+  DBuf tbins;       // int32 x4 per tile bin: cake bin, ent_begin, ent_end, row|direct
+  DBuf ents;        // uint32 x2: index (LDS or global), weight bits
+  int64_t n_tiles = 0, n_tbins = 0, n_ents = 0, n_direct = 0;
+};
+
+struct Map {
+  Ctx *ctx = nullptr;
+  std::mutex mu;
+  int ny = 0, nz = 0, n_r = 0, n_eta = 0;
+  int64_t nnz = 0;
+  int q_mode = 0;
+  int64_t skipped[3] = {0, 0, 0};
+  std::vector<Rec> h_rec;           // canonical records (bin-major)
+  std::vector<int32_t> h_pairs;     // nMap pairs
+  std::vector<int32_t> h_flags;     // maskMap flags
+  std::vector<double> h_area_all;   // sum areaWeight per bin (all entries)
+  std::vector<double> h_area_inb;   // in-bounds entries only (.cu:817-899)
+  DBuf d_rec, d_pairs;
+  DBuf d_flags;                     // int32 [nb]
+  DBuf d_area_inb;                  // float [nb]
+  DBuf d_row_static;                // float4 [n_r]: sum A_inb, n over valid bins;
+                                    // same over valid unflagged bins
+  DBuf inv_area;                    // float [nb]: 1 / sum areaWeight (0 if empty)
+  double bc_y = 0, bc_z = 0;
+  // dark
+  bool has_dark = false;
+  DBuf d_dark;                      // double [npx]
+  // per mode layouts
+  Csr csr[3];
+  std::vector<std::unique_ptr<Tiles>> tiles[3];
+  // per-bin float2 {signed scale, dark term} for (mode, flags)
+  DBuf d_binfo;
+  int binfo_mode = -1;
+  uint32_t binfo_flags = 0xffffffffu;
+  bool binfo_valid = false;
+  // tuning
+  int batch = 64, rows_per_group = 8, lds_bytes = 65536;
+  int64_t nb() const { return (int64_t)n_r * n_eta; }
+  int64_t npx() const { return (int64_t)ny * nz; }
+};
+
+size_t dtype_size(int dtype);
+
+// ---- kernel launchers (mc_kernels.hip) ----
+int launch_area_fill(hipStream_t s, float *p, int64_t n, float v);
+int launch_dark_mean(hipStream_t s, const void *frames, int dtype,
+                     int n_frames, int64_t npx, double scale, double *out);
+int launch_dark_term(hipStream_t s, const int32_t *row, const int32_t *pix,
+                     const float *w, const double *dark, int64_t nb,
+                     float *dterm);
+int launch_binfo(hipStream_t s, const float *inv_area, const int32_t *flags,
+                 const float *dterm, int64_t nb, int normalize, int mask_nan,
+                 float2 *binfo);
+int launch_plain(hipStream_t s, const Map &m, const Csr &c, const void *frame,
+                 int dtype, float *cake, float *lineouts, int n_frames,
+                 int64_t frame_stride);
+int launch_batched(hipStream_t s, const Map &m, const Tiles &t,
+                   const void *frames, int dtype, int n_frames,
+                   const int32_t *slots_dev, float *cakes, float *lineouts);
+int launch_profile(hipStream_t s, const Map &m, const float *cakes, int n_frames,
+                   int mask_nan, float *lineouts);
+int batched_lds_pitch(int batch, int dtype);
+int batched_elem_size(int dtype);
+
+// ---- mapper (mc_mapper.hip) ----
+int map_build_gpu(Ctx *ctx, const mc_geometry *g, const double *mask,
+                  const double *distY, const double *distZ,
+                  const double *residual, const mc_panel *panels,
+                  int n_panels, Map *m);
+
+// ---- host helpers (mc_core.cpp) ----
+int map_finish_host(Map *m);  // derived arrays + upload
+int ensure_csr(Map *m, int mode);
+int ensure_tiles(Map *m, int mode, int esize, Tiles **out);
+int run_integrate(Map *m, const void *frames_dev, int dtype, int n_frames, int mode,
+                  uint32_t flags, float *cakes_dev, float *lineouts_dev,
+                  bool force_plain, bool force_batched);
+int ensure_binfo(Map *m, int mode, uint32_t flags);
+
+}  // namespace mc
+
+struct mc_ctx : mc::Ctx {};
+struct mc_map : mc::Map {};

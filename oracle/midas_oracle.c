@@ -905,7 +905,9 @@ void orc_lineouts(const double *cake, const double *ain, int nR, int nEta,
       double v = cake[(long long)r * nEta + e];
       if (v != 0 && !isnan(v)) { s += v; c++; }
       double a = ain ? ain[(long long)r * nEta + e] : 0;
-      if (a > 1e-9) { sia += v * a; sa += a; sm += v; cm++; }
+      /* NaN (mask) bins are missing values in every definition; the
+       * reference GPU path never produces them. */
+      if (a > 1e-9 && !isnan(v)) { sia += v * a; sa += a; sm += v; cm++; }
     }
     if (lo_cpu) lo_cpu[r] = c > 0 ? s / c : 0;
     if (lo_area) lo_area[r] = sa > 1e-9 ? sia / sa : 0;

@@ -1,0 +1,171 @@
+"""HIP integration kernels (one-frame and batched) vs the FP64 oracle.
+
+Tolerance: |gpu - oracle| <= 1e-5 * S_b per bin, with S_b the oracle's cake
+of |frame - dark| (the sum of absolute terms, normalised like the bin); for
+the non-negative frames used here S_b = |oracle|, i.e. 1e-5 relative.
+NaN (masked) bins must coincide exactly.
+"""
+import numpy as np
+import pytest
+
+import midas_amd as ma
+from midas_amd import synth
+from helpers import panel_shifts_for, rel_err, small_cases, to_oracle
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _setup(name, n_frames=3, seed=1, dtype=np.uint16):
+    P = small_cases()[name]
+    rng = np.random.default_rng(seed)
+    mask = (rng.random((P.NrPixelsZ, P.NrPixelsY)) < 0.01).astype(np.float64)
+    frames = rng.poisson(200.0, (n_frames, P.NrPixelsZ, P.NrPixelsY)).astype(dtype)
+    dark = rng.poisson(20.0, (2, P.NrPixelsZ, P.NrPixelsY)).astype(dtype)
+    return P, mask, frames, dark
+
+
+def _oracle(orc, om, P, frame, dark_mean, mode, normalize=True):
+    img = frame.astype(np.float64) - dark_mean
+    grad = mode == "gradient"
+    cake, _ = orc.integrate(om, img, P.NrPixelsY, P.NrPixelsZ, bc=(P.BC_y, P.BC_z),
+                            gradient=grad, normalize=normalize, semantics=0)
+    scale, _ = orc.integrate(om, np.abs(img), P.NrPixelsY, P.NrPixelsZ, bc=(P.BC_y, P.BC_z),
+                             gradient=grad, normalize=normalize, semantics=0)
+    return cake, scale
+
+
+@pytest.mark.parametrize("name", ["flat", "tilt", "subpix", "qmode", "panels"])
+@pytest.mark.parametrize("mode", ["bilinear", "gradient"])
+def test_one_frame_and_batched_match_oracle(name, mode, oracle_lib):
+    P, mask, frames, dark = _setup(name)
+    print(f"[integrate] case={name} mode={mode}", flush=True)
+    sh = panel_shifts_for(name)
+    g = ma.build_map(P, mask=mask, panel_shifts=sh)
+    g.set_dark(dark)
+    om = oracle_lib.build_map(to_oracle(P), mask=mask, panel_shifts=sh)
+    dmean = dark.astype(np.float64).mean(axis=0)
+    for F in (1, len(frames)):
+        cakes, lo = ma.integrate_batch(frames[:F], g, mode=mode, mask_nan=True)
+        for i in range(F):
+            ref, scale = _oracle(oracle_lib, om, P, frames[i], dmean, mode)
+            e = rel_err(cakes[i], ref, scale)
+            assert e.max() <= TOL, f"F={F} frame {i}: max rel err {e.max():.3g}"
+            ainb = oracle_lib.area_inbounds(om, P.NrPixelsY, P.NrPixelsZ)
+            lc, la, lm = oracle_lib.lineouts(ref, ainb)
+            for row, want in ((0, lc), (1, la), (2, lm)):
+                np.testing.assert_allclose(lo[i, row], want, rtol=2e-5, atol=1e-3,
+                                           err_msg=f"lineout row {row}")
+
+
+def test_floor_mode_matches_gpu_stream_semantics(oracle_lib):
+    """floor: integer pixel via the float offset of .cu:916, no NaN."""
+    P, mask, frames, dark = _setup("subpix")
+    g = ma.build_map(P, mask=mask)
+    g.set_dark(dark)
+    om = oracle_lib.build_map(to_oracle(P), mask=mask)
+    dmean = dark.astype(np.float64).mean(axis=0)
+    cakes, _ = ma.integrate_batch(frames, g, mode="floor", mask_nan=False)
+    for i in range(len(frames)):
+        img = frames[i].astype(np.float64) - dmean
+        ref, _ = oracle_lib.integrate(om, img, P.NrPixelsY, P.NrPixelsZ, semantics=1)
+        sc, _ = oracle_lib.integrate(om, np.abs(img), P.NrPixelsY, P.NrPixelsZ, semantics=1)
+        assert rel_err(cakes[i], ref, sc).max() <= TOL
+
+
+@pytest.mark.parametrize("batch", [8, 16, 32, 64, 128])
+@pytest.mark.parametrize("rows", [1, 4, 8])
+def test_batched_tilings_agree(batch, rows):
+    """Every batch size and row-block height gives the one-frame result,
+    including a partial last pass (F not a multiple of the batch)."""
+    P, mask, frames, dark = _setup("tilt", n_frames=batch + 3, seed=5)
+    g = ma.build_map(P, mask=mask)
+    g.set_dark(dark)
+    g.set_tuning(batch=batch, rows_per_group=rows, lds_bytes=32768 if batch <= 64 else 65536)
+    cb, lb = ma.integrate_batch(frames, g, mode="bilinear", mask_nan=True)
+    for i in (0, batch - 1, batch, len(frames) - 1):
+        c1, l1 = ma.integrate_batch(frames[i], g, mode="bilinear", mask_nan=True)
+        np.testing.assert_allclose(cb[i], c1[0], rtol=3e-6, atol=1e-4, equal_nan=True)
+        np.testing.assert_allclose(lb[i], l1[0], rtol=3e-6, atol=1e-4)
+
+
+def test_small_lds_budget_uses_direct_bins():
+    """A budget below a bin's footprint routes those bins through direct
+    global gathers; results still agree."""
+    P, mask, frames, dark = _setup("flat", n_frames=9, seed=11)
+    g = ma.build_map(P, mask=mask)
+    g.set_tuning(batch=8, rows_per_group=16, lds_bytes=8192)
+    cb, _ = ma.integrate_batch(frames, g, mode="bilinear")
+    g.set_tuning(batch=8, rows_per_group=2, lds_bytes=65536)
+    cr, _ = ma.integrate_batch(frames, g, mode="bilinear")
+    np.testing.assert_allclose(cb, cr, rtol=3e-6, atol=1e-4, equal_nan=True)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.uint32, np.int32, np.int64,
+                                   np.float32, np.float64])
+def test_frame_dtypes(dtype, oracle_lib):
+    P, mask, frames, dark = _setup("centred", n_frames=2, seed=2, dtype=dtype)
+    g = ma.build_map(P, mask=mask)
+    g.set_dark(dark)
+    om = oracle_lib.build_map(to_oracle(P), mask=mask)
+    dmean = dark.astype(np.float64).mean(axis=0)
+    cakes, _ = ma.integrate_batch(frames, g, mode="bilinear", mask_nan=True)
+    for i in range(2):
+        ref, sc = _oracle(oracle_lib, om, P, frames[i], dmean, "bilinear")
+        assert rel_err(cakes[i], ref, sc).max() <= TOL
+
+
+def test_normalize_off_and_no_dark(oracle_lib):
+    P, mask, frames, _ = _setup("flat", n_frames=2)
+    g = ma.build_map(P, mask=mask)
+    om = oracle_lib.build_map(to_oracle(P), mask=mask)
+    cakes, _ = ma.integrate_batch(frames, g, mode="bilinear", normalize=False, mask_nan=False)
+    for i in range(2):
+        img = frames[i].astype(np.float64)
+        ref, _ = oracle_lib.integrate(om, img, P.NrPixelsY, P.NrPixelsZ, normalize=False)
+        ref[np.isnan(ref)] = 0  # oracle CPU semantics NaN-masks; compare values
+        flags = om.flags.reshape(ref.shape).astype(bool)
+        refv, _ = oracle_lib.integrate(om, img, P.NrPixelsY, P.NrPixelsZ, normalize=False,
+                                       semantics=1)
+        e = np.abs(cakes[i] - refv) / np.maximum(np.abs(refv), 1e-30)
+        assert e.max() <= TOL
+        assert not np.isnan(cakes[i][flags]).any()
+
+
+def test_plain_1024_workload(oracle_lib):
+    """The 1024^2 uint16 workload with hot-pixel mask, against FP64."""
+    P = synth.plain_1024()
+    frames, dark, mask = synth.frames(P, 2, seed=0)
+    print("[integrate] plain_1024", flush=True)
+    g = ma.build_map(P, mask=mask)
+    g.set_dark(dark)
+    om = oracle_lib.build_map(to_oracle(P), mask=mask)
+    cakes, lo = ma.integrate_batch(frames, g, mode="bilinear", mask_nan=True)
+    c1 = ma.integrate(frames[0], g, mode="bilinear", mask_nan=True)
+    for i in range(2):
+        ref, sc = _oracle(oracle_lib, om, P, frames[i], dark.astype(np.float64), "bilinear")
+        e = rel_err(cakes[i], ref, sc)
+        assert e.max() <= TOL, e.max()
+    assert rel_err(c1, _oracle(oracle_lib, om, P, frames[0], dark.astype(np.float64),
+                               "bilinear")[0], np.abs(c1) + 1e-30).max() <= TOL
+
+
+def test_profile_1d_matches_fused(oracle_lib):
+    P, mask, frames, dark = _setup("tilt", n_frames=4)
+    g = ma.build_map(P, mask=mask)
+    g.set_dark(dark)
+    cakes, lo = ma.integrate_batch(frames, g, mode="bilinear", mask_nan=True)
+    for mode, row in (("cpu_mean", 0), ("area_weighted", 1), ("simple_mean", 2)):
+        p = ma.profile_1d(cakes, g, mode=mode)
+        np.testing.assert_allclose(p, lo[:, row], rtol=2e-5, atol=1e-4)
+
+
+def test_torch_device_tensors(oracle_lib):
+    torch = pytest.importorskip("torch")
+    P, mask, frames, dark = _setup("flat", n_frames=5)
+    g = ma.build_map(P, mask=mask)
+    g.set_dark(torch.from_numpy(dark).cuda())
+    dc, dl = ma.integrate_batch(torch.from_numpy(frames).cuda(), g, mode="bilinear")
+    hc, hl = ma.integrate_batch(frames, g, mode="bilinear")
+    np.testing.assert_array_equal(dc.cpu().numpy(), hc)
+    np.testing.assert_array_equal(dl.cpu().numpy(), hl)
