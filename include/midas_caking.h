@@ -226,14 +226,16 @@ int mc_fit_windows(mc_ctx *ctx, int32_t model, int32_t n_windows,
                    int32_t n_points, const float *x, const float *y,
                    const float *peak_guess, double rstep, int32_t max_iter,
                    double *out, int32_t on_device);
-/* Cuts windows out of cakes and fits them in one launch: for every frame,
- * every eta bin and every ring centre (in R bin units), the ROI of
- * 2*pad+1 R bins around the ring (IntegratorZarrOMP.c:1966-1988).
- * out is [n_frames][n_eta][n_rings][MC_FIT_OUT]. */
+/* Cuts windows out of device-resident cakes and fits them in one launch:
+ * for every frame, every eta bin and every ring radius ring_r[k] (same
+ * units as r_centers, the n_r R-bin centres), the ROI of R bins whose
+ * centre lies within pad bin widths of the ring (IntegratorZarrOMP.c
+ * :1966-1988); eta profiles with fewer than 5 nonzero bins are skipped
+ * (:1943-1951, status -3).  out is [n_frames][n_eta][n_rings][MC_FIT_OUT]. */
 int mc_fit_cake_rings(mc_map *m, const float *cakes_dev, int32_t n_frames,
-                      const double *ring_r_bins, int32_t n_rings, int32_t pad,
-                      int32_t model, int32_t max_iter, double *out,
-                      int32_t out_on_device);
+                      const double *r_centers, const double *ring_r,
+                      int32_t n_rings, int32_t pad, int32_t model,
+                      int32_t max_iter, double *out, int32_t out_on_device);
 
 /* ---- continuous-batching frame server ---- */
 typedef struct mc_server mc_server;

@@ -81,8 +81,7 @@ int map_finish_host(Map *m) {
   MC_HIP(hipMemcpy(m->d_flags.p, m->h_flags.data(), nb * 4, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(m->d_area_inb.p, ainb.data(), nb * 4, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(m->d_row_static.p, rs.data(), m->n_r * sizeof(float4), hipMemcpyHostToDevice));
-  if ((rc = m->d_binfo.alloc(nb * sizeof(float2)))) return rc;
-  m->binfo_valid = false;
+  for (bool &ok : m->binfo_ok) ok = false;
   if ((rc = m->inv_area.alloc(nb * 4))) return rc;
   MC_HIP(hipMemcpy(m->inv_area.p, inv.data(), nb * 4, hipMemcpyHostToDevice));
   return MC_OK;
@@ -210,7 +209,7 @@ int ensure_csr(Map *m, int mode) {
   c.built = true;
   c.dterm_valid = false;
   m->tiles[mode].clear();
-  if (m->binfo_mode == mode) m->binfo_valid = false;
+  for (int f = 0; f < 4; f++) m->binfo_ok[mode * 4 + f] = false;
   return MC_OK;
 }
 
@@ -238,194 +237,252 @@ struct BBox {
 };
 }  // namespace
 
-int ensure_tiles(Map *m, int mode, int esize, Tiles **out) {
+// LDS of the batched kernel: bin records, entries, lineout sums, planes.
+constexpr int kTileBins = 128, kTileEnts = 1024, kBinRecBytes = 32;
+
+int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   int rc = ensure_csr(m, mode);
   if (rc) return rc;
   const int B = m->batch, H = m->rows_per_group;
   for (auto &t : m->tiles[mode])
-    if (t->batch == B && t->rows == H && t->lds_bytes == m->lds_bytes && t->esize == esize) {
+    if (t->batch == B && t->rows == H && t->lds_bytes == m->lds_bytes && t->in_size == in_size) {
       *out = t.get();
       return MC_OK;
     }
   const Csr &c = m->csr[mode];
   const int NY = m->ny, nE = m->n_eta, nR = m->n_r;
   const int64_t nb = m->nb();
-  const int pitch = batched_lds_pitch(B, esize == 2 ? MC_DTYPE_U16 : MC_DTYPE_F32);
-  const int64_t lo_bytes = (int64_t)H * 4 * B * 4;
-  const int64_t cap_px = (m->lds_bytes - lo_bytes) / ((int64_t)pitch * esize);
-  MC_CHECK(cap_px >= 16, MC_ERR_ARG, "LDS budget %d too small for batch %d", m->lds_bytes, B);
-  // Per-bin pixel bounding boxes.
+  const int tl = in_size <= 2 ? 2 : 4;  // LDS element size
+  // 8-byte staging loads when every frame row starts 8-byte aligned.
+  const int V = (in_size < 8 && ((int64_t)NY * in_size) % 8 == 0 && (m->npx() * in_size) % 8 == 0)
+                    ? 8 / in_size : 1;
+  const int64_t fixed = (int64_t)kTileBins * kBinRecBytes + (int64_t)kTileEnts * 8 +
+                        (int64_t)H * 4 * B * 4;
+  const int64_t plane_cap = m->lds_bytes - fixed;  // bytes for B frame planes
+  MC_CHECK(plane_cap >= (int64_t)B * tl * 16, MC_ERR_ARG,
+           "LDS budget %d too small for batch %d", m->lds_bytes, B);
+  // Plane geometry of a pixel box: aligned start, power-of-two vector count.
+  struct Box {
+    int y0, z0, bwp, bh, ks, S;
+    int64_t bytes;
+  };
+  auto box_of = [&](const BBox &bx, Box &o) -> bool {
+    if (bx.empty()) {
+      o = {0, 0, 0, 0, 0, 0, 0};
+      return true;
+    }
+    int ya = (bx.y0 / V) * V;
+    int nv = 1, ks = 0;
+    while (nv * V < bx.y1 - ya + 1) {
+      nv <<= 1;
+      ks++;
+    }
+    if (nv > 64) return false;
+    const int bwp = nv * V;
+    if (ya + bwp > NY) ya = NY - bwp;
+    if (ya < 0) return false;
+    const int bh = bx.z1 - bx.z0 + 1;
+    int64_t S = (int64_t)bwp * bh;
+    if (tl == 2) {
+      while (S % 4 != 2) S++;
+    } else if (S % 2 == 0) {
+      S++;
+    }
+    o = {ya, bx.z0, bwp, bh, ks, (int)S, (int64_t)B * S * tl};
+    return o.bytes <= plane_cap;
+  };
+  // Per-bin pixel boxes.
   std::vector<BBox> bb(nb);
   parallel_for(nb, [&](int64_t a, int64_t b) {
     for (int64_t q = a; q < b; q++)
       for (int32_t k = c.h_row[q]; k < c.h_row[q + 1]; k++)
         bb[q].add(c.h_pix[k] % NY, c.h_pix[k] / NY);
   });
+  auto cnt = [&](int64_t b) { return (int64_t)(c.h_row[b + 1] - c.h_row[b]); };
   struct TileH {
     BBox box;
+    int64_t ents = 0;
     std::vector<int64_t> bins;
     std::vector<char> direct;
   };
-  struct GroupH {
-    int row0;
-    int64_t ents = 0;
-    std::vector<TileH> tiles;
-  };
-  const int n_groups = (nR + H - 1) / H;
-  std::vector<GroupH> groups(n_groups);
-  parallel_for(n_groups, [&](int64_t a, int64_t bq) {
+  const int n_blocks = (nR + H - 1) / H;
+  std::vector<std::vector<TileH>> blocks(n_blocks);
+  parallel_for(n_blocks, [&](int64_t a, int64_t bq) {
     for (int64_t gi = a; gi < bq; gi++) {
-      GroupH &G = groups[gi];
-      G.row0 = (int)gi * H;
-      const int r1 = std::min(nR, G.row0 + H);
+      auto &tl_ = blocks[gi];
+      const int row0 = (int)gi * H, r1 = std::min(nR, row0 + H);
       TileH cur;
+      Box tmp;
       auto flush = [&]() {
-        if (!cur.bins.empty()) G.tiles.push_back(std::move(cur));
+        if (!cur.bins.empty()) tl_.push_back(std::move(cur));
         cur = TileH();
       };
-      auto fits = [&](const BBox &b) { return b.area() <= cap_px; };
+      auto room = [&](const TileH &T, const BBox &add, int64_t ents, int nbins) {
+        BBox u = T.box;
+        u.add(add);
+        return (int)T.bins.size() + nbins <= kTileBins && T.ents + ents <= kTileEnts &&
+               box_of(u, tmp);
+      };
       for (int e = 0; e < nE; e++) {
         BBox col;
-        for (int r = G.row0; r < r1; r++) col.add(bb[(int64_t)r * nE + e]);
-        BBox uni = cur.box;
-        uni.add(col);
-        if (fits(uni)) {
-          cur.box = uni;
-          for (int r = G.row0; r < r1; r++) {
+        int64_t ce = 0;
+        for (int r = row0; r < r1; r++) {
+          col.add(bb[(int64_t)r * nE + e]);
+          ce += cnt((int64_t)r * nE + e);
+        }
+        const int nbins = r1 - row0;
+        if (!room(cur, col, ce, nbins)) flush();
+        if (room(cur, col, ce, nbins)) {
+          cur.box.add(col);
+          cur.ents += ce;
+          for (int r = row0; r < r1; r++) {
             cur.bins.push_back((int64_t)r * nE + e);
             cur.direct.push_back(0);
           }
           continue;
         }
-        flush();
-        if (fits(col)) {
-          cur.box = col;
-          for (int r = G.row0; r < r1; r++) {
-            cur.bins.push_back((int64_t)r * nE + e);
-            cur.direct.push_back(0);
-          }
-          continue;
-        }
-        // Column too large: bin by bin.
-        for (int r = G.row0; r < r1; r++) {
+        // The column alone does not fit: bin by bin, direct if needed.
+        for (int r = row0; r < r1; r++) {
           const int64_t b = (int64_t)r * nE + e;
-          if (!fits(bb[b])) {
+          TileH solo;
+          if (!room(solo, bb[b], cnt(b), 1)) {
+            if ((int)cur.bins.size() + 1 > kTileBins) flush();
             cur.bins.push_back(b);
             cur.direct.push_back(1);
             continue;
           }
-          BBox u2 = cur.box;
-          u2.add(bb[b]);
-          if (!fits(u2)) {
-            flush();
-            u2 = bb[b];
-          }
-          cur.box = u2;
+          if (!room(cur, bb[b], cnt(b), 1)) flush();
+          cur.box.add(bb[b]);
+          cur.ents += cnt(b);
           cur.bins.push_back(b);
           cur.direct.push_back(0);
         }
       }
       flush();
-      for (auto &T : G.tiles)
-        for (int64_t b : T.bins) G.ents += c.h_row[b + 1] - c.h_row[b];
     }
   });
-  // Launch order: heaviest row blocks first.
-  std::vector<int> order(n_groups);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return groups[a].ents > groups[b].ents; });
-  std::vector<int32_t> g_tile(n_groups + 1, 0), g_row0(n_groups);
-  std::vector<int32_t> tiles;
+  // Chunks: runs of consecutive tiles of one row block with about
+  // target entries each (thousands of workgroups for a 2048^2 map).
+  const int64_t target = std::max<int64_t>(1024, std::min<int64_t>(8192, c.nnz / 4096));
+  std::vector<int32_t> chunks, block_chunk(n_blocks + 1, 0), tiles;
   std::vector<int4> tbins;
   std::vector<uint2> ents;
-  int64_t n_direct = 0;
-  int64_t max_tile_px = 0;
-  for (int oi = 0; oi < n_groups; oi++) {
-    GroupH &G = groups[order[oi]];
-    g_row0[oi] = G.row0;
-    for (auto &T : G.tiles) {
-      const int y0 = T.box.empty() ? 0 : T.box.y0, z0 = T.box.empty() ? 0 : T.box.z0;
-      const int bw = T.box.empty() ? 0 : T.box.y1 - T.box.y0 + 1;
-      const int bh = T.box.empty() ? 0 : T.box.z1 - T.box.z0 + 1;
-      max_tile_px = std::max<int64_t>(max_tile_px, (int64_t)bw * bh);
-      const int32_t b_beg = (int32_t)tbins.size();
-      for (size_t q = 0; q < T.bins.size(); q++) {
-        const int64_t b = T.bins[q];
-        const bool dir = T.direct[q];
-        const int32_t e_beg = (int32_t)ents.size();
+  int64_t n_direct = 0, max_plane = 0, staged = 0;
+  for (int gi = 0; gi < n_blocks; gi++) {
+    const int row0 = gi * H;
+    int64_t acc = target;  // forces a new chunk at the block start
+    for (auto &T : blocks[gi]) {
+      if (acc >= target) {
+        chunks.insert(chunks.end(), {(int32_t)(tiles.size() / 10), (int32_t)(tiles.size() / 10),
+                                     row0, std::min(H, nR - row0)});
+        acc = 0;
+      }
+      Box bx;
+      box_of(T.box, bx);
+      max_plane = std::max(max_plane, bx.bytes);
+      staged += (int64_t)bx.bwp * bx.bh;
+      const int32_t b_beg = (int32_t)tbins.size(), e_beg = (int32_t)ents.size();
+      auto push_entries = [&](int64_t b, bool dir) {
         for (int32_t k = c.h_row[b]; k < c.h_row[b + 1]; k++) {
           const int p = c.h_pix[k];
-          uint32_t idx = dir ? (uint32_t)p : (uint32_t)((p / NY - z0) * bw + (p % NY - y0));
+          const uint32_t idx =
+              dir ? (uint32_t)p : (uint32_t)((p / NY - bx.z0) * bx.bwp + (p % NY - bx.y0));
           uint32_t wb;
           memcpy(&wb, &c.h_w[k], 4);
           ents.push_back(make_uint2(idx, wb));
         }
-        const int rib = (int)(b / nE) - G.row0;
-        tbins.push_back(make_int4((int32_t)b, e_beg, (int32_t)ents.size(),
-                                  rib | (dir ? (1 << 30) : 0)));
-        n_direct += dir;
-      }
-      tiles.insert(tiles.end(), {y0, z0, bw, bh, b_beg, (int32_t)tbins.size()});
+      };
+      // Staged entries first (contiguous), then those of direct bins.
+      std::vector<int4> recs(T.bins.size());
+      for (size_t q = 0; q < T.bins.size(); q++)
+        if (!T.direct[q]) {
+          const int64_t b = T.bins[q];
+          const int32_t eb = (int32_t)ents.size();
+          push_entries(b, false);
+          recs[q] = make_int4((int32_t)b, eb, (int32_t)ents.size(), (int)(b / nE) - row0);
+        }
+      const int32_t e_end = (int32_t)ents.size();
+      for (size_t q = 0; q < T.bins.size(); q++)
+        if (T.direct[q]) {
+          const int64_t b = T.bins[q];
+          const int32_t eb = (int32_t)ents.size();
+          push_entries(b, true);
+          recs[q] = make_int4((int32_t)b, eb, (int32_t)ents.size(),
+                              ((int)(b / nE) - row0) | (1 << 30));
+          n_direct++;
+        }
+      tbins.insert(tbins.end(), recs.begin(), recs.end());
+      tiles.insert(tiles.end(), {bx.y0, bx.z0, bx.bwp, bx.bh, bx.ks, bx.S, b_beg,
+                                 (int32_t)tbins.size(), e_beg, e_end});
+      chunks[chunks.size() - 3] = (int32_t)(tiles.size() / 10);
+      acc += T.ents;
     }
-    g_tile[oi + 1] = (int32_t)(tiles.size() / 6);
+    block_chunk[gi + 1] = (int32_t)(chunks.size() / 4);
   }
   auto t = std::make_unique<Tiles>();
   t->batch = B;
   t->rows = H;
   t->lds_bytes = m->lds_bytes;
-  t->esize = esize;
-  t->pitch = pitch;
-  t->n_groups = n_groups;
-  t->lds_needed = (size_t)lo_bytes + (size_t)max_tile_px * pitch * esize;
-  t->lds_needed = (t->lds_needed + 15) & ~(size_t)15;
-  t->n_tiles = (int64_t)tiles.size() / 6;
+  t->in_size = in_size;
+  t->vec = V;
+  t->n_chunks = (int)(chunks.size() / 4);
+  t->n_blocks = n_blocks;
+  t->lds_needed = ((size_t)fixed + (size_t)max_plane + 15) & ~(size_t)15;
+  t->n_tiles = (int64_t)tiles.size() / 10;
   t->n_tbins = (int64_t)tbins.size();
   t->n_ents = (int64_t)ents.size();
   t->n_direct = n_direct;
+  t->staged_px = staged;
   MC_CHECK(t->n_tbins == nb, MC_ERR_STATE, "tile planner covered %lld of %lld bins",
            (long long)t->n_tbins, (long long)nb);
-  if ((rc = t->group_tile.alloc(g_tile.size() * 4)) || (rc = t->group_row0.alloc(g_row0.size() * 4)) ||
-      (rc = t->tiles.alloc(tiles.size() * 4)) || (rc = t->tbins.alloc(tbins.size() * 16)) ||
-      (rc = t->ents.alloc(ents.size() * 8)))
+  if ((rc = t->chunks.alloc(chunks.size() * 4)) || (rc = t->tiles.alloc(tiles.size() * 4)) ||
+      (rc = t->tbins.alloc(tbins.size() * 16)) || (rc = t->ents.alloc(ents.size() * 8)) ||
+      (rc = t->block_chunk.alloc(block_chunk.size() * 4)) ||
+      (rc = t->part.alloc((size_t)t->n_chunks * H * 4 * B * 4)))
     return rc;
-  MC_HIP(hipMemcpy(t->group_tile.p, g_tile.data(), g_tile.size() * 4, hipMemcpyHostToDevice));
-  MC_HIP(hipMemcpy(t->group_row0.p, g_row0.data(), g_row0.size() * 4, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(t->chunks.p, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->tiles.p, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->tbins.p, tbins.data(), tbins.size() * 16, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->ents.p, ents.data(), ents.size() * 8, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(t->block_chunk.p, block_chunk.data(), block_chunk.size() * 4,
+                   hipMemcpyHostToDevice));
   t->built = true;
   *out = t.get();
   m->tiles[mode].push_back(std::move(t));
   return MC_OK;
 }
 
-int ensure_binfo(Map *m, int mode, uint32_t flags) {
+int ensure_binfo(Map *m, int mode, uint32_t flags, const float2 **out) {
+  MC_CHECK(flags < 4, MC_ERR_ARG, "unknown integration flags 0x%x", flags);
   int rc = ensure_csr(m, mode);
   if (rc) return rc;
   Csr &c = m->csr[mode];
   hipStream_t s = m->ctx->stream;
+  const int key = mode * 4 + (int)flags;
   if (!c.dterm_valid) {
     if ((rc = launch_dark_term(s, c.row.as<int32_t>(), c.pix.as<int32_t>(), c.w.as<float>(),
                                m->has_dark ? m->d_dark.as<double>() : nullptr, m->nb(),
                                c.dterm.as<float>())))
       return rc;
     c.dterm_valid = true;
-    m->binfo_valid = false;
+    for (int f = 0; f < 4; f++) m->binfo_ok[mode * 4 + f] = false;
   }
-  if (m->binfo_valid && m->binfo_mode == mode && m->binfo_flags == flags) return MC_OK;
-  if ((rc = launch_binfo(s, m->inv_area.as<float>(), m->d_flags.as<int32_t>(),
-                         c.dterm.as<float>(), m->nb(), (flags & MC_F_NORMALIZE) ? 1 : 0,
-                         (flags & MC_F_MASK_NAN) ? 1 : 0, m->d_binfo.as<float2>())))
-    return rc;
-  m->binfo_mode = mode;
-  m->binfo_flags = flags;
-  m->binfo_valid = true;
+  if (!m->binfo_ok[key]) {
+    if (!m->d_binfo[key].p && (rc = m->d_binfo[key].alloc(m->nb() * sizeof(float2)))) return rc;
+    if ((rc = launch_binfo(s, m->inv_area.as<float>(), m->d_flags.as<int32_t>(),
+                           c.dterm.as<float>(), m->nb(), (flags & MC_F_NORMALIZE) ? 1 : 0,
+                           (flags & MC_F_MASK_NAN) ? 1 : 0, m->d_binfo[key].as<float2>())))
+      return rc;
+    // Tables are read by kernels on other streams (the frame server).
+    MC_HIP(hipStreamSynchronize(s));
+    m->binfo_ok[key] = true;
+  }
+  *out = m->d_binfo[key].as<float2>();
   return MC_OK;
 }
 
 // Identity frame slots 0..127 for contiguous batches.
-static int identity_slots(Ctx *ctx, int32_t **out) {
+int identity_slots(Ctx *ctx, int32_t **out) {
   static thread_local std::vector<std::pair<Ctx *, void *>> cache;
   for (auto &e : cache)
     if (e.first == ctx) {
@@ -442,20 +499,21 @@ static int identity_slots(Ctx *ctx, int32_t **out) {
   return MC_OK;
 }
 
-int run_integrate(Map *m, const void *frames_dev, int dtype, int n_frames, int mode,
-                  uint32_t flags, float *cakes_dev, float *lineouts_dev, bool force_plain,
-                  bool force_batched) {
+int run_integrate(Map *m, hipStream_t s, const void *frames_dev, int dtype, int n_frames,
+                  int mode, uint32_t flags, float *cakes_dev, float *lineouts_dev,
+                  bool force_plain, bool force_batched) {
   int rc;
-  if ((rc = ensure_binfo(m, mode, flags))) return rc;
-  hipStream_t s = m->ctx->stream;
+  const float2 *binfo = nullptr;
+  if ((rc = ensure_binfo(m, mode, flags, &binfo))) return rc;
+  const int mask_nan = (flags & MC_F_MASK_NAN) ? 1 : 0;
   const size_t es = dtype_size(dtype);
   MC_CHECK(es > 0, MC_ERR_ARG, "unsupported dtype %d", dtype);
   if ((n_frames == 1 && !force_batched) || force_plain) {
-    return launch_plain(s, *m, m->csr[mode], frames_dev, dtype, cakes_dev, lineouts_dev,
-                        n_frames, m->npx());
+    return launch_plain(s, *m, m->csr[mode], binfo, mask_nan, frames_dev, dtype, cakes_dev,
+                        lineouts_dev, n_frames, m->npx());
   }
   Tiles *t = nullptr;
-  if ((rc = ensure_tiles(m, mode, batched_elem_size(dtype), &t))) return rc;
+  if ((rc = ensure_tiles(m, mode, (int)dtype_size(dtype), &t))) return rc;
   int32_t *slots;
   if ((rc = identity_slots(m->ctx, &slots))) return rc;
   for (int f0 = 0; f0 < n_frames; f0 += t->batch) {
@@ -463,7 +521,8 @@ int run_integrate(Map *m, const void *frames_dev, int dtype, int n_frames, int m
     const char *fb = (const char *)frames_dev + (size_t)f0 * m->npx() * es;
     float *cb = cakes_dev + (size_t)f0 * m->nb();
     float *lb = lineouts_dev ? lineouts_dev + (size_t)f0 * 3 * m->n_r : nullptr;
-    if ((rc = launch_batched(s, *m, *t, fb, dtype, nf, slots, cb, lb))) return rc;
+    if ((rc = launch_batched(s, *m, *t, binfo, mask_nan, fb, dtype, nf, slots, cb, lb)))
+      return rc;
   }
   return MC_OK;
 }
@@ -703,13 +762,14 @@ int mc_map_info_get(mc_map *m, mc_map_info *info) {
   info->n_flagged = nf;
   info->q_mode = m->q_mode;
   int64_t hb = m->d_rec.bytes + m->d_pairs.bytes + m->d_flags.bytes + m->d_area_inb.bytes +
-               m->d_row_static.bytes + m->d_binfo.bytes + m->inv_area.bytes + m->d_dark.bytes;
+               m->d_row_static.bytes + m->inv_area.bytes + m->d_dark.bytes;
+  for (const DBuf &b : m->d_binfo) hb += b.bytes;
   for (int q = 0; q < 3; q++) {
     const Csr &c = m->csr[q];
     hb += c.row.bytes + c.pix.bytes + c.w.bytes + c.dterm.bytes;
     for (auto &t : m->tiles[q])
-      hb += t->group_tile.bytes + t->group_row0.bytes + t->tiles.bytes + t->tbins.bytes +
-            t->ents.bytes;
+      hb += t->chunks.bytes + t->tiles.bytes + t->tbins.bytes + t->ents.bytes +
+            t->block_chunk.bytes + t->part.bytes;
   }
   info->hbm_bytes = hb;
   for (int q = 0; q < 3; q++) info->skipped[q] = m->skipped[q];
@@ -760,7 +820,7 @@ int mc_map_set_dark(mc_map *m, const void *dark, int32_t dtype, int32_t n_frames
   std::lock_guard<std::mutex> lk2(m->mu);
   MC_HIP(hipSetDevice(m->ctx->device));
   for (auto &c : m->csr) c.dterm_valid = false;
-  m->binfo_valid = false;
+  for (bool &ok : m->binfo_ok) ok = false;
   if (!dark) {
     m->has_dark = false;
     return MC_OK;
@@ -855,7 +915,8 @@ int mc_integrate(mc_map *m, const void *frames, int32_t dtype, int32_t n_frames,
     if ((rc = dl.alloc(lbytes))) return rc;
     ld = dl.as<float>();
   }
-  if ((rc = run_integrate(m, fd, dtype, n_frames, mode, flags, cd, ld, false, false))) return rc;
+  if ((rc = run_integrate(m, s, fd, dtype, n_frames, mode, flags, cd, ld, false, false)))
+    return rc;
   if (!cakes_on_device) MC_HIP(hipMemcpyAsync(cakes, cd, cbytes, hipMemcpyDeviceToHost, s));
   if (lineouts && !lineouts_on_device)
     MC_HIP(hipMemcpyAsync(lineouts, ld, lbytes, hipMemcpyDeviceToHost, s));
@@ -902,9 +963,11 @@ int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype, int32_t 
   MC_HIP(hipSetDevice(m->ctx->device));
   hipStream_t s = m->ctx->stream;
   int rc;
-  if ((rc = ensure_binfo(m, mode, flags))) return rc;
+  const float2 *binfo = nullptr;
+  if ((rc = ensure_binfo(m, mode, flags, &binfo))) return rc;
+  const int mask_nan = (flags & MC_F_MASK_NAN) ? 1 : 0;
   Tiles *t = nullptr;
-  if (kind == 1 && (rc = ensure_tiles(m, mode, batched_elem_size(dtype), &t))) return rc;
+  if (kind == 1 && (rc = ensure_tiles(m, mode, (int)dtype_size(dtype), &t))) return rc;
   int32_t *slots;
   if ((rc = identity_slots(m->ctx, &slots))) return rc;
   const size_t es = dtype_size(dtype);
@@ -916,8 +979,10 @@ int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype, int32_t 
       float *cb = cakes_dev + (size_t)f0 * m->nb();
       float *lb = lineouts_dev ? lineouts_dev + (size_t)f0 * 3 * m->n_r : nullptr;
       if (ev) MC_HIP(hipEventRecord((*ev)[2 * (f0 / step)], s));
-      int r2 = kind == 1 ? launch_batched(s, *m, *t, fb, dtype, nf, slots, cb, lb)
-                         : launch_plain(s, *m, m->csr[mode], fb, dtype, cb, lb, 1, m->npx());
+      int r2 = kind == 1
+                   ? launch_batched(s, *m, *t, binfo, mask_nan, fb, dtype, nf, slots, cb, lb)
+                   : launch_plain(s, *m, m->csr[mode], binfo, mask_nan, fb, dtype, cb, lb, 1,
+                                  m->npx());
       if (r2) return r2;
       if (ev) MC_HIP(hipEventRecord((*ev)[2 * (f0 / step) + 1], s));
     }
@@ -954,6 +1019,56 @@ int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype, int32_t 
   (void)hipEventDestroy(e1);
   if (ms_per_launch) *ms_per_launch = kern_ms / ((double)iters * launches);
   if (ms_per_iter) *ms_per_iter = tot / iters;
+  return MC_OK;
+}
+
+// Times the batched kernel for each candidate (batch, rows per workgroup,
+// LDS budget) on device frames of the map's shape (constant content: the
+// kernel's work does not depend on pixel values) and keeps the fastest.
+int mc_map_autotune(mc_map *m, int32_t dtype, double *best_fps) {
+  MC_CHECK(m, MC_ERR_ARG, "null map");
+  const size_t es = dtype_size(dtype);
+  MC_CHECK(es > 0, MC_ERR_ARG, "unsupported dtype %d", dtype);
+  const int keep_b = m->batch, keep_h = m->rows_per_group, keep_l = m->lds_bytes;
+  DBuf frames, cakes;
+  int rc;
+  if ((rc = frames.alloc((size_t)128 * m->npx() * es)) ||
+      (rc = cakes.alloc((size_t)128 * m->nb() * 4)))
+    return rc;
+  MC_HIP(hipMemset(frames.p, 0x11, frames.bytes));
+  const int Bs[] = {32, 64, 128}, Hs[] = {4, 8, 16}, Ls[] = {65536, 98304};
+  double best = -1;
+  int bb = keep_b, bh = keep_h, bl = keep_l;
+  for (int B : Bs)
+    for (int H : Hs)
+      for (int L : Ls) {
+        if (mc_map_set_tuning(m, B, H, L)) continue;
+        double per_launch = 0, per_iter = 0;
+        if (mc_time_integrate(m, frames.p, dtype, 128, MC_MODE_BILINEAR,
+                              MC_F_NORMALIZE | MC_F_MASK_NAN, 1, 3, 1, cakes.as<float>(),
+                              nullptr, &per_launch, &per_iter))
+          continue;  // e.g. an LDS budget the device does not grant
+        const double fps = 128.0 / (per_iter / 1e3);
+        if (fps > best) {
+          best = fps;
+          bb = B;
+          bh = H;
+          bl = L;
+        }
+      }
+  MC_CHECK(best > 0, MC_ERR_STATE, "no candidate tuning ran: %s", g_err.c_str());
+  mc_map_set_tuning(m, bb, bh, bl);
+  {
+    // Drop the layouts of the losers.
+    std::lock_guard<std::mutex> lk(m->mu);
+    for (auto &v : m->tiles) {
+      std::vector<std::unique_ptr<Tiles>> keep;
+      for (auto &t : v)
+        if (t->batch == bb && t->rows == bh && t->lds_bytes == bl) keep.push_back(std::move(t));
+      v.swap(keep);
+    }
+  }
+  if (best_fps) *best_fps = best;
   return MC_OK;
 }
 

@@ -96,16 +96,18 @@ struct Csr {
 // LDS budget).
 struct Tiles {
   bool built = false;
-  int batch = 0, rows = 0, lds_bytes = 0, pitch = 0;  // pitch = LDS frames stride
-  int esize = 0;  // bytes per staged LDS element (2: u16, 4: f32)
-  int n_groups = 0;
+  int batch = 0, rows = 0, lds_bytes = 0;
+  int in_size = 0;   // bytes per input pixel (frame dtype)
+  int vec = 1;       // pixels per staging load (8 bytes when rows are aligned)
+  int n_chunks = 0, n_blocks = 0;
   size_t lds_needed = 0;
-  DBuf group_tile;  // int32 [n_groups+1] tile ranges, in launch order
-  DBuf group_row0;  // int32 [n_groups]
-  DBuf tiles;       // int32 x6 per tile: y0 z0 bw bh bin_begin bin_end
-  DBuf tbins;       // int32 x4 per tile bin: cake bin, ent_begin, ent_end, row|direct
-  DBuf ents;        // uint32 x2: index (LDS or global), weight bits
-  int64_t n_tiles = 0, n_tbins = 0, n_ents = 0, n_direct = 0;
+  DBuf chunks;       // int32 x4 per chunk: tile_begin tile_end row0 n_rows
+  DBuf tiles;        // int32 x10 per tile (see mc_kernels.hip)
+  DBuf tbins;        // int32 x4 per tile bin: cake bin, ent_begin, ent_end, row|direct
+  DBuf ents;         // uint32 x2: index (tile box or global), weight bits
+  DBuf block_chunk;  // int32 [n_blocks+1]: chunk range of each row block
+  DBuf part;         // float [n_chunks][rows][4][batch]: lineout partials
+  int64_t n_tiles = 0, n_tbins = 0, n_ents = 0, n_direct = 0, staged_px = 0;
 };
 
 struct Map {
@@ -133,11 +135,10 @@ struct Map {
   // per mode layouts
   Csr csr[3];
   std::vector<std::unique_ptr<Tiles>> tiles[3];
-  // per-bin float2 {signed scale, dark term} for (mode, flags)
-  DBuf d_binfo;
-  int binfo_mode = -1;
-  uint32_t binfo_flags = 0xffffffffu;
-  bool binfo_valid = false;
+  // per-bin float2 {signed scale, dark term}, one table per (mode, flags):
+  // index mode * 4 + flags
+  DBuf d_binfo[12];
+  bool binfo_ok[12] = {};
   // tuning
   int batch = 64, rows_per_group = 8, lds_bytes = 65536;
   int64_t nb() const { return (int64_t)n_r * n_eta; }
@@ -156,16 +157,14 @@ int launch_dark_term(hipStream_t s, const int32_t *row, const int32_t *pix,
 int launch_binfo(hipStream_t s, const float *inv_area, const int32_t *flags,
                  const float *dterm, int64_t nb, int normalize, int mask_nan,
                  float2 *binfo);
-int launch_plain(hipStream_t s, const Map &m, const Csr &c, const void *frame,
-                 int dtype, float *cake, float *lineouts, int n_frames,
-                 int64_t frame_stride);
-int launch_batched(hipStream_t s, const Map &m, const Tiles &t,
-                   const void *frames, int dtype, int n_frames,
+int launch_plain(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo,
+                 int mask_nan, const void *frame, int dtype, float *cake,
+                 float *lineouts, int n_frames, int64_t frame_stride);
+int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
+                   int mask_nan, const void *frames, int dtype, int n_frames,
                    const int32_t *slots_dev, float *cakes, float *lineouts);
 int launch_profile(hipStream_t s, const Map &m, const float *cakes, int n_frames,
                    int mask_nan, float *lineouts);
-int batched_lds_pitch(int batch, int dtype);
-int batched_elem_size(int dtype);
 
 // ---- mapper (mc_mapper.hip) ----
 int map_build_gpu(Ctx *ctx, const mc_geometry *g, const double *mask,
@@ -177,10 +176,14 @@ int map_build_gpu(Ctx *ctx, const mc_geometry *g, const double *mask,
 int map_finish_host(Map *m);  // derived arrays + upload
 int ensure_csr(Map *m, int mode);
 int ensure_tiles(Map *m, int mode, int esize, Tiles **out);
-int run_integrate(Map *m, const void *frames_dev, int dtype, int n_frames, int mode,
-                  uint32_t flags, float *cakes_dev, float *lineouts_dev,
+// Launches the integration of n_frames device frames on stream s (no sync).
+// frame_slots (device, may be NULL) maps batch position -> frame index.
+int run_integrate(Map *m, hipStream_t s, const void *frames_dev, int dtype, int n_frames,
+                  int mode, uint32_t flags, float *cakes_dev, float *lineouts_dev,
                   bool force_plain, bool force_batched);
-int ensure_binfo(Map *m, int mode, uint32_t flags);
+// The per-bin table of (mode, flags); builds it (and the dark term) once.
+int ensure_binfo(Map *m, int mode, uint32_t flags, const float2 **out);
+int identity_slots(Ctx *ctx, int32_t **out);
 
 }  // namespace mc
 

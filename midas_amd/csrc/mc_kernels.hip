@@ -172,78 +172,132 @@ __global__ __launch_bounds__(256) void k_profile(
 }
 
 // ---------------------------------------------------------------------
-// Batched kernel.  Workgroup = one row block (rows r0..r0+H-1, all eta).
-// The block's bins are cut into tiles along eta; a tile's pixel bounding
-// box is staged into LDS for all B frames as [pixel][frame] (pitch P), so
-// one map entry read by the wave serves every frame: lane = frame.
-//   tiles: y0 z0 bw bh bin_begin bin_end
-//   tbins: cake bin, ent_begin, ent_end, (row in block) | (direct << 30)
-//   ents : (LDS pixel index, or global pixel for direct bins), weight
+// Batched kernel.  A workgroup takes one chunk: a run of consecutive tiles
+// of one row block (rows r0..r0+H-1).  For each tile it stages into LDS
+//   * the tile's bin records (cake bin, entry range, per-bin scale/dark,
+//     in-bounds area),
+//   * the tile's map entries (pixel index inside the tile box, weight),
+//   * the pixel box of every frame of the batch, one plane per frame
+//     (plane stride S elements, chosen so lanes reading the same pixel of
+//     consecutive frames hit different LDS banks),
+// then reduces every bin with lane = frame: one entry read (an LDS
+// broadcast) serves all frames of the batch.  Bins whose pixel box does not
+// fit the LDS budget ("direct" bins) gather from global memory instead.
+// Lineout sums per (row, frame) are kept in LDS and written as one partial
+// per chunk; k_lineout_finalize adds the partials of each row block.
+//   chunk: tile_begin tile_end row0 n_rows
+//   tile : y0 z0 bwp bh kshift S b_beg b_end e_beg e_end
+//          (bwp = padded box width = nv * V, nv = 1 << kshift vector loads)
+//   tbin : cake bin, ent_begin, ent_end, (row in block) | (direct << 30)
+//   ent  : (pixel index in the tile box, or global pixel if direct), weight
 // ---------------------------------------------------------------------
-template <typename TIn, typename TL, int B>
+constexpr int kTileBins = 128;    // bin records per tile (LDS)
+constexpr int kTileEnts = 1024;   // map entries per tile (LDS)
+
+struct BinRec {
+  int4 tb;
+  float2 info;
+  float a;
+  float pad;
+};
+
+template <typename TIn, int V> struct alignas(sizeof(TIn) * V) VecLoad {
+  TIn v[V];
+};
+
+template <typename TIn, typename TL, int B, int V>
 __global__ __launch_bounds__(256) void k_batched(
-    const int32_t *__restrict__ group_tile, const int32_t *__restrict__ group_row0,
-    const int32_t *__restrict__ tiles, const int4 *__restrict__ tbins,
-    const uint2 *__restrict__ ents, const float2 *__restrict__ binfo,
-    const float *__restrict__ a_inb, const float4 *__restrict__ row_static,
+    const int32_t *__restrict__ chunks, const int32_t *__restrict__ tiles,
+    const int4 *__restrict__ tbins, const uint2 *__restrict__ ents,
+    const float2 *__restrict__ binfo, const float *__restrict__ a_inb,
     const TIn *__restrict__ frames, int64_t frame_stride,
-    const int32_t *__restrict__ slots, int n_frames, int ny, int n_eta,
-    int n_r, int rows_per_group, int pitch, int64_t nb, int mask_nan,
-    float *__restrict__ cakes, float *__restrict__ lineouts) {
+    const int32_t *__restrict__ slots, int n_frames, int ny, int rows_per_group,
+    int64_t nb, float *__restrict__ cakes, float *__restrict__ part) {
   extern __shared__ __attribute__((aligned(16))) char lds_raw[];
-  constexpr int SPW = B >= 64 ? 1 : 64 / B;   // bins per wave at once
-  constexpr int FPL = B > 64 ? B / 64 : 1;     // frames per lane
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  constexpr int SPW = B >= 64 ? 1 : 64 / B;  // bins per wave at once
+  constexpr int FPL = B > 64 ? B / 64 : 1;    // frames per lane
+  constexpr int U = 8;                        // staging loads in flight per lane
+  BinRec *lbin = reinterpret_cast<BinRec *>(lds_raw);
+  uint2 *lent = reinterpret_cast<uint2 *>(lds_raw + kTileBins * sizeof(BinRec));
+  float *lo = reinterpret_cast<float *>(lds_raw + kTileBins * sizeof(BinRec) +
+                                        kTileEnts * sizeof(uint2));
+  TL *plane = reinterpret_cast<TL *>(reinterpret_cast<char *>(lo) +
+                                     (size_t)rows_per_group * 4 * B * sizeof(float));
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int slot = SPW > 1 ? lane / B : 0;
   const int f0 = SPW > 1 ? lane % B : lane * FPL;
-  const int g = blockIdx.x;
-  const int r0 = group_row0[g];
-  const int t_beg = group_tile[g], t_end = group_tile[g + 1];
-  float *lo = reinterpret_cast<float *>(lds_raw);  // [H][4][B]
-  TL *tile = reinterpret_cast<TL *>(lds_raw + (size_t)rows_per_group * 4 * B * sizeof(float));
-  const bool want_lo = lineouts != nullptr;
-  for (int i = threadIdx.x; i < rows_per_group * 4 * B; i += 256) lo[i] = 0.f;
+  const int c = blockIdx.x;
+  const int t_beg = chunks[4 * c], t_end = chunks[4 * c + 1];
+  const bool want_lo = part != nullptr;
+  if (want_lo)
+    for (int i = tid; i < rows_per_group * 4 * B; i += 256) lo[i] = 0.f;
 
   for (int t = t_beg; t < t_end; ++t) {
-    const int y0 = tiles[6 * t + 0], z0 = tiles[6 * t + 1];
-    const int bw = tiles[6 * t + 2], bh = tiles[6 * t + 3];
-    const int b_beg = tiles[6 * t + 4], b_end = tiles[6 * t + 5];
+    const int32_t *T = tiles + 10 * t;
+    const int y0 = T[0], z0 = T[1], bwp = T[2], bh = T[3], ks = T[4], S = T[5];
+    const int b_beg = T[6], nbt = T[7] - T[6], e_beg = T[8], net = T[9] - T[8];
     __syncthreads();
-    // ---- stage the bounding box of every frame into LDS ----
-    if (bw > 0) {
-      const int cols = bw <= 16 ? 16 : (bw <= 32 ? 32 : 64);
-      const int rpi = 64 / cols;
-      const int lc = lane % cols, lr = lane / cols;
-      const int nrows = n_frames * bh;
-      for (int pr = wave * rpi; pr < nrows; pr += 4 * rpi) {
-        const int q = pr + lr;
-        if (q < nrows) {
-          const int f = q / bh, rr = q - f * bh;
-          const TIn *src = frames + (int64_t)slots[f] * frame_stride +
-                           (int64_t)(z0 + rr) * ny + y0;
-          TL *dst = tile + (size_t)(rr * bw) * pitch + f;
-          for (int c = lc; c < bw; c += cols) dst[(size_t)c * pitch] = static_cast<TL>(src[c]);
+    // ---- bin records and entries ----
+    for (int i = tid; i < nbt; i += 256) {
+      BinRec r;
+      r.tb = tbins[b_beg + i];
+      r.info = binfo[r.tb.x];
+      r.a = a_inb[r.tb.x];
+      r.pad = 0.f;
+      lbin[i] = r;
+    }
+    for (int k = tid; k < net; k += 256) lent[k] = ents[e_beg + k];
+    // ---- pixel boxes of all frames: plane f holds the box row-major ----
+    if (bwp > 0) {
+      const int nv = 1 << ks;
+      const int rp = 64 >> ks;  // box rows per wave instruction
+      const int cv = lane & (nv - 1), lr = lane >> ks;
+      const int passes = (bh + rp - 1) / rp;
+      const int jobs = n_frames * passes;
+      for (int j0 = wave; j0 < jobs; j0 += 4 * U) {
+        VecLoad<TIn, V> v[U];
+        int dst[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int j = j0 + 4 * u;
+          dst[u] = -1;
+          if (j < jobs) {
+            const int f = j / passes, rr = (j - f * passes) * rp + lr;
+            if (rr < bh) {
+              const TIn *src = frames + (int64_t)slots[f] * frame_stride +
+                               (int64_t)(z0 + rr) * ny + y0 + cv * V;
+              v[u] = *reinterpret_cast<const VecLoad<TIn, V> *>(src);
+              dst[u] = f * S + rr * bwp + cv * V;
+            }
+          }
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (dst[u] >= 0) {
+#pragma unroll
+            for (int q = 0; q < V; ++q) plane[dst[u] + q] = static_cast<TL>(v[u].v[q]);
+          }
       }
     }
     __syncthreads();
     // ---- every wave reduces its share of the tile's bins ----
-    for (int bi = b_beg + wave * SPW + slot; bi < b_end; bi += 4 * SPW) {
-      const int4 tb = tbins[bi];
-      const bool direct = (tb.w >> 30) & 1;
-      const int rib = tb.w & 0x3fffffff;
+    for (int bi = wave * SPW + slot; bi < nbt; bi += 4 * SPW) {
+      const BinRec r = lbin[bi];
+      const bool direct = (r.tb.w >> 30) & 1;
+      const int rib = r.tb.w & 0x3fffffff;
       Kahan acc[FPL];
       if (!direct) {
-        for (int k = tb.y; k < tb.z; ++k) {
-          const uint2 e = ents[k];
+        for (int k = r.tb.y - e_beg; k < r.tb.z - e_beg; ++k) {
+          const uint2 e = lent[k];
           const float wk = __uint_as_float(e.y);
-          const TL *p = tile + (size_t)e.x * pitch + f0;
 #pragma unroll
-          for (int j = 0; j < FPL; ++j) acc[j].add(to_f(p[j]) * wk);
+          for (int j = 0; j < FPL; ++j)
+            acc[j].add(to_f(plane[(f0 + j) * S + (int)e.x]) * wk);
         }
       } else {
-        for (int k = tb.y; k < tb.z; ++k) {
+        for (int k = r.tb.y; k < r.tb.z; ++k) {  // entries stay in global memory
           const uint2 e = ents[k];
           const float wk = __uint_as_float(e.y);
 #pragma unroll
@@ -254,23 +308,21 @@ __global__ __launch_bounds__(256) void k_batched(
           }
         }
       }
-      const float2 inf = binfo[tb.x];
-      const float a = want_lo ? a_inb[tb.x] : 0.f;
 #pragma unroll
       for (int j = 0; j < FPL; ++j) {
         const int f = f0 + j;
         if (f < n_frames) {
-          const float val = (acc[j].value() - inf.y) * fabsf(inf.x);
-          const float o = flagged(inf.x) ? NAN : val;
-          cakes[(int64_t)f * nb + tb.x] = o;
+          const float val = (acc[j].value() - r.info.y) * fabsf(r.info.x);
+          const float o = flagged(r.info.x) ? NAN : val;
+          cakes[(int64_t)f * nb + r.tb.x] = o;
           if (want_lo) {
             float *L = lo + (size_t)rib * 4 * B;
             if (o != 0.f && !isnan(o)) {
               atomicAdd(&L[0 * B + f], o);
               atomicAdd(&L[1 * B + f], 1.f);
             }
-            if (a > 1e-9f && !isnan(o)) {
-              atomicAdd(&L[2 * B + f], o * a);
+            if (r.a > 1e-9f && !isnan(o)) {
+              atomicAdd(&L[2 * B + f], o * r.a);
               atomicAdd(&L[3 * B + f], o);
             }
           }
@@ -280,16 +332,31 @@ __global__ __launch_bounds__(256) void k_batched(
   }
   if (want_lo) {
     __syncthreads();
-    for (int i = threadIdx.x; i < rows_per_group * B; i += 256) {
-      const int rib = i / B, f = i - rib * B;
-      const int r = r0 + rib;
-      if (r < n_r && f < n_frames) {
-        const float *L = lo + (size_t)rib * 4 * B;
-        write_lineouts(make_float4(L[f], L[B + f], L[2 * B + f], L[3 * B + f]), row_static[r],
-                       mask_nan, lineouts + (int64_t)f * 3 * n_r, n_r, r);
-      }
-    }
+    float *P = part + (int64_t)c * rows_per_group * 4 * B;
+    for (int i = tid; i < rows_per_group * 4 * B; i += 256) P[i] = lo[i];
   }
+}
+
+// Lineouts from the per-chunk partial sums: one thread per (frame, row);
+// the chunks of a row block are [block_chunk[b], block_chunk[b+1]).
+__global__ void k_lineout_finalize(const float *__restrict__ part,
+                                   const int32_t *__restrict__ block_chunk,
+                                   const float4 *__restrict__ row_static, int n_frames, int n_r,
+                                   int rows_per_group, int B, int mask_nan,
+                                   float *__restrict__ lineouts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n_frames * n_r) return;
+  const int f = (int)(i / n_r), r = (int)(i % n_r);
+  const int blk = r / rows_per_group, rib = r - blk * rows_per_group;
+  float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int c = block_chunk[blk]; c < block_chunk[blk + 1]; ++c) {
+    const float *L = part + ((int64_t)c * rows_per_group + rib) * 4 * B;
+    s.x += L[f];
+    s.y += L[B + f];
+    s.z += L[2 * B + f];
+    s.w += L[3 * B + f];
+  }
+  write_lineouts(s, row_static[r], mask_nan, lineouts + (int64_t)f * 3 * n_r, n_r, r);
 }
 
 // ---------------------------------------------------------------------
@@ -363,18 +430,6 @@ size_t dtype_size(int dtype) {
   }
 }
 
-int batched_elem_size(int dtype) {
-  return (dtype == MC_DTYPE_U16 || dtype == MC_DTYPE_U8) ? 2 : 4;
-}
-
-// LDS pitch (elements per pixel): the frames of one pixel are contiguous,
-// and the odd dword stride between pixels keeps the staging writes of
-// consecutive pixels on different banks.
-int batched_lds_pitch(int batch, int dtype) {
-  if (batched_elem_size(dtype) == 2) return batch + ((batch / 2) % 2 == 1 ? 0 : 2);
-  return batch + 1;
-}
-
 int launch_area_fill(hipStream_t s, float *p, int64_t n, float v) {
   hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
   MC_HIP(hipGetLastError());
@@ -409,17 +464,16 @@ int launch_binfo(hipStream_t s, const float *inv_area, const int32_t *flags,
   return MC_OK;
 }
 
-int launch_plain(hipStream_t s, const Map &m, const Csr &c, const void *frames,
-                 int dtype, float *cake, float *lineouts, int n_frames,
+int launch_plain(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo, int mask_nan,
+                 const void *frames, int dtype, float *cake, float *lineouts, int n_frames,
                  int64_t frame_stride) {
   return dispatch_dtype(dtype, [&](auto tag) -> int {
     using T = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
     hipLaunchKernelGGL(k_plain<T>, dim3(m.n_r, n_frames), dim3(256), 0, s,
-                       c.row.as<int32_t>(), c.pix.as<int32_t>(), c.w.as<float>(),
-                       m.d_binfo.as<float2>(), m.d_area_inb.as<float>(),
-                       m.d_row_static.as<float4>(), (const T *)frames, frame_stride,
-                       m.n_eta, m.n_r, m.nb(), (m.binfo_flags & MC_F_MASK_NAN) ? 1 : 0, cake,
-                       lineouts);
+                       c.row.as<int32_t>(), c.pix.as<int32_t>(), c.w.as<float>(), binfo,
+                       m.d_area_inb.as<float>(), m.d_row_static.as<float4>(),
+                       (const T *)frames, frame_stride, m.n_eta, m.n_r, m.nb(), mask_nan,
+                       cake, lineouts);
     MC_HIP(hipGetLastError());
     return MC_OK;
   });
@@ -434,38 +488,58 @@ int launch_profile(hipStream_t s, const Map &m, const float *cakes, int n_frames
   return MC_OK;
 }
 
-template <typename TIn, int B>
-static int launch_batched_t(hipStream_t s, const Map &m, const Tiles &t,
-                            const void *frames, int n_frames, const int32_t *slots,
-                            float *cakes, float *lineouts) {
+template <typename TIn, int B, int V>
+static int launch_batched_v(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
+                            int mask_nan, const void *frames, int n_frames,
+                            const int32_t *slots, float *cakes, float *lineouts) {
   using TL = typename std::conditional<sizeof(TIn) <= 2, uint16_t, float>::type;
-  auto kern = k_batched<TIn, TL, B>;
+  auto kern = k_batched<TIn, TL, B, V>;
   const size_t lds = t.lds_needed;
-  if (lds > 65536) MC_HIP(hipFuncSetAttribute((const void *)kern,
-                                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              (int)lds));
-  hipLaunchKernelGGL(kern, dim3(t.n_groups), dim3(256), lds, s,
-                     t.group_tile.as<int32_t>(), t.group_row0.as<int32_t>(),
-                     t.tiles.as<int32_t>(), t.tbins.as<int4>(), t.ents.as<uint2>(),
-                     m.d_binfo.as<float2>(), m.d_area_inb.as<float>(),
-                     m.d_row_static.as<float4>(), (const TIn *)frames, m.npx(), slots,
-                     n_frames, m.ny, m.n_eta, m.n_r, t.rows, t.pitch, m.nb(),
-                     (m.binfo_flags & MC_F_MASK_NAN) ? 1 : 0, cakes, lineouts);
+  if (lds > 65536)
+    MC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+  float *part = lineouts ? t.part.as<float>() : nullptr;
+  hipLaunchKernelGGL(kern, dim3(t.n_chunks), dim3(256), lds, s, t.chunks.as<int32_t>(),
+                     t.tiles.as<int32_t>(), t.tbins.as<int4>(), t.ents.as<uint2>(), binfo,
+                     m.d_area_inb.as<float>(), (const TIn *)frames, m.npx(), slots, n_frames,
+                     m.ny, t.rows, m.nb(), cakes, part);
   MC_HIP(hipGetLastError());
+  if (lineouts) {
+    const int64_t n = (int64_t)n_frames * m.n_r;
+    hipLaunchKernelGGL(k_lineout_finalize, dim3(blocks_for(n, 256)), dim3(256), 0, s, part,
+                       t.block_chunk.as<int32_t>(), m.d_row_static.as<float4>(), n_frames,
+                       m.n_r, t.rows, B, mask_nan, lineouts);
+    MC_HIP(hipGetLastError());
+  }
   return MC_OK;
 }
 
-int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const void *frames,
-                   int dtype, int n_frames, const int32_t *slots, float *cakes,
-                   float *lineouts) {
+template <typename TIn, int B>
+static int launch_batched_t(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
+                            int mask_nan, const void *frames, int n_frames,
+                            const int32_t *slots, float *cakes, float *lineouts) {
+  constexpr int VV = sizeof(TIn) < 8 ? 8 / (int)sizeof(TIn) : 1;
+  if (t.vec == VV)
+    return launch_batched_v<TIn, B, VV>(s, m, t, binfo, mask_nan, frames, n_frames, slots,
+                                        cakes, lineouts);
+  return launch_batched_v<TIn, B, 1>(s, m, t, binfo, mask_nan, frames, n_frames, slots, cakes,
+                                     lineouts);
+}
+
+int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
+                   int mask_nan, const void *frames, int dtype, int n_frames,
+                   const int32_t *slots, float *cakes, float *lineouts) {
   return dispatch_dtype(dtype, [&](auto tag) -> int {
     using T = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
+#define MC_LB(Bv) \
+  launch_batched_t<T, Bv>(s, m, t, binfo, mask_nan, frames, n_frames, slots, cakes, lineouts)
     switch (t.batch) {
-      case 8: return launch_batched_t<T, 8>(s, m, t, frames, n_frames, slots, cakes, lineouts);
-      case 16: return launch_batched_t<T, 16>(s, m, t, frames, n_frames, slots, cakes, lineouts);
-      case 32: return launch_batched_t<T, 32>(s, m, t, frames, n_frames, slots, cakes, lineouts);
-      case 64: return launch_batched_t<T, 64>(s, m, t, frames, n_frames, slots, cakes, lineouts);
-      case 128: return launch_batched_t<T, 128>(s, m, t, frames, n_frames, slots, cakes, lineouts);
+      case 8: return MC_LB(8);
+      case 16: return MC_LB(16);
+      case 32: return MC_LB(32);
+      case 64: return MC_LB(64);
+      case 128: return MC_LB(128);
+#undef MC_LB
       default:
         set_error("batch %d not in {8,16,32,64,128}", t.batch);
         return MC_ERR_ARG;

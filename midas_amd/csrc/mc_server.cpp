@@ -1,0 +1,436 @@
+// mc_server.cpp -- continuous-batching frame server (the MI355X analogue of
+// the frame server in IntegratorFitPeaksGPUStream.cu:567-734, 2567-2883).
+//
+//  * Geometries (map + frame dtype + mode + flags) stay resident on the
+//    device; each is admitted only if the maps, the in-flight batch buffers
+//    and the new geometry's tables fit the HBM budget.
+//  * submit() copies a frame into a pinned host buffer and queues it; a
+//    full queue blocks the caller (backpressure) up to its timeout.
+//  * A scheduler thread forms batches while earlier batches are on the GPU:
+//    it takes the oldest queued request and up to max_batch queued frames
+//    of the same geometry (frames are grouped by map), copies them to the
+//    device on one of several in-flight slots, each with its own stream,
+//    and launches the one-frame kernel (one frame) or the batched kernel.
+//  * A completion thread waits for each slot's event in launch order, hands
+//    the cakes and lineouts to the caller's callback and records the
+//    latency from submit to result (p50/p99).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "mc_internal.h"
+
+namespace mc {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_since(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct Geo {
+  mc_map *m = nullptr;
+  int dtype = 1, mode = 1;
+  uint32_t flags = 0;
+  size_t frame_bytes = 0, cake_floats = 0, lo_floats = 0;
+};
+
+struct Req {
+  int geo = 0;
+  int64_t id = 0;
+  void *host = nullptr;  // pinned copy of the frame
+  Clock::time_point t_submit;
+};
+
+struct Slot {
+  hipStream_t s = nullptr;
+  hipEvent_t done = nullptr;
+  DBuf frames, cakes, lo;
+  float *h_cakes = nullptr, *h_lo = nullptr;  // pinned
+  size_t cap_frames = 0, cap_cakes = 0, cap_lo = 0;
+  std::vector<Req> reqs;
+  int geo = -1;
+  bool busy = false;
+  int rc = MC_OK;
+};
+
+}  // namespace
+}  // namespace mc
+
+using namespace mc;
+
+struct mc_server {
+  Ctx *ctx = nullptr;
+  int64_t budget = 0;
+  int max_batch = 64, qdepth = 256;
+  mc_result_cb cb = nullptr;
+  void *user = nullptr;
+  std::vector<Geo> geos;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_space, cv_idle, cv_done;
+  std::deque<Req> queue;
+  std::vector<Slot> slots;
+  std::deque<int> launched;  // slot indices in launch order
+  std::vector<std::vector<void *>> free_host;  // pinned frame buffers per geometry
+  std::thread sched, compl_;
+  bool stop = false;
+  int in_flight = 0;
+  // stats
+  std::vector<double> lat;
+  int64_t frames_done = 0, batches = 0, rejected = 0, peak_bytes = 0;
+  Clock::time_point t_first, t_last;
+  bool any = false;
+
+  int64_t resident_bytes() {
+    int64_t b = 0;
+    std::vector<mc_map *> seen;
+    for (auto &g : geos)
+      if (std::find(seen.begin(), seen.end(), g.m) == seen.end()) {
+        seen.push_back(g.m);
+        mc_map_info info;
+        mc_map_info_get(g.m, &info);
+        b += info.hbm_bytes;
+      }
+    for (auto &s : slots) b += s.frames.bytes + s.cakes.bytes + s.lo.bytes;
+    return b;
+  }
+};
+
+namespace {
+
+int grow_slots(mc_server *sv, size_t fb, size_t cf, size_t lf) {
+  for (auto &s : sv->slots) {
+    int rc;
+    if (s.cap_frames < fb) {
+      if ((rc = s.frames.alloc(fb))) return rc;
+      s.cap_frames = fb;
+    }
+    if (s.cap_cakes < cf) {
+      if ((rc = s.cakes.alloc(cf * 4))) return rc;
+      if (s.h_cakes) (void)hipHostFree(s.h_cakes);
+      MC_HIP(hipHostMalloc((void **)&s.h_cakes, cf * 4, hipHostMallocDefault));
+      s.cap_cakes = cf;
+    }
+    if (s.cap_lo < lf) {
+      if ((rc = s.lo.alloc(lf * 4))) return rc;
+      if (s.h_lo) (void)hipHostFree(s.h_lo);
+      MC_HIP(hipHostMalloc((void **)&s.h_lo, lf * 4, hipHostMallocDefault));
+      s.cap_lo = lf;
+    }
+  }
+  return MC_OK;
+}
+
+void scheduler(mc_server *sv) {
+  (void)hipSetDevice(sv->ctx->device);
+  std::unique_lock<std::mutex> lk(sv->mu);
+  for (;;) {
+    int slot = -1;
+    sv->cv_work.wait(lk, [&] {
+      if (sv->stop && sv->queue.empty()) return true;
+      if (sv->queue.empty()) return false;
+      for (size_t i = 0; i < sv->slots.size(); i++)
+        if (!sv->slots[i].busy) {
+          slot = (int)i;
+          return true;
+        }
+      return false;
+    });
+    if (sv->queue.empty()) break;
+    Slot &S = sv->slots[slot];
+    // The oldest request picks the geometry; take up to max_batch queued
+    // frames of that geometry in arrival order.
+    const int g = sv->queue.front().geo;
+    S.reqs.clear();
+    for (auto it = sv->queue.begin(); it != sv->queue.end() && (int)S.reqs.size() < sv->max_batch;) {
+      if (it->geo == g) {
+        S.reqs.push_back(*it);
+        it = sv->queue.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    S.busy = true;
+    S.geo = g;
+    sv->in_flight++;
+    sv->cv_space.notify_all();
+    const Geo &G = sv->geos[g];
+    const int nf = (int)S.reqs.size();
+    lk.unlock();
+    int rc = MC_OK;
+    for (int i = 0; i < nf && !rc; i++)
+      if (hipMemcpyAsync((char *)S.frames.p + (size_t)i * G.frame_bytes, S.reqs[i].host,
+                         G.frame_bytes, hipMemcpyHostToDevice, S.s) != hipSuccess)
+        rc = MC_ERR_HIP;
+    if (!rc) {
+      std::lock_guard<std::mutex> mlk(G.m->mu);
+      rc = run_integrate(G.m, S.s, S.frames.p, G.dtype, nf, G.mode, G.flags, S.cakes.as<float>(),
+                         S.lo.as<float>(), false, false);
+    }
+    if (!rc && hipMemcpyAsync(S.h_cakes, S.cakes.p, (size_t)nf * G.cake_floats * 4,
+                              hipMemcpyDeviceToHost, S.s) != hipSuccess)
+      rc = MC_ERR_HIP;
+    if (!rc && hipMemcpyAsync(S.h_lo, S.lo.p, (size_t)nf * G.lo_floats * 4,
+                              hipMemcpyDeviceToHost, S.s) != hipSuccess)
+      rc = MC_ERR_HIP;
+    if (!rc && hipEventRecord(S.done, S.s) != hipSuccess) rc = MC_ERR_HIP;
+    S.rc = rc;
+    lk.lock();
+    sv->launched.push_back(slot);
+    sv->batches++;
+    sv->cv_done.notify_all();
+  }
+  sv->cv_done.notify_all();
+}
+
+void completer(mc_server *sv) {
+  (void)hipSetDevice(sv->ctx->device);
+  std::unique_lock<std::mutex> lk(sv->mu);
+  for (;;) {
+    sv->cv_done.wait(lk, [&] { return !sv->launched.empty() || (sv->stop && sv->in_flight == 0); });
+    if (sv->launched.empty()) break;
+    const int slot = sv->launched.front();
+    sv->launched.pop_front();
+    Slot &S = sv->slots[slot];
+    lk.unlock();
+    int rc = S.rc;
+    if (!rc && hipEventSynchronize(S.done) != hipSuccess) rc = MC_ERR_HIP;
+    const auto t = Clock::now();
+    const Geo &G = sv->geos[S.geo];
+    for (size_t i = 0; i < S.reqs.size(); i++)
+      if (sv->cb)
+        sv->cb(sv->user, S.reqs[i].id, S.geo, rc ? nullptr : S.h_cakes + i * G.cake_floats,
+               rc ? nullptr : S.h_lo + i * G.lo_floats, rc);
+    lk.lock();
+    for (auto &r : S.reqs) {
+      sv->lat.push_back(ms_since(r.t_submit, t));
+      sv->free_host[r.geo].push_back(r.host);
+      if (!sv->any) {
+        sv->t_first = r.t_submit;
+        sv->any = true;
+      }
+      if (r.t_submit < sv->t_first) sv->t_first = r.t_submit;
+    }
+    sv->t_last = t;
+    sv->frames_done += (int64_t)S.reqs.size();
+    S.reqs.clear();
+    S.busy = false;
+    sv->in_flight--;
+    sv->cv_work.notify_all();
+    sv->cv_space.notify_all();
+    sv->cv_idle.notify_all();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mc_server_create(mc_ctx *ctx, int64_t hbm_budget_bytes, int32_t max_batch,
+                     int32_t queue_depth, mc_result_cb cb, void *user, mc_server **out) {
+  MC_CHECK(ctx && out && hbm_budget_bytes > 0, MC_ERR_ARG, "bad arguments");
+  MC_CHECK(max_batch >= 1 && max_batch <= 128, MC_ERR_ARG, "max_batch %d not in 1..128",
+           max_batch);
+  MC_CHECK(queue_depth >= 1, MC_ERR_ARG, "queue_depth %d", queue_depth);
+  MC_HIP(hipSetDevice(ctx->device));
+  auto *sv = new mc_server();
+  sv->ctx = ctx;
+  sv->budget = hbm_budget_bytes;
+  sv->max_batch = max_batch;
+  sv->qdepth = queue_depth;
+  sv->cb = cb;
+  sv->user = user;
+  sv->slots = std::vector<Slot>(3);
+  for (auto &s : sv->slots) {
+    if (hipStreamCreateWithFlags(&s.s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+      set_error("server: stream/event creation failed");
+      delete sv;
+      return MC_ERR_HIP;
+    }
+  }
+  sv->sched = std::thread(scheduler, sv);
+  sv->compl_ = std::thread(completer, sv);
+  *out = sv;
+  return MC_OK;
+}
+
+int mc_server_add_geometry(mc_server *sv, mc_map *m, int32_t dtype, int32_t mode, uint32_t flags,
+                           int32_t *geometry) {
+  MC_CHECK(sv && m && geometry, MC_ERR_ARG, "null argument");
+  MC_CHECK(dtype_size(dtype) > 0, MC_ERR_ARG, "unsupported dtype %d", dtype);
+  MC_CHECK(mode >= 0 && mode <= 2 && flags < 4, MC_ERR_ARG, "mode %d flags %u", mode, flags);
+  MC_CHECK(m->ctx == sv->ctx, MC_ERR_ARG, "map belongs to another context");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  // Wait for the device to be idle: slot buffers may have to grow.
+  sv->cv_idle.wait(lk, [&] { return sv->in_flight == 0 && sv->queue.empty(); });
+  int rc;
+  {
+    std::lock_guard<std::recursive_mutex> clk(sv->ctx->mu);
+    std::lock_guard<std::mutex> mlk(m->mu);
+    const float2 *binfo;
+    if ((rc = ensure_binfo(m, mode, flags, &binfo))) return rc;
+    Tiles *t;
+    if ((rc = ensure_tiles(m, mode, (int)dtype_size(dtype), &t))) return rc;
+  }
+  Geo G;
+  G.m = m;
+  G.dtype = dtype;
+  G.mode = mode;
+  G.flags = flags;
+  G.frame_bytes = (size_t)m->npx() * dtype_size(dtype);
+  G.cake_floats = (size_t)m->nb();
+  G.lo_floats = (size_t)3 * m->n_r;
+  size_t fb = 0, cf = 0, lf = 0;
+  for (auto &g : sv->geos) {
+    fb = std::max(fb, g.frame_bytes);
+    cf = std::max(cf, g.cake_floats);
+    lf = std::max(lf, g.lo_floats);
+  }
+  fb = std::max(fb, G.frame_bytes) * sv->max_batch;
+  cf = std::max(cf, G.cake_floats) * sv->max_batch;
+  lf = std::max(lf, G.lo_floats) * sv->max_batch;
+  // Admission: resident maps + in-flight buffers after growth.
+  int64_t need = sv->resident_bytes();
+  {
+    mc_map_info info;
+    mc_map_info_get(m, &info);
+    bool known = false;
+    for (auto &g : sv->geos) known |= g.m == m;
+    if (!known) need += info.hbm_bytes;
+  }
+  for (auto &s : sv->slots)
+    need += (int64_t)(std::max(fb, s.cap_frames) - s.frames.bytes) +
+            (int64_t)(std::max(cf, s.cap_cakes) * 4 - s.cakes.bytes) +
+            (int64_t)(std::max(lf, s.cap_lo) * 4 - s.lo.bytes);
+  if (need > sv->budget) {
+    sv->rejected++;
+    set_error("geometry needs %lld bytes resident, budget is %lld", (long long)need,
+              (long long)sv->budget);
+    return MC_ERR_BUDGET;
+  }
+  MC_HIP(hipSetDevice(sv->ctx->device));
+  if ((rc = grow_slots(sv, fb, cf, lf))) return rc;
+  sv->geos.push_back(G);
+  sv->free_host.emplace_back();
+  sv->peak_bytes = std::max(sv->peak_bytes, sv->resident_bytes());
+  *geometry = (int32_t)sv->geos.size() - 1;
+  return MC_OK;
+}
+
+int mc_server_submit(mc_server *sv, int32_t geometry, const void *frame, int64_t request_id,
+                     int32_t timeout_ms) {
+  MC_CHECK(sv && frame, MC_ERR_ARG, "null argument");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  MC_CHECK(geometry >= 0 && geometry < (int)sv->geos.size(), MC_ERR_ARG, "geometry %d",
+           geometry);
+  MC_CHECK(!sv->stop, MC_ERR_STATE, "server is shutting down");
+  const auto t0 = Clock::now();
+  const auto deadline = t0 + std::chrono::milliseconds(timeout_ms < 0 ? 0 : timeout_ms);
+  if (!sv->cv_space.wait_until(lk, deadline, [&] { return (int)sv->queue.size() < sv->qdepth; })) {
+    set_error("queue full for %d ms", timeout_ms);
+    return MC_ERR_TIMEOUT;
+  }
+  const Geo &G = sv->geos[geometry];
+  void *buf = nullptr;
+  auto &pool = sv->free_host[geometry];
+  if (!pool.empty()) {
+    buf = pool.back();
+    pool.pop_back();
+  } else {
+    lk.unlock();
+    MC_HIP(hipHostMalloc(&buf, G.frame_bytes, hipHostMallocDefault));
+    lk.lock();
+  }
+  lk.unlock();
+  memcpy(buf, frame, G.frame_bytes);
+  lk.lock();
+  Req r;
+  r.geo = geometry;
+  r.id = request_id;
+  r.host = buf;
+  r.t_submit = t0;
+  sv->queue.push_back(r);
+  sv->cv_work.notify_one();
+  return MC_OK;
+}
+
+int mc_server_drain(mc_server *sv, int32_t timeout_ms) {
+  MC_CHECK(sv, MC_ERR_ARG, "null server");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  const auto deadline = Clock::now() + std::chrono::milliseconds(timeout_ms);
+  if (!sv->cv_idle.wait_until(lk, deadline,
+                              [&] { return sv->queue.empty() && sv->in_flight == 0; })) {
+    set_error("drain timed out with %zu queued, %d in flight", sv->queue.size(), sv->in_flight);
+    return MC_ERR_TIMEOUT;
+  }
+  return MC_OK;
+}
+
+int mc_server_stats_get(mc_server *sv, mc_server_stats *st) {
+  MC_CHECK(sv && st, MC_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  memset(st, 0, sizeof(*st));
+  st->frames_done = sv->frames_done;
+  st->batches = sv->batches;
+  st->rejected = sv->rejected;
+  st->admitted_bytes = sv->resident_bytes();
+  st->peak_bytes = sv->peak_bytes;
+  if (!sv->lat.empty()) {
+    std::vector<double> v = sv->lat;
+    std::sort(v.begin(), v.end());
+    auto pct = [&](double p) {
+      double idx = p * (v.size() - 1);
+      size_t a = (size_t)idx;
+      size_t b = std::min(a + 1, v.size() - 1);
+      return v[a] + (v[b] - v[a]) * (idx - a);
+    };
+    st->p50_ms = pct(0.50);
+    st->p99_ms = pct(0.99);
+    st->max_ms = v.back();
+    double s = 0;
+    for (double x : v) s += x;
+    st->mean_ms = s / v.size();
+    const double span = ms_since(sv->t_first, sv->t_last);
+    st->frames_per_s = span > 0 ? sv->frames_done / (span / 1e3) : 0;
+  }
+  return MC_OK;
+}
+
+int mc_server_destroy(mc_server *sv) {
+  if (!sv) return MC_OK;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+  }
+  sv->cv_work.notify_all();
+  sv->cv_done.notify_all();
+  if (sv->sched.joinable()) sv->sched.join();
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+  }
+  sv->cv_done.notify_all();
+  if (sv->compl_.joinable()) sv->compl_.join();
+  for (auto &s : sv->slots) {
+    if (s.s) (void)hipStreamSynchronize(s.s);
+    if (s.h_cakes) (void)hipHostFree(s.h_cakes);
+    if (s.h_lo) (void)hipHostFree(s.h_lo);
+    if (s.done) (void)hipEventDestroy(s.done);
+    if (s.s) (void)hipStreamDestroy(s.s);
+  }
+  for (auto &pool : sv->free_host)
+    for (void *p : pool) (void)hipHostFree(p);
+  for (auto &r : sv->queue) (void)hipHostFree(r.host);
+  delete sv;
+  return MC_OK;
+}
+
+}  // extern "C"

@@ -60,12 +60,15 @@ def panel_shifts_for(name: str):
     return None
 
 
-def assert_maps_equal(gpu_pm, orc_map, frac_atol=1e-10):
+def assert_maps_equal(gpu_pm, orc_map, frac_atol=1e-6):
     """Bin membership bit-exact (per-bin counts, offsets, mask flags, the
     pixel y/z of every record in the same order); the FP64 overlap areas to
     frac_atol in pixel-area units.  The GPU's sin/cos/atan2 may round the
-    last bit differently from glibc; the shoelace sum of a clipped polygon
-    turns that into ~1e-12 absolute differences of the area."""
+    last bit differently from glibc: the shoelace sum of a clipped polygon
+    turns that into ~1e-12 absolute differences of the area, and where a
+    vertex sits exactly on a bin's arc (beam centre on a pixel corner) the
+    1e-6 relative arc test of the area routine can take the other branch,
+    a chord/arc difference below 1e-6 of a pixel area."""
     np.testing.assert_array_equal(gpu_pm.counts, orc_map.counts, "per-bin counts")
     np.testing.assert_array_equal(gpu_pm.offsets, orc_map.offsets, "per-bin offsets")
     np.testing.assert_array_equal(gpu_pm.mask_flags, orc_map.flags, "mask flags")
@@ -76,7 +79,10 @@ def assert_maps_equal(gpu_pm, orc_map, frac_atol=1e-10):
     np.testing.assert_allclose(g["areaWeight"], o["areaWeight"], rtol=1e-6, atol=1e-7)
     np.testing.assert_allclose(g["deltaR"], o["deltaR"], rtol=0, atol=1e-5)
     exact = float(np.mean(g["frac"] == o["frac"]))
-    print(f"[map] {g.size} records, frac bit-identical for {exact:.4f} of them", flush=True)
+    loose = int(np.sum(np.abs(g["frac"] - o["frac"]) > 1e-10))
+    print(f"[map] {g.size} records, frac bit-identical for {exact:.4f} of them, "
+          f"{loose} beyond 1e-10", flush=True)
+    assert loose <= max(10, g.size // 1000)
 
 
 def rel_err(gpu, ref, scale):

@@ -1,0 +1,98 @@
+"""Batched LM fits on the GPU vs the FP64 CPU LM of the oracle, and exact
+recovery of noise-free peaks.  Centres must agree within 1e-4 px."""
+import numpy as np
+import pytest
+
+import midas_amd as ma
+from midas_amd import peakfit, synth
+
+pytestmark = pytest.mark.gpu
+CENTER_TOL = 1e-4
+
+
+def _pv_windows(n_win, n_pts, seed, noise):
+    rng = np.random.default_rng(seed)
+    x0 = rng.uniform(50, 950, n_win)
+    x = x0[:, None] + 0.25 * np.arange(n_pts)[None, :]
+    c = x0 + 0.25 * (n_pts // 2) + rng.uniform(-0.6, 0.6, n_win)
+    mu = rng.uniform(0, 1, n_win)
+    fw = rng.uniform(1.0, 3.0, n_win)  # FWHM 1-3 px
+    imax = rng.uniform(200, 2000, n_win)
+    bg = rng.uniform(10, 100, n_win)
+    y = np.stack([peakfit.pseudo_voigt(x[i], c[i], mu[i], fw[i], imax[i], bg[i])
+                  for i in range(n_win)])
+    if noise:
+        y = y + rng.normal(0, 1, y.shape) * imax[:, None] / rng.uniform(20, 200, n_win)[:, None]
+    return x, y, c
+
+
+def test_pv_noise_free_recovery():
+    x, y, c = _pv_windows(2000, 41, 0, noise=False)
+    out = peakfit.fit_windows(x, y, c + 0.1, 0.25, model="pv", max_iter=200)
+    assert (out[:, 7] > 0).all()
+    err = np.abs(out[:, 1] - c)
+    assert np.quantile(err, 0.99) < CENTER_TOL, np.sort(err)[-5:]
+
+
+@pytest.mark.parametrize("model", ["pv", "tch"])
+def test_gpu_lm_matches_fp64_oracle(model, oracle_lib):
+    m = 0 if model == "pv" else 1
+    rng = np.random.default_rng(4)
+    n_win, n_pts = 300, 41
+    x = 100.0 + 0.25 * np.arange(n_pts)[None, :] + rng.uniform(0, 500, (n_win, 1))
+    xs, ys, gs = [], [], []
+    for i in range(n_win):
+        ctr = x[i, 20] + rng.uniform(-0.5, 0.5)
+        if model == "pv":
+            p = [ctr, rng.uniform(0, 1), rng.uniform(0.5, 1.5), rng.uniform(500, 3000), 50.0]
+            yy = oracle_lib.pv_model(p, x[i])
+            amp = p[3]
+        else:
+            p = [rng.uniform(500, 3000), ctr, rng.uniform(50, 400), rng.uniform(5, 60), 40.0, 3.0]
+            yy = oracle_lib.tch_model(p, x[i])
+            amp = yy.max() - 40
+        yy = yy + rng.normal(0, amp / rng.uniform(20, 200), n_pts)
+        xs.append(x[i]); ys.append(yy); gs.append(x[i, 20])
+    X, Y, G = np.array(xs), np.array(ys), np.array(gs)
+    out = peakfit.fit_windows(X, Y, G, 0.25, model=model, max_iter=300)
+    cpu = []
+    for i in range(n_win):
+        p0, lo, hi = oracle_lib.window_init(m, X[i], Y[i].astype(np.float32).astype(np.float64),
+                                            0.25, G[i])
+        p, chi, it = oracle_lib.lm_fit(m, X[i], Y[i].astype(np.float32).astype(np.float64),
+                                       p0, lo, hi, max_iter=300)
+        cpu.append(p[0] if model == "pv" else p[1])
+    d = np.abs(out[:, 1] - np.array(cpu))
+    ok = out[:, 7] > 0
+    assert ok.mean() > 0.99
+    assert np.quantile(d[ok], 0.98) < CENTER_TOL, np.sort(d[ok])[-8:]
+
+
+def test_fit_cake_rings_finds_the_rings():
+    torch = pytest.importorskip("torch")
+    P = synth.plain_1024()
+    frames, dark, mask = synth.frames(P, 2, seed=3)
+    g = ma.build_map(P, mask=mask)
+    g.set_dark(dark)
+    cakes, _ = ma.integrate_batch(torch.from_numpy(frames).cuda(), g, mode="bilinear",
+                                  mask_nan=False)
+    RLo, RHi, _, _ = ma.bin_edges(P)
+    rc = (RLo + RHi) / 2
+    rings = synth.ring_radii(P, 8)
+    out = peakfit.fit_cake_rings(cakes, g, rc, rings, pad=20, model="pv")
+    assert out.shape == (2, P.n_eta_bins, 8, 8)
+    ok = out[..., 7] > 0
+    assert ok.mean() > 0.95
+    # noisy rings of sigma 1.5 px: centres within a small fraction of a bin
+    assert np.median(np.abs(out[..., 1][ok] - np.broadcast_to(rings, ok.shape)[ok])) < 0.05
+
+
+def test_fit_peaks_on_a_lineout():
+    x = 10.0 + 0.25 * np.arange(400)
+    locs = [30.0, 55.5, 80.25]
+    y = np.full_like(x, 20.0)
+    for c in locs:
+        y += peakfit.pseudo_voigt(x, c, 0.4, 1.2, 500.0, 0.0)
+    out = peakfit.fit_peaks(x, y, locs, 0.25, 20)
+    assert out.shape == (3, 7)
+    np.testing.assert_allclose(out[:, 1], locs, atol=0.02)
