@@ -60,7 +60,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(frames shard by frame with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(frames would shard by frame across GPUs, with no exchange)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -169,12 +169,18 @@ int mc_map_set_dark(mc_map *m, const void *dark, int32_t dtype,
                     int32_t n_frames, double scale, int32_t on_device);
 int mc_map_set_beam_center(mc_map *m, double bc_y, double bc_z);
 
-/* Tuning of the batched kernel: frames per pass (8..128), R rows per
- * workgroup, LDS budget in bytes.  0 keeps the current value. */
+/* Tuning of the batched kernel: frames per pass (8, 16, 32, 64 or 128),
+ * R rows per band (16 or 32), LDS budget per workgroup in bytes
+ * (8192..163840).  0 keeps the current value. */
 int mc_map_set_tuning(mc_map *m, int32_t batch, int32_t rows_per_group,
                       int32_t lds_bytes);
 int mc_map_get_tuning(mc_map *m, int32_t *batch, int32_t *rows_per_group,
                       int32_t *lds_bytes);
+/* Layout of the batched kernel for (mode, dtype) under the current tuning
+ * (built if needed): out[8] = tiles, chunks (workgroups), direct bins,
+ * staged pixels per frame, entries, LDS bytes per workgroup, bands,
+ * pixels per staging load. */
+int mc_map_layout_info(mc_map *m, int32_t mode, int32_t dtype, int64_t *out);
 /* Times candidate tunings on synthetic frames of the map's shape and keeps
  * the fastest; writes the winner's figures. */
 int mc_map_autotune(mc_map *m, int32_t dtype, double *best_fps);

@@ -10,7 +10,9 @@ import os
 import threading
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().with_name("libmidas_caking.so")
+# MIDAS_AMD_LIB selects another build of the library (same ABI).
+LIB_PATH = Path(os.environ.get("MIDAS_AMD_LIB") or
+                Path(__file__).resolve().with_name("libmidas_caking.so"))
 
 MC_DTYPE_U8, MC_DTYPE_U16, MC_DTYPE_U32, MC_DTYPE_I64 = 0, 1, 2, 3
 MC_DTYPE_F32, MC_DTYPE_F64, MC_DTYPE_HYBRID, MC_DTYPE_I32 = 4, 5, 6, 7
@@ -105,6 +107,7 @@ SIGNATURES = {
     "mc_map_set_tuning": (_I, [_V, _I32, _I32, _I32]),
     "mc_map_get_tuning": (_I, [_V, _P(_I32), _P(_I32), _P(_I32)]),
     "mc_map_autotune": (_I, [_V, _I32, _PD]),
+    "mc_map_layout_info": (_I, [_V, _I32, _I32, _P(_I64)]),
     "mc_integrate": (_I, [_V, _V, _I32, _I32, _I32, _I32, _U32, _V, _I32, _V, _I32]),
     "mc_profile": (_I, [_V, _V, _I32, _I32, _V, _I32]),
     "mc_time_integrate": (_I, [_V, _V, _I32, _I32, _I32, _U32, _I32, _I32, _I32, _V, _V,

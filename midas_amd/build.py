@@ -24,7 +24,8 @@ ARCH = os.environ.get("MIDAS_AMD_ARCH", "gfx950")
 
 COMMON = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
-    # packed FP32 instructions off ("Build")
+    # packed FP32 instructions off: some packed forms have produced wrong
+    # results on this hardware, and the kernels do not need them
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{INCLUDE}", "-Wno-unused-command-line-argument",
 ]

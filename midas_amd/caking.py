@@ -231,6 +231,22 @@ class CSRGeometry:
         check(lib().mc_map_get_tuning(self.handle, C.byref(b), C.byref(r), C.byref(l)))
         return b.value, r.value, l.value
 
+    def autotune(self, dtype=np.uint16) -> float:
+        """Times the candidate tunings of the batched kernel on frames of
+        this geometry, keeps the fastest and returns its frames/s."""
+        fps = C.c_double()
+        check(lib().mc_map_autotune(self.handle, NP_TO_CODE[np.dtype(dtype)], C.byref(fps)))
+        return fps.value
+
+    def layout_info(self, mode: str = "bilinear", dtype=np.uint16) -> dict:
+        """Tile layout of the batched kernel under the current tuning."""
+        out = (C.c_int64 * 8)()
+        check(lib().mc_map_layout_info(self.handle, _lib.MODES[mode],
+                                       NP_TO_CODE[np.dtype(dtype)], out))
+        keys = ("tiles", "chunks", "direct_bins", "staged_px", "entries", "lds_bytes", "bands",
+                "vec")
+        return dict(zip(keys, (int(v) for v in out)))
+
     def close(self):
         if self.handle:
             lib().mc_map_destroy(self.handle)
@@ -394,7 +410,8 @@ def time_integrate(geom: CSRGeometry, frames_dev, *, kind: str, mode="bilinear",
     t.cuda.current_stream(frames_dev.device).synchronize()
     check(lib().mc_time_integrate(
         geom.handle, C.c_void_p(frames_dev.data_ptr()), _dtype_code(frames_dev), F,
-        _lib.MODES[mode], _flags(normalize, mask_nan), 1 if kind == "batched" else 0, iters,
+        _lib.MODES[mode], _flags(normalize, mask_nan),
+        {"plain": 0, "batched": 1}[kind], iters,
         warmup, C.c_void_p(cakes_dev.data_ptr()),
         C.c_void_p(lineouts_dev.data_ptr()) if lineouts_dev is not None else None,
         C.byref(a), C.byref(b)))

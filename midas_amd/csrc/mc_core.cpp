@@ -237,9 +237,17 @@ struct BBox {
 };
 }  // namespace
 
-// LDS of the batched kernel: bin records, entries, lineout sums, planes.
-constexpr int kTileBins = 128, kTileEnts = 1024, kBinRecBytes = 32;
+// LDS of the batched kernel: bin records, staged entries, frame planes.
+constexpr int kTileBins = 256, kTileEnts = 2048, kBinRecBytes = 32, kWaves = 16, kTileInts = 14,
+              kStageVecs = 8;
 
+// Tiles of the batched kernel.  Bands of H cake rows; inside a band, tiles
+// are rectangles of rows x eta columns grown column by column while the
+// bounding box of their entries' pixels (the per-frame LDS plane) fits the
+// budget; a column that does not fit whole is split into row ranges, and a
+// bin that does not fit alone is "direct" (gathered from global memory).
+// Chunks are runs of consecutive tiles of one band of about `target`
+// entries: the work of one workgroup.
 int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   int rc = ensure_csr(m, mode);
   if (rc) return rc;
@@ -249,6 +257,8 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
       *out = t.get();
       return MC_OK;
     }
+  MC_CHECK(H == 16 || H == 32, MC_ERR_ARG, "rows per band %d not in {16, 32}", H);
+  MC_CHECK(B >= 1 && B <= 128, MC_ERR_ARG, "batch %d not in 1..128", B);
   const Csr &c = m->csr[mode];
   const int NY = m->ny, nE = m->n_eta, nR = m->n_r;
   const int64_t nb = m->nb();
@@ -256,11 +266,17 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   // 8-byte staging loads when every frame row starts 8-byte aligned.
   const int V = (in_size < 8 && ((int64_t)NY * in_size) % 8 == 0 && (m->npx() * in_size) % 8 == 0)
                     ? 8 / in_size : 1;
-  const int64_t fixed = (int64_t)kTileBins * kBinRecBytes + (int64_t)kTileEnts * 8 +
-                        (int64_t)H * 4 * B * 4;
-  const int64_t plane_cap = m->lds_bytes - fixed;  // bytes for B frame planes
-  MC_CHECK(plane_cap >= (int64_t)B * tl * 16, MC_ERR_ARG,
-           "LDS budget %d too small for batch %d", m->lds_bytes, B);
+  const int64_t fixed = (int64_t)kTileBins * kBinRecBytes + (int64_t)kTileEnts * 8;
+  const int64_t plane_cap = m->lds_bytes - fixed;  // bytes for the B frame planes
+  // Each wave stages frames wave, wave + kWaves, ...: (B / kWaves) frames of
+  // (kStageVecs >> fw_log) vectors per lane.
+  const int fw_log = B <= kWaves ? 0 : B <= 2 * kWaves ? 1 : B <= 4 * kWaves ? 2 : 3;
+  int stage_log = 0;
+  while ((1 << stage_log) < kStageVecs) stage_log++;
+  const int spf_log = stage_log - fw_log;
+  MC_CHECK(spf_log >= 0, MC_ERR_ARG, "batch %d needs more staging registers", B);
+  MC_CHECK(plane_cap >= (int64_t)B * tl * 64, MC_ERR_ARG, "LDS budget %d too small for batch %d",
+           m->lds_bytes, B);
   // Plane geometry of a pixel box: aligned start, power-of-two vector count.
   struct Box {
     int y0, z0, bwp, bh, ks, S;
@@ -282,99 +298,122 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
     if (ya + bwp > NY) ya = NY - bwp;
     if (ya < 0) return false;
     const int bh = bx.z1 - bx.z0 + 1;
-    int64_t S = (int64_t)bwp * bh;
+    int64_t S = (int64_t)bwp * bh + 1;  // + the zero pad element read by sentinels
     if (tl == 2) {
       while (S % 4 != 2) S++;
     } else if (S % 2 == 0) {
       S++;
     }
     o = {ya, bx.z0, bwp, bh, ks, (int)S, (int64_t)B * S * tl};
-    return o.bytes <= plane_cap;
+    // every staging vector of the batch in one register pass
+    return o.bytes <= plane_cap && (int64_t)nv * bh <= ((int64_t)64 << spf_log);
   };
-  // Per-bin pixel boxes.
+  // Per-bin pixel boxes; a bin that does not fit alone is direct.
   std::vector<BBox> bb(nb);
+  std::vector<char> big(nb, 0);
   parallel_for(nb, [&](int64_t a, int64_t b) {
-    for (int64_t q = a; q < b; q++)
+    Box tmp;
+    for (int64_t q = a; q < b; q++) {
       for (int32_t k = c.h_row[q]; k < c.h_row[q + 1]; k++)
         bb[q].add(c.h_pix[k] % NY, c.h_pix[k] / NY);
+      big[q] = !box_of(bb[q], tmp) || ((c.h_row[q + 1] - c.h_row[q] + 7) & ~7) > kTileEnts;
+    }
   });
   auto cnt = [&](int64_t b) { return (int64_t)(c.h_row[b + 1] - c.h_row[b]); };
+  // Staged entries of a bin are padded to whole groups of 8 (zero-weight
+  // entries on the plane's zero pad element).
+  auto cntp = [&](int64_t b) { return (cnt(b) + 7) & ~(int64_t)7; };
   struct TileH {
+    int ra = 0, nrows = 0, e0 = 0, nc = 0;
     BBox box;
-    int64_t ents = 0;
-    std::vector<int64_t> bins;
-    std::vector<char> direct;
+    int64_t ents = 0;  // staged entries
   };
-  const int n_blocks = (nR + H - 1) / H;
-  std::vector<std::vector<TileH>> blocks(n_blocks);
-  parallel_for(n_blocks, [&](int64_t a, int64_t bq) {
+  const int n_bands = (nR + H - 1) / H;
+  std::vector<std::vector<TileH>> bands(n_bands);
+  parallel_for(n_bands, [&](int64_t a, int64_t bq) {
     for (int64_t gi = a; gi < bq; gi++) {
-      auto &tl_ = blocks[gi];
-      const int row0 = (int)gi * H, r1 = std::min(nR, row0 + H);
-      TileH cur;
+      auto &tv = bands[gi];
+      const int row0 = (int)gi * H, nrows = std::min(H, nR - row0);
       Box tmp;
-      auto flush = [&]() {
-        if (!cur.bins.empty()) tl_.push_back(std::move(cur));
-        cur = TileH();
-      };
       auto room = [&](const TileH &T, const BBox &add, int64_t ents, int nbins) {
         BBox u = T.box;
         u.add(add);
-        return (int)T.bins.size() + nbins <= kTileBins && T.ents + ents <= kTileEnts &&
+        return (int64_t)T.nrows * T.nc + nbins <= kTileBins && T.ents + ents <= kTileEnts &&
                box_of(u, tmp);
+      };
+      TileH cur;
+      auto flush = [&]() {
+        if (cur.nc > 0) tv.push_back(cur);
+        cur = TileH();
       };
       for (int e = 0; e < nE; e++) {
         BBox col;
         int64_t ce = 0;
-        for (int r = row0; r < r1; r++) {
-          col.add(bb[(int64_t)r * nE + e]);
-          ce += cnt((int64_t)r * nE + e);
+        for (int j = 0; j < nrows; j++) {
+          const int64_t b = (int64_t)(row0 + j) * nE + e;
+          if (big[b]) continue;
+          col.add(bb[b]);
+          ce += cntp(b);
         }
-        const int nbins = r1 - row0;
-        if (!room(cur, col, ce, nbins)) flush();
-        if (room(cur, col, ce, nbins)) {
+        if (cur.nc > 0 && cur.nrows == nrows && room(cur, col, ce, nrows)) {
           cur.box.add(col);
           cur.ents += ce;
-          for (int r = row0; r < r1; r++) {
-            cur.bins.push_back((int64_t)r * nE + e);
-            cur.direct.push_back(0);
-          }
+          cur.nc++;
           continue;
         }
-        // The column alone does not fit: bin by bin, direct if needed.
-        for (int r = row0; r < r1; r++) {
-          const int64_t b = (int64_t)r * nE + e;
-          TileH solo;
-          if (!room(solo, bb[b], cnt(b), 1)) {
-            if ((int)cur.bins.size() + 1 > kTileBins) flush();
-            cur.bins.push_back(b);
-            cur.direct.push_back(1);
-            continue;
-          }
-          if (!room(cur, bb[b], cnt(b), 1)) flush();
-          cur.box.add(bb[b]);
-          cur.ents += cnt(b);
-          cur.bins.push_back(b);
-          cur.direct.push_back(0);
+        flush();
+        TileH fresh;
+        fresh.nrows = nrows;
+        if (room(fresh, col, ce, nrows)) {
+          cur = fresh;
+          cur.e0 = e;
+          cur.nc = 1;
+          cur.box = col;
+          cur.ents = ce;
+          continue;
         }
+        // The whole column does not fit: row ranges of one column each.
+        TileH piece;
+        piece.e0 = e;
+        piece.nc = 1;
+        for (int j = 0; j < nrows; j++) {
+          const int64_t b = (int64_t)(row0 + j) * nE + e;
+          const BBox add = big[b] ? BBox() : bb[b];
+          const int64_t ae = big[b] ? 0 : cntp(b);
+          TileH probe = piece;
+          probe.nc = 1;
+          if (piece.nrows > 0 && !room(probe, add, ae, 1)) {
+            tv.push_back(piece);
+            piece = TileH();
+            piece.e0 = e;
+            piece.nc = 1;
+            piece.ra = j;
+          }
+          piece.box.add(add);
+          piece.ents += ae;
+          piece.nrows++;
+        }
+        tv.push_back(piece);
       }
       flush();
     }
   });
-  // Chunks: runs of consecutive tiles of one row block with about
-  // target entries each (thousands of workgroups for a 2048^2 map).
-  const int64_t target = std::max<int64_t>(1024, std::min<int64_t>(8192, c.nnz / 4096));
-  std::vector<int32_t> chunks, block_chunk(n_blocks + 1, 0), tiles;
+  // Flatten.  Chunks: runs of consecutive tiles of one band with about
+  // `target` entries (a few thousand workgroups for a 2048^2 map).
+  const int64_t target = std::max<int64_t>(2048, c.nnz / 2048);
+  std::vector<int32_t> chunks, band_chunk(n_bands + 1, 0), tiles;
   std::vector<int4> tbins;
   std::vector<uint2> ents;
-  int64_t n_direct = 0, max_plane = 0, staged = 0;
-  for (int gi = 0; gi < n_blocks; gi++) {
-    const int row0 = gi * H;
-    int64_t acc = target;  // forces a new chunk at the block start
-    for (auto &T : blocks[gi]) {
+  tbins.reserve(nb);
+  ents.reserve(c.nnz);
+  int64_t n_direct = 0, max_plane = 0, staged = 0, real_ents = 0;
+  for (int gi = 0; gi < n_bands; gi++) {
+    const int row0 = gi * H, nrows_band = std::min(H, nR - row0);
+    int64_t acc = target;  // forces a new chunk at the band start
+    for (auto &T : bands[gi]) {
       if (acc >= target) {
-        chunks.insert(chunks.end(), {(int32_t)(tiles.size() / 10), (int32_t)(tiles.size() / 10),
-                                     row0, std::min(H, nR - row0)});
+        chunks.insert(chunks.end(), {(int32_t)(tiles.size() / kTileInts),
+                                     (int32_t)(tiles.size() / kTileInts), row0, nrows_band});
         acc = 0;
       }
       Box bx;
@@ -386,65 +425,78 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
         for (int32_t k = c.h_row[b]; k < c.h_row[b + 1]; k++) {
           const int p = c.h_pix[k];
           const uint32_t idx =
-              dir ? (uint32_t)p : (uint32_t)((p / NY - bx.z0) * bx.bwp + (p % NY - bx.y0));
+              dir ? (uint32_t)p
+                  : (uint32_t)(((p / NY - bx.z0) * bx.bwp + (p % NY - bx.y0)) * tl);
           uint32_t wb;
           memcpy(&wb, &c.h_w[k], 4);
           ents.push_back(make_uint2(idx, wb));
+          real_ents++;
         }
+        if (!dir)
+          while ((ents.size() - (size_t)e_beg) % 8)
+            ents.push_back(make_uint2((uint32_t)((bx.S - 1) * tl), 0u));
       };
-      // Staged entries first (contiguous), then those of direct bins.
-      std::vector<int4> recs(T.bins.size());
-      for (size_t q = 0; q < T.bins.size(); q++)
-        if (!T.direct[q]) {
-          const int64_t b = T.bins[q];
+      // Records row-major; staged entries first (contiguous), then direct.
+      std::vector<int4> recs((size_t)T.nrows * T.nc);
+      int64_t all_ents = 0;
+      for (int j = 0; j < T.nrows; j++)
+        for (int e = 0; e < T.nc; e++) {
+          const int64_t b = (int64_t)(row0 + T.ra + j) * nE + T.e0 + e;
+          all_ents += cnt(b);
+          if (big[b]) continue;
           const int32_t eb = (int32_t)ents.size();
           push_entries(b, false);
-          recs[q] = make_int4((int32_t)b, eb, (int32_t)ents.size(), (int)(b / nE) - row0);
+          recs[(size_t)j * T.nc + e] = make_int4((int32_t)b, eb, (int32_t)ents.size(), 0);
         }
       const int32_t e_end = (int32_t)ents.size();
-      for (size_t q = 0; q < T.bins.size(); q++)
-        if (T.direct[q]) {
-          const int64_t b = T.bins[q];
+      for (int j = 0; j < T.nrows; j++)
+        for (int e = 0; e < T.nc; e++) {
+          const int64_t b = (int64_t)(row0 + T.ra + j) * nE + T.e0 + e;
+          if (!big[b]) continue;
           const int32_t eb = (int32_t)ents.size();
           push_entries(b, true);
-          recs[q] = make_int4((int32_t)b, eb, (int32_t)ents.size(),
-                              ((int)(b / nE) - row0) | (1 << 30));
+          recs[(size_t)j * T.nc + e] = make_int4((int32_t)b, eb, (int32_t)ents.size(), 1);
           n_direct++;
         }
       tbins.insert(tbins.end(), recs.begin(), recs.end());
+      const int pv = bx.bwp > 0 ? (bx.bwp / V) * bx.bh : 0;
       tiles.insert(tiles.end(), {bx.y0, bx.z0, bx.bwp, bx.bh, bx.ks, bx.S, b_beg,
-                                 (int32_t)tbins.size(), e_beg, e_end});
-      chunks[chunks.size() - 3] = (int32_t)(tiles.size() / 10);
-      acc += T.ents;
+                                 (int32_t)tbins.size(), e_beg, e_end, T.ra, T.nc, pv, 0});
+      chunks[chunks.size() - 3] = (int32_t)(tiles.size() / kTileInts);
+      acc += all_ents;
     }
-    block_chunk[gi + 1] = (int32_t)(chunks.size() / 4);
+    band_chunk[gi + 1] = (int32_t)(chunks.size() / 4);
   }
+  MC_CHECK((int64_t)tbins.size() == nb, MC_ERR_STATE, "tile planner covered %lld of %lld bins",
+           (long long)tbins.size(), (long long)nb);
+  MC_CHECK(real_ents == c.nnz, MC_ERR_STATE, "tile planner placed %lld of %lld entries",
+           (long long)real_ents, (long long)c.nnz);
   auto t = std::make_unique<Tiles>();
   t->batch = B;
   t->rows = H;
   t->lds_bytes = m->lds_bytes;
   t->in_size = in_size;
   t->vec = V;
+  t->fpl = B > 64 ? 2 : 1;
+  t->spf_log = spf_log;
   t->n_chunks = (int)(chunks.size() / 4);
-  t->n_blocks = n_blocks;
+  t->n_bands = n_bands;
   t->lds_needed = ((size_t)fixed + (size_t)max_plane + 15) & ~(size_t)15;
-  t->n_tiles = (int64_t)tiles.size() / 10;
+  t->n_tiles = (int64_t)tiles.size() / kTileInts;
   t->n_tbins = (int64_t)tbins.size();
   t->n_ents = (int64_t)ents.size();
   t->n_direct = n_direct;
   t->staged_px = staged;
-  MC_CHECK(t->n_tbins == nb, MC_ERR_STATE, "tile planner covered %lld of %lld bins",
-           (long long)t->n_tbins, (long long)nb);
   if ((rc = t->chunks.alloc(chunks.size() * 4)) || (rc = t->tiles.alloc(tiles.size() * 4)) ||
       (rc = t->tbins.alloc(tbins.size() * 16)) || (rc = t->ents.alloc(ents.size() * 8)) ||
-      (rc = t->block_chunk.alloc(block_chunk.size() * 4)) ||
-      (rc = t->part.alloc((size_t)t->n_chunks * H * 4 * B * 4)))
+      (rc = t->band_chunk.alloc(band_chunk.size() * 4)) ||
+      (rc = t->part.alloc((size_t)t->n_chunks * H * 4 * 64 * t->fpl * 4)))
     return rc;
   MC_HIP(hipMemcpy(t->chunks.p, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->tiles.p, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->tbins.p, tbins.data(), tbins.size() * 16, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->ents.p, ents.data(), ents.size() * 8, hipMemcpyHostToDevice));
-  MC_HIP(hipMemcpy(t->block_chunk.p, block_chunk.data(), block_chunk.size() * 4,
+  MC_HIP(hipMemcpy(t->band_chunk.p, band_chunk.data(), band_chunk.size() * 4,
                    hipMemcpyHostToDevice));
   t->built = true;
   *out = t.get();
@@ -476,26 +528,25 @@ int ensure_binfo(Map *m, int mode, uint32_t flags, const float2 **out) {
     // Tables are read by kernels on other streams (the frame server).
     MC_HIP(hipStreamSynchronize(s));
     m->binfo_ok[key] = true;
+    m->binfo_version[key]++;
   }
   *out = m->d_binfo[key].as<float2>();
   return MC_OK;
 }
 
-// Identity frame slots 0..127 for contiguous batches.
-int identity_slots(Ctx *ctx, int32_t **out) {
-  static thread_local std::vector<std::pair<Ctx *, void *>> cache;
-  for (auto &e : cache)
-    if (e.first == ctx) {
-      *out = (int32_t *)e.second;
-      return MC_OK;
-    }
-  void *p = nullptr;
-  MC_HIP(hipMalloc(&p, 128 * 4));
-  int32_t h[128];
-  for (int i = 0; i < 128; i++) h[i] = i;
-  MC_HIP(hipMemcpy(p, h, sizeof(h), hipMemcpyHostToDevice));
-  cache.push_back({ctx, p});
-  *out = (int32_t *)p;
+// Tile records of t for the binfo table of (mode, flags); rebuilt when the
+// table was (a new dark, other flags).  Synchronous like the table itself:
+// the records are read by kernels on other streams (the frame server).
+static int ensure_recs(Map *m, hipStream_t s, Tiles *t, int mode, uint32_t flags,
+                       const float2 *binfo) {
+  const int key = mode * 4 + (int)flags;
+  if (t->recs.p && t->recs_binfo == binfo && t->recs_version == m->binfo_version[key])
+    return MC_OK;
+  int rc = launch_tile_recs(s, *m, *t, binfo);
+  if (rc) return rc;
+  MC_HIP(hipStreamSynchronize(s));
+  t->recs_binfo = binfo;
+  t->recs_version = m->binfo_version[key];
   return MC_OK;
 }
 
@@ -514,14 +565,19 @@ int run_integrate(Map *m, hipStream_t s, const void *frames_dev, int dtype, int 
   }
   Tiles *t = nullptr;
   if ((rc = ensure_tiles(m, mode, (int)dtype_size(dtype), &t))) return rc;
-  int32_t *slots;
-  if ((rc = identity_slots(m->ctx, &slots))) return rc;
+  // The staging loads of the batched kernel are 8-byte vectors when the
+  // rows allow it; a frame buffer that does not start 8-byte aligned (a
+  // view into a larger tensor) takes the one-frame kernel instead.
+  if (t->vec > 1 && ((uintptr_t)frames_dev & 7) != 0)
+    return launch_plain(s, *m, m->csr[mode], binfo, mask_nan, frames_dev, dtype, cakes_dev,
+                        lineouts_dev, n_frames, m->npx());
+  if ((rc = ensure_recs(m, s, t, mode, flags, binfo))) return rc;
   for (int f0 = 0; f0 < n_frames; f0 += t->batch) {
     const int nf = std::min(t->batch, n_frames - f0);
     const char *fb = (const char *)frames_dev + (size_t)f0 * m->npx() * es;
     float *cb = cakes_dev + (size_t)f0 * m->nb();
     float *lb = lineouts_dev ? lineouts_dev + (size_t)f0 * 3 * m->n_r : nullptr;
-    if ((rc = launch_batched(s, *m, *t, binfo, mask_nan, fb, dtype, nf, slots, cb, lb)))
+    if ((rc = launch_batched(s, *m, *t, binfo, mask_nan, fb, dtype, nf, cb, lb)))
       return rc;
   }
   return MC_OK;
@@ -769,7 +825,7 @@ int mc_map_info_get(mc_map *m, mc_map_info *info) {
     hb += c.row.bytes + c.pix.bytes + c.w.bytes + c.dterm.bytes;
     for (auto &t : m->tiles[q])
       hb += t->chunks.bytes + t->tiles.bytes + t->tbins.bytes + t->ents.bytes +
-            t->block_chunk.bytes + t->part.bytes;
+            t->band_chunk.bytes + t->part.bytes;
   }
   info->hbm_bytes = hb;
   for (int q = 0; q < 3; q++) info->skipped[q] = m->skipped[q];
@@ -863,8 +919,8 @@ int mc_map_set_tuning(mc_map *m, int32_t batch, int32_t rows_per_group, int32_t 
     m->batch = batch;
   }
   if (rows_per_group) {
-    MC_CHECK(rows_per_group >= 1 && rows_per_group <= 64, MC_ERR_ARG, "rows_per_group %d",
-             rows_per_group);
+    MC_CHECK(rows_per_group == 16 || rows_per_group == 32, MC_ERR_ARG,
+             "rows per band %d not in {16, 32}", rows_per_group);
     m->rows_per_group = rows_per_group;
   }
   if (lds_bytes) {
@@ -880,6 +936,23 @@ int mc_map_get_tuning(mc_map *m, int32_t *batch, int32_t *rows_per_group, int32_
   if (batch) *batch = m->batch;
   if (rows_per_group) *rows_per_group = m->rows_per_group;
   if (lds_bytes) *lds_bytes = m->lds_bytes;
+  return MC_OK;
+}
+
+int mc_map_layout_info(mc_map *m, int32_t mode, int32_t dtype, int64_t *out) {
+  MC_CHECK(m && out, MC_ERR_ARG, "null argument");
+  MC_CHECK(mode >= 0 && mode <= 2, MC_ERR_ARG, "mode %d", mode);
+  const size_t es = dtype_size(dtype);
+  MC_CHECK(es > 0, MC_ERR_ARG, "unsupported dtype %d", dtype);
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  std::lock_guard<std::mutex> lk2(m->mu);
+  MC_HIP(hipSetDevice(m->ctx->device));
+  Tiles *t = nullptr;
+  int rc = ensure_tiles(m, mode, (int)es, &t);
+  if (rc) return rc;
+  const int64_t v[8] = {t->n_tiles, t->n_chunks, t->n_direct, t->staged_px,
+                        t->n_ents,  (int64_t)t->lds_needed, t->n_bands, t->vec};
+  for (int i = 0; i < 8; i++) out[i] = v[i];
   return MC_OK;
 }
 
@@ -967,9 +1040,9 @@ int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype, int32_t 
   if ((rc = ensure_binfo(m, mode, flags, &binfo))) return rc;
   const int mask_nan = (flags & MC_F_MASK_NAN) ? 1 : 0;
   Tiles *t = nullptr;
-  if (kind == 1 && (rc = ensure_tiles(m, mode, (int)dtype_size(dtype), &t))) return rc;
-  int32_t *slots;
-  if ((rc = identity_slots(m->ctx, &slots))) return rc;
+  if (kind == 1 && ((rc = ensure_tiles(m, mode, (int)dtype_size(dtype), &t)) ||
+                    (rc = ensure_recs(m, s, t, mode, flags, binfo))))
+    return rc;
   const size_t es = dtype_size(dtype);
   auto one_iter = [&](std::vector<hipEvent_t> *ev) -> int {
     const int step = kind == 1 ? t->batch : 1;
@@ -979,10 +1052,9 @@ int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype, int32_t 
       float *cb = cakes_dev + (size_t)f0 * m->nb();
       float *lb = lineouts_dev ? lineouts_dev + (size_t)f0 * 3 * m->n_r : nullptr;
       if (ev) MC_HIP(hipEventRecord((*ev)[2 * (f0 / step)], s));
-      int r2 = kind == 1
-                   ? launch_batched(s, *m, *t, binfo, mask_nan, fb, dtype, nf, slots, cb, lb)
-                   : launch_plain(s, *m, m->csr[mode], binfo, mask_nan, fb, dtype, cb, lb, 1,
-                                  m->npx());
+      int r2 = kind == 1 ? launch_batched(s, *m, *t, binfo, mask_nan, fb, dtype, nf, cb, lb)
+                         : launch_plain(s, *m, m->csr[mode], binfo, mask_nan, fb, dtype, cb, lb,
+                                        1, m->npx());
       if (r2) return r2;
       if (ev) MC_HIP(hipEventRecord((*ev)[2 * (f0 / step) + 1], s));
     }
@@ -1036,7 +1108,7 @@ int mc_map_autotune(mc_map *m, int32_t dtype, double *best_fps) {
       (rc = cakes.alloc((size_t)128 * m->nb() * 4)))
     return rc;
   MC_HIP(hipMemset(frames.p, 0x11, frames.bytes));
-  const int Bs[] = {32, 64, 128}, Hs[] = {4, 8, 16}, Ls[] = {65536, 98304};
+  const int Bs[] = {32, 64, 128}, Hs[] = {16, 32}, Ls[] = {81920, 163840};
   double best = -1;
   int bb = keep_b, bh = keep_h, bl = keep_l;
   for (int B : Bs)

@@ -26,6 +26,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "mc_internal.h"
 
@@ -172,27 +173,45 @@ __global__ __launch_bounds__(256) void k_profile(
 }
 
 // ---------------------------------------------------------------------
-// Batched kernel.  A workgroup takes one chunk: a run of consecutive tiles
-// of one row block (rows r0..r0+H-1).  For each tile it stages into LDS
-//   * the tile's bin records (cake bin, entry range, per-bin scale/dark,
-//     in-bounds area),
-//   * the tile's map entries (pixel index inside the tile box, weight),
-//   * the pixel box of every frame of the batch, one plane per frame
-//     (plane stride S elements, chosen so lanes reading the same pixel of
+// Batched kernel.  The cake rows are cut into bands of 16*G rows and a band
+// into tiles: rectangles of rows x eta columns whose pixel footprint (the
+// bounding box of their map entries, padded to whole staging vectors) fits
+// the LDS plane budget.  A workgroup of 16 waves takes one chunk -- a run of
+// consecutive tiles of one band -- and for each tile stages into LDS
+//   * the tile's bin records (cake bin, entry range, scale/dark, area),
+//   * its map entries (pixel index inside the box, weight),
+//   * the box of every frame of the batch, one plane per frame (plane
+//     stride S elements, so that lanes reading the same pixel of
 //     consecutive frames hit different LDS banks),
-// then reduces every bin with lane = frame: one entry read (an LDS
-// broadcast) serves all frames of the batch.  Bins whose pixel box does not
-// fit the LDS budget ("direct" bins) gather from global memory instead.
-// Lineout sums per (row, frame) are kept in LDS and written as one partial
-// per chunk; k_lineout_finalize adds the partials of each row block.
-//   chunk: tile_begin tile_end row0 n_rows
-//   tile : y0 z0 bwp bh kshift S b_beg b_end e_beg e_end
-//          (bwp = padded box width = nv * V, nv = 1 << kshift vector loads)
-//   tbin : cake bin, ent_begin, ent_end, (row in block) | (direct << 30)
-//   ent  : (pixel index in the tile box, or global pixel if direct), weight
+// then reduces the bins with lane = frame: one entry read (an LDS
+// broadcast) serves every frame of the batch.  Wave w owns the band rows
+// j = w (mod 16), so it keeps the lineout sums of its G rows in registers
+// across the whole chunk and writes them once at the end;
+// k_lineout_finalize adds the chunks of each band.  Bins whose own
+// footprint exceeds the budget ("direct" bins) gather from global memory.
+//   chunk: tile_begin tile_end band_row0 band_rows
+//   tile : y0 z0 bwp bh kshift S b_beg b_end e_beg e_end ra nc pv 0
+//          (box of bwp x bh pixels at (y0, z0), bwp = (1 << kshift) * V;
+//          plane stride S elements with a zero pad element at S-1;
+//          band rows ra .. ra + (b_end - b_beg) / nc - 1, nc columns,
+//          records row-major; pv = staging vectors per frame plane)
+//   rec  : cake bin, ent_begin, ent_end, direct flag, scale, dark term,
+//          in-bounds area
+//   ent  : (byte offset of the pixel in its box plane, or global pixel if
+//          direct), weight
+// The loads of tile t+1 (this thread's share of the boxes, records and
+// entries, kStageVecs staging vectors at most) are issued into registers
+// before the waves reduce tile t, so global latency hides behind the
+// reduction.
+// Workgroup b takes chunk (b % 8) * per + b / 8: hardware hands
+// consecutive workgroups to the 8 XCDs in turn, so each XCD works through
+// one contiguous range of bands and neighbouring bands share its L2.
 // ---------------------------------------------------------------------
-constexpr int kTileBins = 128;    // bin records per tile (LDS)
-constexpr int kTileEnts = 1024;   // map entries per tile (LDS)
+constexpr int kWaves = 16;
+constexpr int kTileBins = 256;    // bin records per tile (LDS)
+constexpr int kTileEnts = 2048;   // staged map entries per tile (LDS)
+constexpr int kTileInts = 14;
+constexpr int kStageVecs = 8;     // staging vectors per thread and tile
 
 struct BinRec {
   int4 tb;
@@ -205,156 +224,355 @@ template <typename TIn, int V> struct alignas(sizeof(TIn) * V) VecLoad {
   TIn v[V];
 };
 
-template <typename TIn, typename TL, int B, int V>
-__global__ __launch_bounds__(256) void k_batched(
-    const int32_t *__restrict__ chunks, const int32_t *__restrict__ tiles,
-    const int4 *__restrict__ tbins, const uint2 *__restrict__ ents,
-    const float2 *__restrict__ binfo, const float *__restrict__ a_inb,
-    const TIn *__restrict__ frames, int64_t frame_stride,
-    const int32_t *__restrict__ slots, int n_frames, int ny, int rows_per_group,
-    int64_t nb, float *__restrict__ cakes, float *__restrict__ part) {
-  extern __shared__ __attribute__((aligned(16))) char lds_raw[];
-  constexpr int SPW = B >= 64 ? 1 : 64 / B;  // bins per wave at once
-  constexpr int FPL = B > 64 ? B / 64 : 1;    // frames per lane
-  constexpr int U = 8;                        // staging loads in flight per lane
-  BinRec *lbin = reinterpret_cast<BinRec *>(lds_raw);
-  uint2 *lent = reinterpret_cast<uint2 *>(lds_raw + kTileBins * sizeof(BinRec));
-  float *lo = reinterpret_cast<float *>(lds_raw + kTileBins * sizeof(BinRec) +
-                                        kTileEnts * sizeof(uint2));
-  TL *plane = reinterpret_cast<TL *>(reinterpret_cast<char *>(lo) +
-                                     (size_t)rows_per_group * 4 * B * sizeof(float));
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int slot = SPW > 1 ? lane / B : 0;
-  const int f0 = SPW > 1 ? lane % B : lane * FPL;
-  const int c = blockIdx.x;
-  const int t_beg = chunks[4 * c], t_end = chunks[4 * c + 1];
-  const bool want_lo = part != nullptr;
-  if (want_lo)
-    for (int i = tid; i < rows_per_group * 4 * B; i += 256) lo[i] = 0.f;
+// Cake values are written per lane (frame) in whole granules of LG
+// consecutive bins: each row slot of a wave keeps the granule it is
+// filling in a register vector (the position inside the granule is a
+// run-time index, which the hardware's register indexing serves) and
+// stores it when its next bin falls in another granule.  Storing a frame's
+// bins one at a time would leave partially written lines for the L2 to
+// evict (a 16-byte write per 4-byte value).
+template <int LG> struct GranVec;
+template <> struct GranVec<16> { typedef float type __attribute__((ext_vector_type(16))); };
+template <> struct GranVec<8> { typedef float type __attribute__((ext_vector_type(8))); };
+typedef float f4v __attribute__((ext_vector_type(4)));
 
-  for (int t = t_beg; t < t_end; ++t) {
-    const int32_t *T = tiles + 10 * t;
-    const int y0 = T[0], z0 = T[1], bwp = T[2], bh = T[3], ks = T[4], S = T[5];
-    const int b_beg = T[6], nbt = T[7] - T[6], e_beg = T[8], net = T[9] - T[8];
-    __syncthreads();
-    // ---- bin records and entries ----
-    for (int i = tid; i < nbt; i += 256) {
-      BinRec r;
-      r.tb = tbins[b_beg + i];
-      r.info = binfo[r.tb.x];
-      r.a = a_inb[r.tb.x];
-      r.pad = 0.f;
-      lbin[i] = r;
+template <int LG, typename VT>
+__device__ __forceinline__ void flush_granule(const VT &v, int idx, uint32_t mask, int f,
+                                              float *__restrict__ cakes, int64_t nb,
+                                              bool vec_ok) {
+  float *dst = cakes + (int64_t)f * nb + (int64_t)idx * LG;
+  if (mask == ((1u << LG) - 1u) && vec_ok) {
+#pragma unroll
+    for (int q = 0; q < LG; q += 4) {
+      f4v w;
+      w.x = v[q];
+      w.y = v[q + 1];
+      w.z = v[q + 2];
+      w.w = v[q + 3];
+      *reinterpret_cast<f4v *>(dst + q) = w;
     }
-    for (int k = tid; k < net; k += 256) lent[k] = ents[e_beg + k];
-    // ---- pixel boxes of all frames: plane f holds the box row-major ----
-    if (bwp > 0) {
-      const int nv = 1 << ks;
-      const int rp = 64 >> ks;  // box rows per wave instruction
-      const int cv = lane & (nv - 1), lr = lane >> ks;
-      const int passes = (bh + rp - 1) / rp;
-      const int jobs = n_frames * passes;
-      for (int j0 = wave; j0 < jobs; j0 += 4 * U) {
-        VecLoad<TIn, V> v[U];
-        int dst[U];
+  } else {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int j = j0 + 4 * u;
-          dst[u] = -1;
-          if (j < jobs) {
-            const int f = j / passes, rr = (j - f * passes) * rp + lr;
-            if (rr < bh) {
-              const TIn *src = frames + (int64_t)slots[f] * frame_stride +
-                               (int64_t)(z0 + rr) * ny + y0 + cv * V;
-              v[u] = *reinterpret_cast<const VecLoad<TIn, V> *>(src);
-              dst[u] = f * S + rr * bwp + cv * V;
-            }
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-          if (dst[u] >= 0) {
-#pragma unroll
-            for (int q = 0; q < V; ++q) plane[dst[u] + q] = static_cast<TL>(v[u].v[q]);
-          }
-      }
-    }
-    __syncthreads();
-    // ---- every wave reduces its share of the tile's bins ----
-    for (int bi = wave * SPW + slot; bi < nbt; bi += 4 * SPW) {
-      const BinRec r = lbin[bi];
-      const bool direct = (r.tb.w >> 30) & 1;
-      const int rib = r.tb.w & 0x3fffffff;
-      Kahan acc[FPL];
-      if (!direct) {
-        for (int k = r.tb.y - e_beg; k < r.tb.z - e_beg; ++k) {
-          const uint2 e = lent[k];
-          const float wk = __uint_as_float(e.y);
-#pragma unroll
-          for (int j = 0; j < FPL; ++j)
-            acc[j].add(to_f(plane[(f0 + j) * S + (int)e.x]) * wk);
-        }
-      } else {
-        for (int k = r.tb.y; k < r.tb.z; ++k) {  // entries stay in global memory
-          const uint2 e = ents[k];
-          const float wk = __uint_as_float(e.y);
-#pragma unroll
-          for (int j = 0; j < FPL; ++j) {
-            const int f = f0 + j;
-            if (f < n_frames)
-              acc[j].add(to_f(frames[(int64_t)slots[f] * frame_stride + e.x]) * wk);
-          }
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < FPL; ++j) {
-        const int f = f0 + j;
-        if (f < n_frames) {
-          const float val = (acc[j].value() - r.info.y) * fabsf(r.info.x);
-          const float o = flagged(r.info.x) ? NAN : val;
-          cakes[(int64_t)f * nb + r.tb.x] = o;
-          if (want_lo) {
-            float *L = lo + (size_t)rib * 4 * B;
-            if (o != 0.f && !isnan(o)) {
-              atomicAdd(&L[0 * B + f], o);
-              atomicAdd(&L[1 * B + f], 1.f);
-            }
-            if (r.a > 1e-9f && !isnan(o)) {
-              atomicAdd(&L[2 * B + f], o * r.a);
-              atomicAdd(&L[3 * B + f], o);
-            }
-          }
-        }
-      }
-    }
-  }
-  if (want_lo) {
-    __syncthreads();
-    float *P = part + (int64_t)c * rows_per_group * 4 * B;
-    for (int i = tid; i < rows_per_group * 4 * B; i += 256) P[i] = lo[i];
+    for (int q = 0; q < LG; ++q)
+      if ((mask >> q) & 1u) dst[q] = v[q];
   }
 }
 
+// Registers of one tile's prefetch: this thread's share of the frame
+// boxes (kStageVecs staging vectors), of the bin records and entries.
+typedef int i4v __attribute__((ext_vector_type(4)));
+typedef unsigned u2v __attribute__((ext_vector_type(2)));
+
+template <typename TIn, int V> struct TilePrefetch {
+  VecLoad<TIn, V> v[kStageVecs];
+  i4v r0, r1;  // one 32-byte bin record
+  u2v e[kTileEnts / (64 * kWaves)];
+};
+
+struct TileHdr {
+  int y0, z0, bwp, bh, ks, S, b_beg, nbt, e_beg, net, ra, nc, pv;
+};
+
+__device__ __forceinline__ TileHdr tile_hdr(const int32_t *__restrict__ tiles, int t) {
+  const int32_t *T = tiles + kTileInts * t;
+  TileHdr h;
+  h.y0 = T[0];
+  h.z0 = T[1];
+  h.bwp = T[2];
+  h.bh = T[3];
+  h.ks = T[4];
+  h.S = T[5];
+  h.b_beg = T[6];
+  h.nbt = T[7] - T[6];
+  h.e_beg = T[8];
+  h.net = T[9] - T[8];
+  h.ra = T[10];
+  h.nc = T[11];
+  h.pv = T[12];
+  return h;
+}
+
+// Staging slot u of a thread: frame wave + kWaves * (u >> spf_log) (the
+// wave's frames), vector q = lane + 64 * (u & (spf - 1)) of that frame's
+// box.  Boxes are at most 64 vectors wide, so a lane keeps its column.
+template <typename TIn, int V>
+__device__ __forceinline__ void tile_issue(TilePrefetch<TIn, V> &P, const TileHdr &h, int tid,
+                                           int lane, int wave, int spf_log,
+                                           const BinRec *__restrict__ recs,
+                                           const uint2 *__restrict__ ents,
+                                           const TIn *__restrict__ frames, int64_t frame_stride,
+                                           int n_frames, int ny) {
+  if (tid < h.nbt) {
+    const i4v *rp = reinterpret_cast<const i4v *>(recs + h.b_beg + tid);
+    P.r0 = rp[0];
+    P.r1 = rp[1];
+  }
+#pragma unroll
+  for (int q = 0; q < kTileEnts / (64 * kWaves); ++q) {
+    const int k = tid + 64 * kWaves * q;
+    if (k < h.net) P.e[q] = reinterpret_cast<const u2v *>(ents)[h.e_beg + k];
+  }
+  if (h.bwp > 0) {
+    const int cv = lane & ((1 << h.ks) - 1), r0 = lane >> h.ks, rstep = 64 >> h.ks;
+    const TIn *col = frames + (int64_t)(h.z0 + r0) * ny + h.y0 + cv * V;
+#pragma unroll
+    for (int u = 0; u < kStageVecs; ++u) {
+      const int f = wave + kWaves * (u >> spf_log);
+      const int s = u & ((1 << spf_log) - 1);
+      if (f < n_frames && lane + 64 * s < h.pv)
+        P.v[u] = *reinterpret_cast<const VecLoad<TIn, V> *>(
+            col + (int64_t)f * frame_stride + (int64_t)(s * rstep) * ny);
+    }
+  }
+}
+
+template <typename TIn, typename TL, int V>
+__device__ __forceinline__ void tile_commit(const TilePrefetch<TIn, V> &P, const TileHdr &h,
+                                            int tid, int lane, int wave, int spf_log,
+                                            int n_frames, BinRec *lbin, uint2 *lent, TL *plane) {
+  if (tid < h.nbt) {
+    i4v *rp = reinterpret_cast<i4v *>(lbin + tid);
+    rp[0] = P.r0;
+    rp[1] = P.r1;
+  }
+#pragma unroll
+  for (int q = 0; q < kTileEnts / (64 * kWaves); ++q) {
+    const int k = tid + 64 * kWaves * q;
+    if (k < h.net) reinterpret_cast<u2v *>(lent)[k] = P.e[q];
+  }
+  // Zero pad element S-1 of every plane: the padding entries of each bin
+  // (weight 0) point at it.
+  if (tid < n_frames) plane[tid * h.S + h.S - 1] = static_cast<TL>(0);
+  if (h.bwp > 0) {
+    const int cv = lane & ((1 << h.ks) - 1), r0 = lane >> h.ks, rstep = 64 >> h.ks;
+    const int colo = r0 * h.bwp + cv * V;
+#pragma unroll
+    for (int u = 0; u < kStageVecs; ++u) {
+      const int f = wave + kWaves * (u >> spf_log);
+      const int s = u & ((1 << spf_log) - 1);
+      if (f < n_frames && lane + 64 * s < h.pv) {
+        const int dst = f * h.S + colo + s * rstep * h.bwp;
+        if constexpr (sizeof(TL) == 2 && V % 2 == 0) {
+          // pairs of 16-bit values as 32-bit words (dst is even)
+#pragma unroll
+          for (int q = 0; q < V; q += 2)
+            *reinterpret_cast<uint32_t *>(plane + dst + q) =
+                (uint32_t)(uint16_t)P.v[u].v[q] | ((uint32_t)(uint16_t)P.v[u].v[q + 1] << 16);
+        } else {
+#pragma unroll
+          for (int q = 0; q < V; ++q) plane[dst + q] = static_cast<TL>(P.v[u].v[q]);
+        }
+      }
+    }
+  }
+}
+
+// Compensated sum: products are added in blocks of 8 with plain FMAs and
+// each block sum enters a Kahan accumulator (relative error about 8 ulp of
+// a block plus 2 ulp, independent of the entry count).
+__device__ __forceinline__ void kahan_add(Kahan &k, float x) {
+  const float y = x - k.c;
+  const float t = k.s + y;
+  k.c = (t - k.s) - y;
+  k.s = t;
+}
+
+template <typename TIn, typename TL, int FPL, int V, int G>
+__global__ __launch_bounds__(64 * kWaves) void k_batched(
+    const int32_t *__restrict__ chunks, int n_chunks, const int32_t *__restrict__ tiles,
+    const BinRec *__restrict__ recs, const uint2 *__restrict__ ents,
+    const TIn *__restrict__ frames, int64_t frame_stride, int n_frames, int spf_log, int ny,
+    int64_t nb, float *__restrict__ cakes, float *__restrict__ part) {
+  extern __shared__ __attribute__((aligned(16))) char lds_raw[];
+  constexpr int NP = 64 * FPL;  // planes (frames) per launch
+  constexpr int RB = kWaves * G;
+  BinRec *lbin = reinterpret_cast<BinRec *>(lds_raw);
+  uint2 *lent = reinterpret_cast<uint2 *>(lds_raw + kTileBins * sizeof(BinRec));
+  TL *plane = reinterpret_cast<TL *>(lds_raw + kTileBins * sizeof(BinRec) +
+                                     kTileEnts * sizeof(uint2));
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int per = (n_chunks + 7) / 8;
+  const int c = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (c >= n_chunks) return;
+  const int t_beg = chunks[4 * c], t_end = chunks[4 * c + 1];
+  const bool want_lo = part != nullptr;
+  constexpr int LG = 16 / FPL;
+  typedef typename GranVec<LG>::type VT;
+  float lo[G][4][FPL];
+  VT ob[G][FPL];
+  int og[G];
+  uint32_t om[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    og[g] = -1;
+    om[g] = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int j = 0; j < FPL; ++j) lo[g][k][j] = 0.f;
+#pragma unroll
+    for (int j = 0; j < FPL; ++j) ob[g][j] = 0.f;
+  }
+  const bool vec_ok = ((nb & 3) == 0) && (((uintptr_t)cakes & 15) == 0);
+
+  // Software pipeline: the loads of tile t+1 are in flight while the waves
+  // reduce tile t.
+  TilePrefetch<TIn, V> P;
+  TileHdr hn = tile_hdr(tiles, t_beg);
+  tile_issue(P, hn, tid, lane, wave, spf_log, recs, ents, frames, frame_stride, n_frames, ny);
+  for (int t = t_beg; t < t_end; ++t) {
+    const TileHdr h = hn;
+    __syncthreads();  // every wave is done with the previous tile's LDS
+    tile_commit<TIn, TL, V>(P, h, tid, lane, wave, spf_log, n_frames, lbin, lent, plane);
+    __syncthreads();
+    if (t + 1 < t_end) {
+      hn = tile_hdr(tiles, t + 1);
+      tile_issue(P, hn, tid, lane, wave, spf_log, recs, ents, frames, frame_stride, n_frames,
+                 ny);
+    }
+    const int nrows = h.nbt / h.nc;
+    // Byte address of this lane's plane(s); entries hold byte offsets.
+    // Lanes past the batch read the last staged plane; their sums are
+    // dropped.
+    const char *pb[FPL];
+#pragma unroll
+    for (int j = 0; j < FPL; ++j)
+      pb[j] = reinterpret_cast<const char *>(plane) +
+              (size_t)min(lane + 64 * j, n_frames - 1) * h.S * sizeof(TL);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int jr = kWaves * g + wave - h.ra;  // record row inside the tile
+      if (jr < 0 || jr >= nrows) continue;
+      const BinRec *rrow = lbin + jr * h.nc;
+      BinRec rn = rrow[0];
+      for (int e = 0; e < h.nc; ++e) {
+        const BinRec r = rn;
+        if (e + 1 < h.nc) rn = rrow[e + 1];  // next record early: hides its LDS latency
+        Kahan acc[FPL];
+        if (!r.tb.w) {
+          // Staged entries come in whole groups of 8 (zero-weight padding on
+          // the plane's zero pad element), 16-byte aligned in LDS.
+          const int kb = r.tb.y - h.e_beg, ke = r.tb.z - h.e_beg;
+          for (int k = kb; k < ke; k += 8) {
+            uint2 en[8];
+            const uint4 *ep = reinterpret_cast<const uint4 *>(lent + k);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const uint4 two = ep[q];
+              en[2 * q] = make_uint2(two.x, two.y);
+              en[2 * q + 1] = make_uint2(two.z, two.w);
+            }
+            TL pv[8][FPL];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+#pragma unroll
+              for (int j = 0; j < FPL; ++j)
+                pv[q][j] = *reinterpret_cast<const TL *>(pb[j] + en[q].x);
+            float gs[FPL];
+#pragma unroll
+            for (int j = 0; j < FPL; ++j) gs[j] = to_f(pv[0][j]) * __uint_as_float(en[0].y);
+#pragma unroll
+            for (int q = 1; q < 8; ++q)
+#pragma unroll
+              for (int j = 0; j < FPL; ++j)
+                gs[j] = fmaf(to_f(pv[q][j]), __uint_as_float(en[q].y), gs[j]);
+#pragma unroll
+            for (int j = 0; j < FPL; ++j) kahan_add(acc[j], gs[j]);
+          }
+        } else {
+          for (int k = r.tb.y; k < r.tb.z; ++k) {  // entries stay in global memory
+            const uint2 en = ents[k];
+#pragma unroll
+            for (int j = 0; j < FPL; ++j) {
+              const int f = lane + 64 * j;
+              if (f < n_frames)
+                kahan_add(acc[j],
+                          to_f(frames[(int64_t)f * frame_stride + en.x]) * __uint_as_float(en.y));
+            }
+          }
+        }
+        const int gr = r.tb.x / LG, gk = r.tb.x - gr * LG;
+        if (gr != og[g]) {
+          if (om[g])
+#pragma unroll
+            for (int j = 0; j < FPL; ++j)
+              if (lane + 64 * j < n_frames)
+                flush_granule<LG>(ob[g][j], og[g], om[g], lane + 64 * j, cakes, nb, vec_ok);
+          og[g] = gr;
+          om[g] = 0u;
+        }
+        om[g] |= 1u << gk;
+#pragma unroll
+        for (int j = 0; j < FPL; ++j) {
+          const float val = (acc[j].value() - r.info.y) * fabsf(r.info.x);
+          const float o = flagged(r.info.x) ? NAN : val;
+          ob[g][j][gk] = o;
+          if (o != 0.f && !isnan(o)) {
+            lo[g][0][j] += o;
+            lo[g][1][j] += 1.f;
+          }
+          if (r.a > 1e-9f && !isnan(o)) {
+            lo[g][2][j] += o * r.a;
+            lo[g][3][j] += o;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    if (om[g])
+#pragma unroll
+      for (int j = 0; j < FPL; ++j)
+        if (lane + 64 * j < n_frames)
+          flush_granule<LG>(ob[g][j], og[g], om[g], lane + 64 * j, cakes, nb, vec_ok);
+    if (want_lo) {
+      // part[c][row][k][NP] for the band row this slot holds.
+      float *Pp = part + ((int64_t)c * RB + kWaves * g + wave) * 4 * NP;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < FPL; ++j) Pp[k * NP + lane + 64 * j] = lo[g][k][j];
+    }
+  }
+}
+
+// Tile bin records with the per-bin scale/dark and in-bounds area of the
+// current (mode, flags, dark), so the kernel reads each record once.
+__global__ void k_tile_recs(const int4 *__restrict__ tbins, const float2 *__restrict__ binfo,
+                            const float *__restrict__ a_inb, int64_t n, BinRec *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  BinRec r;
+  r.tb = tbins[i];
+  r.info = binfo[r.tb.x];
+  r.a = a_inb[r.tb.x];
+  r.pad = 0.f;
+  out[i] = r;
+}
+
 // Lineouts from the per-chunk partial sums: one thread per (frame, row);
-// the chunks of a row block are [block_chunk[b], block_chunk[b+1]).
+// the chunks of band b are [band_chunk[b], band_chunk[b+1]).
 __global__ void k_lineout_finalize(const float *__restrict__ part,
-                                   const int32_t *__restrict__ block_chunk,
+                                   const int32_t *__restrict__ band_chunk,
                                    const float4 *__restrict__ row_static, int n_frames, int n_r,
-                                   int rows_per_group, int B, int mask_nan,
+                                   int band_rows, int np, int mask_nan,
                                    float *__restrict__ lineouts) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)n_frames * n_r) return;
-  const int f = (int)(i / n_r), r = (int)(i % n_r);
-  const int blk = r / rows_per_group, rib = r - blk * rows_per_group;
+  // Consecutive threads take consecutive frames of one row: the partials
+  // of a (chunk, row) are frame-contiguous.
+  const int r = (int)(i / n_frames), f = (int)(i % n_frames);
+  const int band = r / band_rows, rib = r - band * band_rows;
   float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int c = block_chunk[blk]; c < block_chunk[blk + 1]; ++c) {
-    const float *L = part + ((int64_t)c * rows_per_group + rib) * 4 * B;
+  for (int c = band_chunk[band]; c < band_chunk[band + 1]; ++c) {
+    const float *L = part + ((int64_t)c * band_rows + rib) * 4 * np;
     s.x += L[f];
-    s.y += L[B + f];
-    s.z += L[2 * B + f];
-    s.w += L[3 * B + f];
+    s.y += L[np + f];
+    s.z += L[2 * np + f];
+    s.w += L[3 * np + f];
   }
   write_lineouts(s, row_static[r], mask_nan, lineouts + (int64_t)f * 3 * n_r, n_r, r);
 }
@@ -488,62 +706,103 @@ int launch_profile(hipStream_t s, const Map &m, const float *cakes, int n_frames
   return MC_OK;
 }
 
-template <typename TIn, int B, int V>
+int launch_tile_recs(hipStream_t s, const Map &m, Tiles &t, const float2 *binfo) {
+  if (!t.recs.p) {
+    int rc = t.recs.alloc((size_t)t.n_tbins * sizeof(BinRec));
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_tile_recs, dim3(blocks_for(t.n_tbins, 256)), dim3(256), 0, s,
+                     t.tbins.as<int4>(), binfo, m.d_area_inb.as<float>(), t.n_tbins,
+                     t.recs.as<BinRec>());
+  MC_HIP(hipGetLastError());
+  return MC_OK;
+}
+
+template <typename TIn, int FPL, int V, int G>
 static int launch_batched_v(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
-                            int mask_nan, const void *frames, int n_frames,
-                            const int32_t *slots, float *cakes, float *lineouts) {
+                            int mask_nan, const void *frames, int n_frames, float *cakes,
+                            float *lineouts) {
   using TL = typename std::conditional<sizeof(TIn) <= 2, uint16_t, float>::type;
-  auto kern = k_batched<TIn, TL, B, V>;
+  auto kern = k_batched<TIn, TL, FPL, V, G>;
+  if (V > 1 && ((uintptr_t)frames % (sizeof(TIn) * V)) != 0) {
+    set_error("frame buffer not %d-byte aligned for vector staging", (int)(sizeof(TIn) * V));
+    return MC_ERR_ARG;
+  }
+  if (n_frames < 1 || n_frames > t.batch) {
+    set_error("batch of %d frames outside 1..%d", n_frames, t.batch);
+    return MC_ERR_ARG;
+  }
   const size_t lds = t.lds_needed;
   if (lds > 65536)
     MC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
   float *part = lineouts ? t.part.as<float>() : nullptr;
-  hipLaunchKernelGGL(kern, dim3(t.n_chunks), dim3(256), lds, s, t.chunks.as<int32_t>(),
-                     t.tiles.as<int32_t>(), t.tbins.as<int4>(), t.ents.as<uint2>(), binfo,
-                     m.d_area_inb.as<float>(), (const TIn *)frames, m.npx(), slots, n_frames,
-                     m.ny, t.rows, m.nb(), cakes, part);
+  const unsigned grid = 8u * (unsigned)((t.n_chunks + 7) / 8);
+  (void)binfo;  // folded into the tile records (launch_tile_recs)
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kWaves), lds, s, t.chunks.as<int32_t>(),
+                     t.n_chunks, t.tiles.as<int32_t>(), t.recs.as<BinRec>(), t.ents.as<uint2>(),
+                     (const TIn *)frames, m.npx(), n_frames, t.spf_log, m.ny, m.nb(), cakes, part);
   MC_HIP(hipGetLastError());
   if (lineouts) {
     const int64_t n = (int64_t)n_frames * m.n_r;
     hipLaunchKernelGGL(k_lineout_finalize, dim3(blocks_for(n, 256)), dim3(256), 0, s, part,
-                       t.block_chunk.as<int32_t>(), m.d_row_static.as<float4>(), n_frames,
-                       m.n_r, t.rows, B, mask_nan, lineouts);
+                       t.band_chunk.as<int32_t>(), m.d_row_static.as<float4>(), n_frames,
+                       m.n_r, t.rows, 64 * FPL, mask_nan, lineouts);
     MC_HIP(hipGetLastError());
   }
   return MC_OK;
 }
 
-template <typename TIn, int B>
+template <typename TIn, int FPL, int V>
+static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
+                            int mask_nan, const void *frames, int n_frames, float *cakes,
+                            float *lineouts) {
+  switch (t.rows / kWaves) {
+    case 1:
+      return launch_batched_v<TIn, FPL, V, 1>(s, m, t, binfo, mask_nan, frames, n_frames, cakes,
+                                              lineouts);
+    case 2:
+      return launch_batched_v<TIn, FPL, V, 2>(s, m, t, binfo, mask_nan, frames, n_frames, cakes,
+                                              lineouts);
+    default:
+      set_error("rows per band %d not in {16, 32}", t.rows);
+      return MC_ERR_ARG;
+  }
+}
+
+template <typename TIn, int FPL>
 static int launch_batched_t(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
-                            int mask_nan, const void *frames, int n_frames,
-                            const int32_t *slots, float *cakes, float *lineouts) {
+                            int mask_nan, const void *frames, int n_frames, float *cakes,
+                            float *lineouts) {
   constexpr int VV = sizeof(TIn) < 8 ? 8 / (int)sizeof(TIn) : 1;
   if (t.vec == VV)
-    return launch_batched_v<TIn, B, VV>(s, m, t, binfo, mask_nan, frames, n_frames, slots,
-                                        cakes, lineouts);
-  return launch_batched_v<TIn, B, 1>(s, m, t, binfo, mask_nan, frames, n_frames, slots, cakes,
-                                     lineouts);
+    return launch_batched_g<TIn, FPL, VV>(s, m, t, binfo, mask_nan, frames, n_frames, cakes,
+                                          lineouts);
+  if constexpr (VV != 1) {
+    if (t.vec != 1) {
+      set_error("staging vector %d not supported", t.vec);
+      return MC_ERR_STATE;
+    }
+    return launch_batched_g<TIn, FPL, 1>(s, m, t, binfo, mask_nan, frames, n_frames, cakes,
+                                         lineouts);
+  }
+  set_error("staging vector %d not supported", t.vec);
+  return MC_ERR_STATE;
 }
 
 int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
-                   int mask_nan, const void *frames, int dtype, int n_frames,
-                   const int32_t *slots, float *cakes, float *lineouts) {
+                   int mask_nan, const void *frames, int dtype, int n_frames, float *cakes,
+                   float *lineouts) {
   return dispatch_dtype(dtype, [&](auto tag) -> int {
     using T = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
-#define MC_LB(Bv) \
-  launch_batched_t<T, Bv>(s, m, t, binfo, mask_nan, frames, n_frames, slots, cakes, lineouts)
-    switch (t.batch) {
-      case 8: return MC_LB(8);
-      case 16: return MC_LB(16);
-      case 32: return MC_LB(32);
-      case 64: return MC_LB(64);
-      case 128: return MC_LB(128);
-#undef MC_LB
-      default:
-        set_error("batch %d not in {8,16,32,64,128}", t.batch);
-        return MC_ERR_ARG;
-    }
+    if (t.fpl == 1)
+      return launch_batched_t<T, 1>(s, m, t, binfo, mask_nan, frames, n_frames, cakes,
+                                    lineouts);
+    if (t.fpl == 2)
+      return launch_batched_t<T, 2>(s, m, t, binfo, mask_nan, frames, n_frames, cakes,
+                                    lineouts);
+    set_error("frames per lane %d not in {1, 2}", t.fpl);
+    return MC_ERR_STATE;
   });
 }
 

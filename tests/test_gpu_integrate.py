@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import midas_amd as ma
+from midas_amd.params import IntegrationParams
 from midas_amd import synth
 from helpers import panel_shifts_for, rel_err, small_cases, to_oracle
 
@@ -74,14 +75,15 @@ def test_floor_mode_matches_gpu_stream_semantics(oracle_lib):
 
 
 @pytest.mark.parametrize("batch", [8, 16, 32, 64, 128])
-@pytest.mark.parametrize("rows", [1, 4, 8])
+@pytest.mark.parametrize("rows", [16, 32])
 def test_batched_tilings_agree(batch, rows):
-    """Every batch size and row-block height gives the one-frame result,
+    """Every batch size and band height gives the one-frame result,
     including a partial last pass (F not a multiple of the batch)."""
     P, mask, frames, dark = _setup("tilt", n_frames=batch + 3, seed=5)
     g = ma.build_map(P, mask=mask)
     g.set_dark(dark)
-    g.set_tuning(batch=batch, rows_per_group=rows, lds_bytes=32768 if batch <= 64 else 65536)
+    g.set_tuning(batch=batch, rows_per_group=rows, lds_bytes=49152 if batch <= 64 else 81920)
+    assert g.layout_info()["tiles"] > 0
     cb, lb = ma.integrate_batch(frames, g, mode="bilinear", mask_nan=True)
     for i in (0, batch - 1, batch, len(frames) - 1):
         c1, l1 = ma.integrate_batch(frames[i], g, mode="bilinear", mask_nan=True)
@@ -90,15 +92,26 @@ def test_batched_tilings_agree(batch, rows):
 
 
 def test_small_lds_budget_uses_direct_bins():
-    """A budget below a bin's footprint routes those bins through direct
-    global gathers; results still agree."""
-    P, mask, frames, dark = _setup("flat", n_frames=9, seed=11)
-    g = ma.build_map(P, mask=mask)
-    g.set_tuning(batch=8, rows_per_group=16, lds_bytes=8192)
-    cb, _ = ma.integrate_batch(frames, g, mode="bilinear")
-    g.set_tuning(batch=8, rows_per_group=2, lds_bytes=65536)
-    cr, _ = ma.integrate_batch(frames, g, mode="bilinear")
+    """Coarse bins under a small LDS budget: bins whose own pixel box does
+    not fit become direct (global gathers) and columns split into row
+    ranges; results still agree with the roomy layout."""
+    P = IntegrationParams(NrPixelsY=160, NrPixelsZ=144, pxY=200.0, pxZ=200.0, Lsd=1e6,
+                          BC_y=80.5, BC_z=70.25, RMin=2.0, RMax=100.0, RBinSize=1.5,
+                          EtaBinSize=30.0)
+    rng = np.random.default_rng(11)
+    frames = rng.poisson(200.0, (9, P.NrPixelsZ, P.NrPixelsY)).astype(np.uint16)
+    g = ma.build_map(P)
+    g.set_tuning(batch=8, rows_per_group=16, lds_bytes=24576 + 8 * 2 * 64)
+    tight = g.layout_info()
+    assert tight["direct_bins"] > 0
+    cb, lb = ma.integrate_batch(frames, g, mode="bilinear")
+    g.set_tuning(batch=8, rows_per_group=32, lds_bytes=163840)
+    assert g.layout_info()["direct_bins"] < tight["direct_bins"]
+    cr, lr = ma.integrate_batch(frames, g, mode="bilinear")
     np.testing.assert_allclose(cb, cr, rtol=3e-6, atol=1e-4, equal_nan=True)
+    np.testing.assert_allclose(lb, lr, rtol=3e-6, atol=1e-4)
+    c1, _ = ma.integrate_batch(frames[4], g, mode="bilinear")
+    np.testing.assert_allclose(cb[4], c1[0], rtol=3e-6, atol=1e-4, equal_nan=True)
 
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.uint32, np.int32, np.int64,
