@@ -11,8 +11,15 @@ GPU and stay resident: the timed region is integration only.
 A step is one pass of the batched kernel over the 64 frames (one launch at
 batch 64).  `value` = frames / s of the batched kernel; the one-frame kernel
 (64 launches per step) is reported beside it, with per-frame latency
-percentiles from HIP events, the HBM roofline of the batched kernel and the
-FP64 CPU oracle (all host cores) on a bounded sample of the same frames.
+percentiles (host clock around a synchronous one-frame call), the HBM
+roofline of the batched kernel and the FP64 CPU oracle (all host cores) on
+a bounded sample of the same frames.
+
+`roofline.traffic` is the measured HBM traffic per launch of the batched
+kernel (FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc runs of this script
+with --copy-calibration, scaled by the counters' reading for a 1 GiB device
+copy), read from profiles/traffic_batch_2048.json when that file was
+measured for the same tuning; otherwise null.
 """
 from __future__ import annotations
 
@@ -57,6 +64,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="finish with a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -152,6 +161,24 @@ def main():
         except Exception as exc:  # the checker is optional on the box
             log(f"CPU baseline skipped: {exc}")
 
+    layout = g.layout_info(args.mode)
+    traffic = None
+    tf = ROOT / "profiles" / "traffic_batch_2048.json"
+    if tf.exists():
+        try:
+            tj = json.loads(tf.read_text())
+            if (tj.get("batch"), tj.get("rows_per_group"), tj.get("lds_bytes")) == (B, H, L):
+                traffic = tj["hbm_bytes_per_launch"]
+        except (ValueError, KeyError):
+            traffic = None
+    if args.copy_calibration:
+        # an elementwise kernel that reads and writes exactly 1 GiB
+        x = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        torch.mul(x, 2.0, out=y)
+        torch.cuda.synchronize()
+        del x, y
+
     out = {
         "metric": "frames/s (2048^2 uint16 -> RxEta cake) batched vs plain; p50/p99 frame latency",
         "value": fps_b,
@@ -169,14 +196,14 @@ def main():
                                f"{P.n_r_bins}x{P.n_eta_bins} bins (R 0.25 px, eta 1 deg), "
                                f"solid angle + polarisation, {args.mode}, map nnz {info.nnz}",
                    "batch": B, "rows_per_group": H, "lds_bytes": L,
-                   "frames_per_step": args.frames},
+                   "frames_per_step": args.frames, "layout": layout},
         "plain_fps": fps_p,
         "batched_over_plain": fps_b / fps_p,
         "latency_ms": {"p50": p50, "p99": p99, "what": "one frame through mc_integrate "
                        "(device-resident frame, host-synchronous call)"},
         "kernel_us": {"batched_per_launch": ms_launch_b * 1e3, "plain_per_launch": ms_launch_p * 1e3},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_frame": bpf, "plain_achieved": achieved_plain},
         "cpu_baseline": cpu,
         "map_build_s": t_map,
