@@ -51,6 +51,94 @@ def algorithmic_bytes_per_frame(n_px, n_bins, n_r, nnz, batch, frame_bytes=2):
     return s_frame + s_out + (s_map + s_static) / batch
 
 
+def serving_bench(rates, n_frames, log):
+    """Poisson arrivals mixing three geometries (1024^2 u16, 2048^2 u16,
+    Pilatus 1679x1475 int32 with module gaps masked), each with its own
+    resident map and dark; the server batches in-flight frames by map.
+    Returns per rate: offered rate, achieved frames/s, p50/p99 latency (ms,
+    submit to result ready, host frames over PCIe included)."""
+    import numpy as np
+    import torch
+
+    import midas_amd as ma
+    from midas_amd import synth
+    from midas_amd.server import FrameServer
+
+    geos = []
+    for P, dt, mask in ((synth.plain_1024(), np.uint16, None),
+                        (synth.batch_2048(), np.uint16, None),
+                        (synth.pilatus(), np.int32, "gaps")):
+        m = synth.pilatus_gap_mask(P) if mask == "gaps" else None
+        g = ma.build_map(P, mask=m)
+        fr, dk = synth.torch_frames(P, 8, seed=7, n_rings=8, device="cuda",
+                                    dtype=torch.int32 if dt == np.int32 else None)
+        g.set_dark(dk)
+        geos.append((g, dt, fr.cpu().numpy().astype(dt)))
+    rng = np.random.default_rng(11)
+    # Warm-up: first launches of each kernel for each geometry (code-object
+    # loading) happen here, outside the measured runs.
+    with FrameServer(hbm_budget_bytes=64 << 30, max_batch=32, keep_results=False) as srv:
+        gids = [srv.add_geometry(g, dt) for g, dt, _ in geos]
+        for k in range(3):
+            srv.submit(gids[k], geos[k][2][0], k)
+            srv.drain()
+            for i in range(24):
+                srv.submit(gids[k], geos[k][2][i % 8], 100 + i)
+            srv.drain()
+    out = []
+    for rate in rates:
+        with FrameServer(hbm_budget_bytes=64 << 30, max_batch=32, queue_depth=256,
+                         keep_results=False) as srv:
+            gids = [srv.add_geometry(g, dt) for g, dt, _ in geos]
+            which = rng.integers(0, 3, n_frames)
+            gaps = rng.exponential(1.0 / rate, n_frames)
+            t0 = time.perf_counter()
+            t_next = t0
+            for i in range(n_frames):
+                t_next += gaps[i]
+                while time.perf_counter() < t_next:
+                    pass
+                k = int(which[i])
+                srv.submit(gids[k], geos[k][2][i % 8], i)
+            srv.drain()
+            wall = time.perf_counter() - t0
+            st = srv.stats()
+        out.append({"offered_fps": rate, "frames": n_frames, "fps": n_frames / wall,
+                    "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
+                    "batches": st["batches"]})
+        log(f"serving {rate} fps offered: {n_frames / wall:,.0f} fps, p50 {st['p50_ms']:.2f} ms, "
+            f"p99 {st['p99_ms']:.2f} ms, {st['batches']} batches")
+    return out
+
+
+def fit_bench(g, P, frames, log, n_frames=32):
+    """Integration + batched LM fits: 360 eta sectors x 10 rings = 3600
+    windows of 41 radial points per frame, pV and TCH, one launch for all
+    frames of the batch."""
+    import torch
+
+    import midas_amd as ma
+    from midas_amd import peakfit, synth
+
+    cakes, _ = ma.integrate_batch(frames[:n_frames], g, mode="bilinear", mask_nan=False)
+    RLo, RHi, _, _ = ma.bin_edges(P)
+    rc = (RLo + RHi) / 2
+    rings = synth.ring_radii(P, 10)
+    res = {}
+    for model in ("pv", "tch"):
+        peakfit.fit_cake_rings(cakes, g, rc, rings, pad=20, model=model)  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = peakfit.fit_cake_rings(cakes, g, rc, rings, pad=20, model=model)
+        dt = time.perf_counter() - t0
+        n = out.shape[0] * out.shape[1] * out.shape[2]
+        ok = float((out[..., 7] > 0).mean())
+        res[model] = {"fits": n, "fits_per_s": n / dt, "ms": dt * 1e3, "converged": ok}
+        log(f"fits {model}: {n} windows in {dt * 1e3:.1f} ms -> {n / dt:,.0f} fits/s "
+            f"({ok:.3f} converged)")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +154,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="finish with a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-frames", type=int, default=1500)
+    ap.add_argument("--no-fits", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -161,6 +252,11 @@ def main():
         except Exception as exc:  # the checker is optional on the box
             log(f"CPU baseline skipped: {exc}")
 
+    fits = None if args.no_fits else fit_bench(g, P, frames, log)
+    serving = None
+    if not args.no_serving:
+        serving = serving_bench((500, 2000, 5000), args.serving_frames, log)
+
     layout = g.layout_info(args.mode)
     traffic = None
     tf = ROOT / "profiles" / "traffic_batch_2048.json"
@@ -206,6 +302,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_frame": bpf, "plain_achieved": achieved_plain},
         "cpu_baseline": cpu,
+        "fits": fits,
+        "serving": serving,
         "map_build_s": t_map,
     }
     print(json.dumps(out))

@@ -550,6 +550,16 @@ static int ensure_recs(Map *m, hipStream_t s, Tiles *t, int mode, uint32_t flags
   return MC_OK;
 }
 
+int prepare_batched(Map *m, int mode, uint32_t flags, int dtype) {
+  const float2 *binfo = nullptr;
+  Tiles *t = nullptr;
+  int rc;
+  if ((rc = ensure_binfo(m, mode, flags, &binfo)) ||
+      (rc = ensure_tiles(m, mode, (int)dtype_size(dtype), &t)))
+    return rc;
+  return ensure_recs(m, m->ctx->stream, t, mode, flags, binfo);
+}
+
 int run_integrate(Map *m, hipStream_t s, const void *frames_dev, int dtype, int n_frames,
                   int mode, uint32_t flags, float *cakes_dev, float *lineouts_dev,
                   bool force_plain, bool force_batched) {

@@ -188,6 +188,8 @@ int ensure_tiles(Map *m, int mode, int esize, Tiles **out);
 int run_integrate(Map *m, hipStream_t s, const void *frames_dev, int dtype, int n_frames,
                   int mode, uint32_t flags, float *cakes_dev, float *lineouts_dev,
                   bool force_plain, bool force_batched);
+// Everything the batched kernel reads for (mode, flags, dtype), built ahead.
+int prepare_batched(Map *m, int mode, uint32_t flags, int dtype);
 // The per-bin table of (mode, flags); builds it (and the dark term) once.
 int ensure_binfo(Map *m, int mode, uint32_t flags, const float2 **out);
 

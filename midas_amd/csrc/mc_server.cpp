@@ -276,10 +276,7 @@ int mc_server_add_geometry(mc_server *sv, mc_map *m, int32_t dtype, int32_t mode
   {
     std::lock_guard<std::recursive_mutex> clk(sv->ctx->mu);
     std::lock_guard<std::mutex> mlk(m->mu);
-    const float2 *binfo;
-    if ((rc = ensure_binfo(m, mode, flags, &binfo))) return rc;
-    Tiles *t;
-    if ((rc = ensure_tiles(m, mode, (int)dtype_size(dtype), &t))) return rc;
+    if ((rc = prepare_batched(m, mode, flags, dtype))) return rc;
   }
   Geo G;
   G.m = m;
@@ -321,6 +318,13 @@ int mc_server_add_geometry(mc_server *sv, mc_map *m, int32_t dtype, int32_t mode
   if ((rc = grow_slots(sv, fb, cf, lf))) return rc;
   sv->geos.push_back(G);
   sv->free_host.emplace_back();
+  // Pinned staging for the first frames of this geometry, so that early
+  // requests do not pay for page-locked allocations.
+  for (int i = 0; i < std::min(sv->max_batch, 16); i++) {
+    void *buf = nullptr;
+    if (hipHostMalloc(&buf, G.frame_bytes, hipHostMallocDefault) != hipSuccess) break;
+    sv->free_host.back().push_back(buf);
+  }
   sv->peak_bytes = std::max(sv->peak_bytes, sv->resident_bytes());
   *geometry = (int32_t)sv->geos.size() - 1;
   return MC_OK;
