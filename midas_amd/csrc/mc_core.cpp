@@ -237,17 +237,19 @@ struct BBox {
 };
 }  // namespace
 
-// LDS of the batched kernel: bin records, staged entries, frame planes.
-constexpr int kTileBins = 256, kTileEnts = 2048, kBinRecBytes = 32, kWaves = 16, kTileInts = 14,
-              kStageVecs = 8;
+// LDS of the batched kernel: bin records, staged entries, the box of the
+// pixel-major batch (one row of 64 frames per pixel) and a zero pixel row.
+constexpr int kTileBins = 256, kTileEnts = 2048, kBinRecBytes = 32, kWaves = 16, kTileInts = 12,
+              kStageVecs = 4, kXtFrames = 64;
 
 // Tiles of the batched kernel.  Bands of H cake rows; inside a band, tiles
 // are rectangles of rows x eta columns grown column by column while the
-// bounding box of their entries' pixels (the per-frame LDS plane) fits the
-// budget; a column that does not fit whole is split into row ranges, and a
-// bin that does not fit alone is "direct" (gathered from global memory).
-// Chunks are runs of consecutive tiles of one band of about `target`
-// entries: the work of one workgroup.
+// bounding box of their entries' pixels fits the budget (LDS and the
+// staging registers); a column that does not fit whole is split into row
+// ranges, and a bin that does not fit alone is "direct" (gathered from
+// global memory).  Chunks are runs of consecutive tiles of one band of
+// about `target` entries: the work of one workgroup.  in_size is the
+// element size of the pixel-major batch (2 or 4 bytes).
 int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   int rc = ensure_csr(m, mode);
   if (rc) return rc;
@@ -259,69 +261,32 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
     }
   MC_CHECK(H == 16 || H == 32, MC_ERR_ARG, "rows per band %d not in {16, 32}", H);
   MC_CHECK(B >= 1 && B <= 128, MC_ERR_ARG, "batch %d not in 1..128", B);
+  MC_CHECK(in_size == 2 || in_size == 4, MC_ERR_ARG, "batch element size %d", in_size);
   const Csr &c = m->csr[mode];
   const int NY = m->ny, nE = m->n_eta, nR = m->n_r;
   const int64_t nb = m->nb();
-  const int tl = in_size <= 2 ? 2 : 4;  // LDS element size
-  // 8-byte staging loads when every frame row starts 8-byte aligned.
-  const int V = (in_size < 8 && ((int64_t)NY * in_size) % 8 == 0 && (m->npx() * in_size) % 8 == 0)
-                    ? 8 / in_size : 1;
+  const int RB = kXtFrames * in_size;  // bytes per pixel row
   const int64_t fixed = (int64_t)kTileBins * kBinRecBytes + (int64_t)kTileEnts * 8;
-  const int64_t plane_cap = m->lds_bytes - fixed;  // bytes for the B frame planes
-  // Each wave stages frames wave, wave + kWaves, ...: (B / kWaves) frames of
-  // (kStageVecs >> fw_log) vectors per lane.
-  const int fw_log = B <= kWaves ? 0 : B <= 2 * kWaves ? 1 : B <= 4 * kWaves ? 2 : 3;
-  int stage_log = 0;
-  while ((1 << stage_log) < kStageVecs) stage_log++;
-  const int spf_log = stage_log - fw_log;
-  MC_CHECK(spf_log >= 0, MC_ERR_ARG, "batch %d needs more staging registers", B);
-  MC_CHECK(plane_cap >= (int64_t)B * tl * 64, MC_ERR_ARG, "LDS budget %d too small for batch %d",
-           m->lds_bytes, B);
-  // Plane geometry of a pixel box: aligned start, power-of-two vector count.
-  struct Box {
-    int y0, z0, bwp, bh, ks, S;
-    int64_t bytes;
-  };
-  auto box_of = [&](const BBox &bx, Box &o) -> bool {
-    if (bx.empty()) {
-      o = {0, 0, 0, 0, 0, 0, 0};
-      return true;
-    }
-    int ya = (bx.y0 / V) * V;
-    int nv = 1, ks = 0;
-    while (nv * V < bx.y1 - ya + 1) {
-      nv <<= 1;
-      ks++;
-    }
-    if (nv > 64) return false;
-    const int bwp = nv * V;
-    if (ya + bwp > NY) ya = NY - bwp;
-    if (ya < 0) return false;
-    const int bh = bx.z1 - bx.z0 + 1;
-    int64_t S = (int64_t)bwp * bh + 1;  // + the zero pad element read by sentinels
-    if (tl == 2) {
-      while (S % 4 != 2) S++;
-    } else if (S % 2 == 0) {
-      S++;
-    }
-    o = {ya, bx.z0, bwp, bh, ks, (int)S, (int64_t)B * S * tl};
-    // every staging vector of the batch in one register pass
-    return o.bytes <= plane_cap && (int64_t)nv * bh <= ((int64_t)64 << spf_log);
+  // Box pixels: LDS (plus the zero row) and one register pass of staging.
+  const int64_t cap_px = std::min<int64_t>((m->lds_bytes - fixed) / RB - 1,
+                                           (int64_t)64 * kWaves * kStageVecs * 16 / RB);
+  MC_CHECK(cap_px >= 16, MC_ERR_ARG, "LDS budget %d too small", m->lds_bytes);
+  auto box_ok = [&](const BBox &bx) {
+    return bx.empty() || (int64_t)(bx.y1 - bx.y0 + 1) * (bx.z1 - bx.z0 + 1) <= cap_px;
   };
   // Per-bin pixel boxes; a bin that does not fit alone is direct.
   std::vector<BBox> bb(nb);
   std::vector<char> big(nb, 0);
   parallel_for(nb, [&](int64_t a, int64_t b) {
-    Box tmp;
     for (int64_t q = a; q < b; q++) {
       for (int32_t k = c.h_row[q]; k < c.h_row[q + 1]; k++)
         bb[q].add(c.h_pix[k] % NY, c.h_pix[k] / NY);
-      big[q] = !box_of(bb[q], tmp) || ((c.h_row[q + 1] - c.h_row[q] + 7) & ~7) > kTileEnts;
+      big[q] = !box_ok(bb[q]) || ((c.h_row[q + 1] - c.h_row[q] + 7) & ~7) > kTileEnts;
     }
   });
   auto cnt = [&](int64_t b) { return (int64_t)(c.h_row[b + 1] - c.h_row[b]); };
   // Staged entries of a bin are padded to whole groups of 8 (zero-weight
-  // entries on the plane's zero pad element).
+  // entries on the zero pixel row).
   auto cntp = [&](int64_t b) { return (cnt(b) + 7) & ~(int64_t)7; };
   struct TileH {
     int ra = 0, nrows = 0, e0 = 0, nc = 0;
@@ -334,12 +299,11 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
     for (int64_t gi = a; gi < bq; gi++) {
       auto &tv = bands[gi];
       const int row0 = (int)gi * H, nrows = std::min(H, nR - row0);
-      Box tmp;
       auto room = [&](const TileH &T, const BBox &add, int64_t ents, int nbins) {
         BBox u = T.box;
         u.add(add);
         return (int64_t)T.nrows * T.nc + nbins <= kTileBins && T.ents + ents <= kTileEnts &&
-               box_of(u, tmp);
+               box_ok(u);
       };
       TileH cur;
       auto flush = [&]() {
@@ -380,9 +344,7 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
           const int64_t b = (int64_t)(row0 + j) * nE + e;
           const BBox add = big[b] ? BBox() : bb[b];
           const int64_t ae = big[b] ? 0 : cntp(b);
-          TileH probe = piece;
-          probe.nc = 1;
-          if (piece.nrows > 0 && !room(probe, add, ae, 1)) {
+          if (piece.nrows > 0 && !room(piece, add, ae, 1)) {
             tv.push_back(piece);
             piece = TileH();
             piece.e0 = e;
@@ -398,15 +360,23 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
       flush();
     }
   });
-  // Flatten.  Chunks: runs of consecutive tiles of one band with about
-  // `target` entries (a few thousand workgroups for a 2048^2 map).
-  const int64_t target = std::max<int64_t>(2048, c.nnz / 2048);
+  // Flatten.  Chunks: runs of consecutive tiles of one band of about
+  // `target` cost (a few thousand workgroups for a 2048^2 map).  The cost
+  // of a tile counts its entries plus a fixed charge per bin and per tile:
+  // a bin of 2 entries costs a wave about as much as one of 30.
+  constexpr int64_t kBinCost = 24, kTileCost = 256;
+  int64_t total_cost = 0;
+  for (auto &tv : bands)
+    for (auto &T : tv) total_cost += kTileCost + kBinCost * T.nrows * T.nc;
+  total_cost += c.nnz;
+  const int64_t target = std::max<int64_t>(4096, total_cost / 2048);
+  std::vector<int64_t> chunk_cost;
   std::vector<int32_t> chunks, band_chunk(n_bands + 1, 0), tiles;
   std::vector<int4> tbins;
   std::vector<uint2> ents;
   tbins.reserve(nb);
-  ents.reserve(c.nnz);
-  int64_t n_direct = 0, max_plane = 0, staged = 0, real_ents = 0;
+  ents.reserve(c.nnz + c.nnz / 4);
+  int64_t n_direct = 0, max_lds = 0, staged = 0, real_ents = 0;
   for (int gi = 0; gi < n_bands; gi++) {
     const int row0 = gi * H, nrows_band = std::min(H, nR - row0);
     int64_t acc = target;  // forces a new chunk at the band start
@@ -414,27 +384,32 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
       if (acc >= target) {
         chunks.insert(chunks.end(), {(int32_t)(tiles.size() / kTileInts),
                                      (int32_t)(tiles.size() / kTileInts), row0, nrows_band});
+        chunk_cost.push_back(0);
         acc = 0;
       }
-      Box bx;
-      box_of(T.box, bx);
-      max_plane = std::max(max_plane, bx.bytes);
-      staged += (int64_t)bx.bwp * bx.bh;
+      int bx0 = 0, bz0 = 0, bw = 0, bh = 0;
+      if (!T.box.empty()) {
+        bx0 = T.box.y0;
+        bz0 = T.box.z0;
+        bw = T.box.y1 - T.box.y0 + 1;
+        bh = T.box.z1 - T.box.z0 + 1;
+      }
+      max_lds = std::max<int64_t>(max_lds, ((int64_t)bw * bh + 1) * RB);
+      staged += (int64_t)bw * bh;
       const int32_t b_beg = (int32_t)tbins.size(), e_beg = (int32_t)ents.size();
+      const uint32_t zero_row = (uint32_t)((int64_t)bw * bh * RB);
       auto push_entries = [&](int64_t b, bool dir) {
         for (int32_t k = c.h_row[b]; k < c.h_row[b + 1]; k++) {
           const int p = c.h_pix[k];
-          const uint32_t idx =
-              dir ? (uint32_t)p
-                  : (uint32_t)(((p / NY - bx.z0) * bx.bwp + (p % NY - bx.y0)) * tl);
+          const uint32_t off =
+              dir ? (uint32_t)p : (uint32_t)(((p / NY - bz0) * bw + (p % NY - bx0)) * RB);
           uint32_t wb;
           memcpy(&wb, &c.h_w[k], 4);
-          ents.push_back(make_uint2(idx, wb));
+          ents.push_back(make_uint2(off, wb));
           real_ents++;
         }
         if (!dir)
-          while ((ents.size() - (size_t)e_beg) % 8)
-            ents.push_back(make_uint2((uint32_t)((bx.S - 1) * tl), 0u));
+          while ((ents.size() - (size_t)e_beg) % 8) ents.push_back(make_uint2(zero_row, 0u));
       };
       // Records row-major; staged entries first (contiguous), then direct.
       std::vector<int4> recs((size_t)T.nrows * T.nc);
@@ -459,11 +434,15 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
           n_direct++;
         }
       tbins.insert(tbins.end(), recs.begin(), recs.end());
-      const int pv = bx.bwp > 0 ? (bx.bwp / V) * bx.bh : 0;
-      tiles.insert(tiles.end(), {bx.y0, bx.z0, bx.bwp, bx.bh, bx.ks, bx.S, b_beg,
-                                 (int32_t)tbins.size(), e_beg, e_end, T.ra, T.nc, pv, 0});
+      const int vpr = bw * RB / 16;  // staging vectors per box row
+      // floor(j / vpr) = umulhi(j, magic) for j < 2^32 / vpr (j < 4096 here)
+      const uint32_t magic = vpr > 1 ? (uint32_t)(((uint64_t)1 << 32) / (uint64_t)vpr + 1) : 0u;
+      tiles.insert(tiles.end(), {bx0, bz0, bw, bh, b_beg, (int32_t)tbins.size(), e_beg, e_end,
+                                 T.ra, T.nc, vpr, (int32_t)magic});
       chunks[chunks.size() - 3] = (int32_t)(tiles.size() / kTileInts);
-      acc += all_ents;
+      const int64_t tcost = all_ents + kTileCost + kBinCost * T.nrows * T.nc;
+      acc += tcost;
+      chunk_cost.back() += tcost;
     }
     band_chunk[gi + 1] = (int32_t)(chunks.size() / 4);
   }
@@ -471,17 +450,31 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
            (long long)tbins.size(), (long long)nb);
   MC_CHECK(real_ents == c.nnz, MC_ERR_STATE, "tile planner placed %lld of %lld entries",
            (long long)real_ents, (long long)c.nnz);
+  // The 8 XCDs take contiguous chunk ranges of equal cost (workgroups are
+  // handed to the XCDs in turn; see k_batched).
+  std::vector<int32_t> xcd(9, 0);
+  {
+    const int64_t n = (int64_t)chunk_cost.size();
+    int64_t sum = 0, all = 0;
+    for (int64_t v : chunk_cost) all += v;
+    int x = 1;
+    for (int64_t i = 0; i < n && x < 8; i++) {
+      sum += chunk_cost[i];
+      while (x < 8 && sum * 8 >= all * x) xcd[x++] = (int32_t)(i + 1);
+    }
+    while (x < 9) xcd[x++] = (int32_t)n;
+  }
+  int32_t per_xcd = 0;
+  for (int x = 0; x < 8; x++) per_xcd = std::max(per_xcd, xcd[x + 1] - xcd[x]);
   auto t = std::make_unique<Tiles>();
   t->batch = B;
+  t->grid = 8 * per_xcd;
   t->rows = H;
   t->lds_bytes = m->lds_bytes;
   t->in_size = in_size;
-  t->vec = V;
-  t->fpl = B > 64 ? 2 : 1;
-  t->spf_log = spf_log;
   t->n_chunks = (int)(chunks.size() / 4);
   t->n_bands = n_bands;
-  t->lds_needed = ((size_t)fixed + (size_t)max_plane + 15) & ~(size_t)15;
+  t->lds_needed = ((size_t)fixed + (size_t)max_lds + 15) & ~(size_t)15;
   t->n_tiles = (int64_t)tiles.size() / kTileInts;
   t->n_tbins = (int64_t)tbins.size();
   t->n_ents = (int64_t)ents.size();
@@ -489,8 +482,8 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   t->staged_px = staged;
   if ((rc = t->chunks.alloc(chunks.size() * 4)) || (rc = t->tiles.alloc(tiles.size() * 4)) ||
       (rc = t->tbins.alloc(tbins.size() * 16)) || (rc = t->ents.alloc(ents.size() * 8)) ||
-      (rc = t->band_chunk.alloc(band_chunk.size() * 4)) ||
-      (rc = t->part.alloc((size_t)t->n_chunks * H * 4 * 64 * t->fpl * 4)))
+      (rc = t->band_chunk.alloc(band_chunk.size() * 4)) || (rc = t->xcd.alloc(9 * 4)) ||
+      (rc = t->part.alloc((size_t)t->n_chunks * H * 4 * kXtFrames * 4)))
     return rc;
   MC_HIP(hipMemcpy(t->chunks.p, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->tiles.p, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice));
@@ -498,6 +491,7 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   MC_HIP(hipMemcpy(t->ents.p, ents.data(), ents.size() * 8, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->band_chunk.p, band_chunk.data(), band_chunk.size() * 4,
                    hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(t->xcd.p, xcd.data(), 9 * 4, hipMemcpyHostToDevice));
   t->built = true;
   *out = t.get();
   m->tiles[mode].push_back(std::move(t));
@@ -550,19 +544,30 @@ static int ensure_recs(Map *m, hipStream_t s, Tiles *t, int mode, uint32_t flags
   return MC_OK;
 }
 
+// Element size of the pixel-major batch for a frame dtype.
+static int batch_elem(int dtype) { return dtype_size(dtype) <= 2 ? 2 : 4; }
+
+// Frames per batched pass (the pixel-major batch holds 64 frames).
+static int pass_frames(const Tiles &t) { return std::min(t.batch, 64); }
+
+static int ensure_xt(DBuf &xt, const Map &m, int dtype) {
+  const size_t need = batch_xt_bytes(m, dtype);
+  return xt.bytes >= need ? MC_OK : xt.alloc(need);
+}
+
 int prepare_batched(Map *m, int mode, uint32_t flags, int dtype) {
   const float2 *binfo = nullptr;
   Tiles *t = nullptr;
   int rc;
   if ((rc = ensure_binfo(m, mode, flags, &binfo)) ||
-      (rc = ensure_tiles(m, mode, (int)dtype_size(dtype), &t)))
+      (rc = ensure_tiles(m, mode, batch_elem(dtype), &t)))
     return rc;
   return ensure_recs(m, m->ctx->stream, t, mode, flags, binfo);
 }
 
 int run_integrate(Map *m, hipStream_t s, const void *frames_dev, int dtype, int n_frames,
                   int mode, uint32_t flags, float *cakes_dev, float *lineouts_dev,
-                  bool force_plain, bool force_batched) {
+                  bool force_plain, bool force_batched, DBuf *xt) {
   int rc;
   const float2 *binfo = nullptr;
   if ((rc = ensure_binfo(m, mode, flags, &binfo))) return rc;
@@ -574,20 +579,17 @@ int run_integrate(Map *m, hipStream_t s, const void *frames_dev, int dtype, int 
                         lineouts_dev, n_frames, m->npx());
   }
   Tiles *t = nullptr;
-  if ((rc = ensure_tiles(m, mode, (int)dtype_size(dtype), &t))) return rc;
-  // The staging loads of the batched kernel are 8-byte vectors when the
-  // rows allow it; a frame buffer that does not start 8-byte aligned (a
-  // view into a larger tensor) takes the one-frame kernel instead.
-  if (t->vec > 1 && ((uintptr_t)frames_dev & 7) != 0)
-    return launch_plain(s, *m, m->csr[mode], binfo, mask_nan, frames_dev, dtype, cakes_dev,
-                        lineouts_dev, n_frames, m->npx());
+  if ((rc = ensure_tiles(m, mode, batch_elem(dtype), &t))) return rc;
   if ((rc = ensure_recs(m, s, t, mode, flags, binfo))) return rc;
-  for (int f0 = 0; f0 < n_frames; f0 += t->batch) {
-    const int nf = std::min(t->batch, n_frames - f0);
+  DBuf &ws = xt ? *xt : m->xt;
+  if ((rc = ensure_xt(ws, *m, dtype))) return rc;
+  const int step = pass_frames(*t);
+  for (int f0 = 0; f0 < n_frames; f0 += step) {
+    const int nf = std::min(step, n_frames - f0);
     const char *fb = (const char *)frames_dev + (size_t)f0 * m->npx() * es;
     float *cb = cakes_dev + (size_t)f0 * m->nb();
     float *lb = lineouts_dev ? lineouts_dev + (size_t)f0 * 3 * m->n_r : nullptr;
-    if ((rc = launch_batched(s, *m, *t, binfo, mask_nan, fb, dtype, nf, cb, lb)))
+    if ((rc = launch_batched(s, *m, *t, binfo, mask_nan, fb, dtype, nf, cb, lb, ws.p)))
       return rc;
   }
   return MC_OK;
@@ -828,14 +830,14 @@ int mc_map_info_get(mc_map *m, mc_map_info *info) {
   info->n_flagged = nf;
   info->q_mode = m->q_mode;
   int64_t hb = m->d_rec.bytes + m->d_pairs.bytes + m->d_flags.bytes + m->d_area_inb.bytes +
-               m->d_row_static.bytes + m->inv_area.bytes + m->d_dark.bytes;
+               m->d_row_static.bytes + m->inv_area.bytes + m->d_dark.bytes + m->xt.bytes;
   for (const DBuf &b : m->d_binfo) hb += b.bytes;
   for (int q = 0; q < 3; q++) {
     const Csr &c = m->csr[q];
     hb += c.row.bytes + c.pix.bytes + c.w.bytes + c.dterm.bytes;
     for (auto &t : m->tiles[q])
       hb += t->chunks.bytes + t->tiles.bytes + t->tbins.bytes + t->ents.bytes +
-            t->band_chunk.bytes + t->part.bytes;
+            t->band_chunk.bytes + t->part.bytes + t->recs.bytes;
   }
   info->hbm_bytes = hb;
   for (int q = 0; q < 3; q++) info->skipped[q] = m->skipped[q];
@@ -958,10 +960,10 @@ int mc_map_layout_info(mc_map *m, int32_t mode, int32_t dtype, int64_t *out) {
   std::lock_guard<std::mutex> lk2(m->mu);
   MC_HIP(hipSetDevice(m->ctx->device));
   Tiles *t = nullptr;
-  int rc = ensure_tiles(m, mode, (int)es, &t);
+  int rc = ensure_tiles(m, mode, batch_elem(dtype), &t);
   if (rc) return rc;
   const int64_t v[8] = {t->n_tiles, t->n_chunks, t->n_direct, t->staged_px,
-                        t->n_ents,  (int64_t)t->lds_needed, t->n_bands, t->vec};
+                        t->n_ents,  (int64_t)t->lds_needed, t->n_bands, t->in_size};
   for (int i = 0; i < 8; i++) out[i] = v[i];
   return MC_OK;
 }
@@ -1050,19 +1052,21 @@ int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype, int32_t 
   if ((rc = ensure_binfo(m, mode, flags, &binfo))) return rc;
   const int mask_nan = (flags & MC_F_MASK_NAN) ? 1 : 0;
   Tiles *t = nullptr;
-  if (kind == 1 && ((rc = ensure_tiles(m, mode, (int)dtype_size(dtype), &t)) ||
-                    (rc = ensure_recs(m, s, t, mode, flags, binfo))))
+  if (kind == 1 && ((rc = ensure_tiles(m, mode, batch_elem(dtype), &t)) ||
+                    (rc = ensure_recs(m, s, t, mode, flags, binfo)) ||
+                    (rc = ensure_xt(m->xt, *m, dtype))))
     return rc;
   const size_t es = dtype_size(dtype);
   auto one_iter = [&](std::vector<hipEvent_t> *ev) -> int {
-    const int step = kind == 1 ? t->batch : 1;
+    const int step = kind == 1 ? pass_frames(*t) : 1;
     for (int f0 = 0; f0 < n_frames; f0 += step) {
       const int nf = std::min(step, n_frames - f0);
       const char *fb = (const char *)frames_dev + (size_t)f0 * m->npx() * es;
       float *cb = cakes_dev + (size_t)f0 * m->nb();
       float *lb = lineouts_dev ? lineouts_dev + (size_t)f0 * 3 * m->n_r : nullptr;
       if (ev) MC_HIP(hipEventRecord((*ev)[2 * (f0 / step)], s));
-      int r2 = kind == 1 ? launch_batched(s, *m, *t, binfo, mask_nan, fb, dtype, nf, cb, lb)
+      int r2 = kind == 1 ? launch_batched(s, *m, *t, binfo, mask_nan, fb, dtype, nf, cb, lb,
+                                          m->xt.p)
                          : launch_plain(s, *m, m->csr[mode], binfo, mask_nan, fb, dtype, cb, lb,
                                         1, m->npx());
       if (r2) return r2;
@@ -1073,7 +1077,7 @@ int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype, int32_t 
   for (int w = 0; w < warmup; w++)
     if ((rc = one_iter(nullptr))) return rc;
   MC_HIP(hipStreamSynchronize(s));
-  const int step = kind == 1 ? t->batch : 1;
+  const int step = kind == 1 ? pass_frames(*t) : 1;
   const int launches = (n_frames + step - 1) / step;
   std::vector<hipEvent_t> ev(2 * (size_t)launches * iters);
   for (auto &e : ev) MC_HIP(hipEventCreate(&e));

@@ -96,22 +96,21 @@ struct Csr {
 // LDS budget).
 struct Tiles {
   bool built = false;
-  int batch = 0, rows = 0, lds_bytes = 0;  // planes per launch, band rows (8 * G), LDS budget
-  int in_size = 0;   // bytes per input pixel (frame dtype)
-  int vec = 1;       // pixels per staging load (8 bytes when rows are aligned)
-  int fpl = 1;       // frames per lane (batch / 64, at least 1)
-  int spf_log = 0;   // log2 of staging vectors per frame and thread
+  int batch = 0, rows = 0, lds_bytes = 0;  // frames per pass, band rows (16 * G), LDS budget
+  int in_size = 0;   // bytes per element of the pixel-major batch (2 or 4)
   int n_chunks = 0, n_bands = 0;
   size_t lds_needed = 0;
   DBuf chunks;       // int32 x4 per chunk: tile_begin tile_end band_row0 band_rows
-  DBuf tiles;        // int32 x14 per tile (see mc_kernels.hip)
+  DBuf tiles;        // int32 x12 per tile (see mc_kernels.hip)
   DBuf tbins;        // int32 x4 per tile bin: cake bin, ent_begin, ent_end, direct flag
   DBuf recs;         // 32-byte records: tbins + scale, dark term, area of one binfo table
   const void *recs_binfo = nullptr;
   uint64_t recs_version = 0;
-  DBuf ents;         // uint32 x2: index (tile box or global), weight bits
+  DBuf ents;         // uint32 x2: byte offset in the tile box (or global pixel), weight bits
   DBuf band_chunk;   // int32 [n_bands+1]: chunk range of each band
-  DBuf part;         // float [n_chunks][rows][4][64 * fpl]: lineout partials
+  DBuf xcd;          // int32 [9]: chunk range of each XCD (equal cost)
+  int grid = 0;      // workgroups of the launch: 8 x the largest XCD range
+  DBuf part;         // float [n_chunks][rows][4][64]: lineout partials
   int64_t n_tiles = 0, n_tbins = 0, n_ents = 0, n_direct = 0, staged_px = 0;
 };
 
@@ -147,6 +146,7 @@ struct Map {
   uint64_t binfo_version[12] = {};  // bumped whenever a table is rebuilt
   // tuning
   int batch = 64, rows_per_group = 32, lds_bytes = 98304;
+  DBuf xt;  // pixel-major batch of the calls made through the map's context
   int64_t nb() const { return (int64_t)n_r * n_eta; }
   int64_t npx() const { return (int64_t)ny * nz; }
 };
@@ -168,9 +168,11 @@ int launch_plain(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo,
                  float *lineouts, int n_frames, int64_t frame_stride);
 // Tile records of t for one binfo table (scale/dark/area folded in).
 int launch_tile_recs(hipStream_t s, const Map &m, Tiles &t, const float2 *binfo);
+// One batched pass of up to 64 frames; xt is scratch of batch_xt_bytes().
 int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
                    int mask_nan, const void *frames, int dtype, int n_frames, float *cakes,
-                   float *lineouts);
+                   float *lineouts, void *xt);
+size_t batch_xt_bytes(const Map &m, int dtype);
 int launch_profile(hipStream_t s, const Map &m, const float *cakes, int n_frames,
                    int mask_nan, float *lineouts);
 
@@ -185,9 +187,10 @@ int map_finish_host(Map *m);  // derived arrays + upload
 int ensure_csr(Map *m, int mode);
 int ensure_tiles(Map *m, int mode, int esize, Tiles **out);
 // Launches the integration of n_frames device frames on stream s (no sync).
+// xt: scratch of the batched passes (NULL: the map's own).
 int run_integrate(Map *m, hipStream_t s, const void *frames_dev, int dtype, int n_frames,
                   int mode, uint32_t flags, float *cakes_dev, float *lineouts_dev,
-                  bool force_plain, bool force_batched);
+                  bool force_plain, bool force_batched, DBuf *xt = nullptr);
 // Everything the batched kernel reads for (mode, flags, dtype), built ahead.
 int prepare_batched(Map *m, int mode, uint32_t flags, int dtype);
 // The per-bin table of (mode, flags); builds it (and the dark term) once.

@@ -173,45 +173,56 @@ __global__ __launch_bounds__(256) void k_profile(
 }
 
 // ---------------------------------------------------------------------
-// Batched kernel.  The cake rows are cut into bands of 16*G rows and a band
-// into tiles: rectangles of rows x eta columns whose pixel footprint (the
-// bounding box of their map entries, padded to whole staging vectors) fits
-// the LDS plane budget.  A workgroup of 16 waves takes one chunk -- a run of
+// Batched path: two kernels per pass of up to 64 frames.
+//
+// k_transpose turns the pass's frames [F][npx] into the pixel-major batch
+// Xt[npx][64] (frame fastest, frames past F zero): 128 bytes per pixel for
+// 8/16-bit frames (kept as uint16), 256 bytes for the other types (FP32,
+// exact for integer counts below 2^24).
+//
+// k_batched cuts the cake rows into bands of 16*G rows and a band into
+// tiles: rectangles of rows x eta columns whose pixel bounding box fits the
+// LDS budget.  A workgroup of 16 waves takes one chunk -- a run of
 // consecutive tiles of one band -- and for each tile stages into LDS
 //   * the tile's bin records (cake bin, entry range, scale/dark, area),
-//   * its map entries (pixel index inside the box, weight),
-//   * the box of every frame of the batch, one plane per frame (plane
-//     stride S elements, so that lanes reading the same pixel of
-//     consecutive frames hit different LDS banks),
-// then reduces the bins with lane = frame: one entry read (an LDS
-// broadcast) serves every frame of the batch.  Wave w owns the band rows
-// j = w (mod 16), so it keeps the lineout sums of its G rows in registers
-// across the whole chunk and writes them once at the end;
-// k_lineout_finalize adds the chunks of each band.  Bins whose own
-// footprint exceeds the budget ("direct" bins) gather from global memory.
+//   * its map entries (byte offset of the pixel row in the box, weight),
+//     each bin's entries padded to groups of 8 with zero-weight entries on
+//     a zero pixel row,
+//   * the box of the batch: box row after box row, each pixel's 64 frames
+//     contiguous -- whole 128-byte lines of Xt, so staging reads only what
+//     it keeps.
+// The loads of tile t+1 are issued into registers before the waves reduce
+// tile t.  In the reduction the four 16-lane groups of a wave take the
+// entries of a bin in turn (group g the entries 2g, 2g+1 of each group of
+// 8) and lane l of a group holds frames 4l..4l+3: one 8-byte LDS read
+// serves 4 frames of one entry.  The groups' partial sums meet by lane
+// exchange at the end of the bin; the lane (g, l) then keeps frame 4l+g.
+// Wave w owns the band rows j = w (mod 16), so it keeps the lineout sums
+// of its G rows in registers across the chunk; k_lineout_finalize adds the
+// chunks of each band.  Cake values are stored per frame in whole 16-bin
+// granules.  Bins whose own box exceeds the budget ("direct" bins) gather
+// from Xt in global memory.
 //   chunk: tile_begin tile_end band_row0 band_rows
-//   tile : y0 z0 bwp bh kshift S b_beg b_end e_beg e_end ra nc pv 0
-//          (box of bwp x bh pixels at (y0, z0), bwp = (1 << kshift) * V;
-//          plane stride S elements with a zero pad element at S-1;
-//          band rows ra .. ra + (b_end - b_beg) / nc - 1, nc columns,
-//          records row-major; pv = staging vectors per frame plane)
+//   tile : y0 z0 bw bh b_beg b_end e_beg e_end ra nc vpr magic
+//          (box of bw x bh pixels at (y0, z0), band rows ra ..
+//          ra + (b_end - b_beg) / nc - 1, nc columns, records row-major;
+//          vpr = 16-byte staging vectors per box row, magic = the
+//          multiplier of the division by vpr)
 //   rec  : cake bin, ent_begin, ent_end, direct flag, scale, dark term,
 //          in-bounds area
-//   ent  : (byte offset of the pixel in its box plane, or global pixel if
+//   ent  : (byte offset of the pixel row in the box, or global pixel if
 //          direct), weight
-// The loads of tile t+1 (this thread's share of the boxes, records and
-// entries, kStageVecs staging vectors at most) are issued into registers
-// before the waves reduce tile t, so global latency hides behind the
-// reduction.
-// Workgroup b takes chunk (b % 8) * per + b / 8: hardware hands
-// consecutive workgroups to the 8 XCDs in turn, so each XCD works through
-// one contiguous range of bands and neighbouring bands share its L2.
+// Workgroup b takes chunk xcd[b % 8] + b / 8: hardware hands consecutive
+// workgroups to the 8 XCDs in turn, so each XCD works through one
+// contiguous range of bands of equal estimated cost, and neighbouring bands
+// share its L2.
 // ---------------------------------------------------------------------
 constexpr int kWaves = 16;
 constexpr int kTileBins = 256;    // bin records per tile (LDS)
 constexpr int kTileEnts = 2048;   // staged map entries per tile (LDS)
-constexpr int kTileInts = 14;
-constexpr int kStageVecs = 8;     // staging vectors per thread and tile
+constexpr int kTileInts = 12;
+constexpr int kStageVecs = 4;     // 16-byte staging vectors per thread and tile
+constexpr int kXtFrames = 64;     // frames per pixel row of the batch
 
 struct BinRec {
   int4 tb;
@@ -220,22 +231,93 @@ struct BinRec {
   float pad;
 };
 
-template <typename TIn, int V> struct alignas(sizeof(TIn) * V) VecLoad {
-  TIn v[V];
-};
-
-// Cake values are written per lane (frame) in whole granules of LG
-// consecutive bins: each row slot of a wave keeps the granule it is
-// filling in a register vector (the position inside the granule is a
-// run-time index, which the hardware's register indexing serves) and
-// stores it when its next bin falls in another granule.  Storing a frame's
-// bins one at a time would leave partially written lines for the L2 to
-// evict (a 16-byte write per 4-byte value).
-template <int LG> struct GranVec;
-template <> struct GranVec<16> { typedef float type __attribute__((ext_vector_type(16))); };
-template <> struct GranVec<8> { typedef float type __attribute__((ext_vector_type(8))); };
+typedef int i4v __attribute__((ext_vector_type(4)));
+typedef unsigned u2v __attribute__((ext_vector_type(2)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
+// ---- pixel-major batch ----
+template <typename TIn, typename TL>
+__global__ __launch_bounds__(256) void k_transpose(const TIn *__restrict__ frames,
+                                                   int64_t frame_stride, int n_frames,
+                                                   int64_t npx, int vec_ok, TL *__restrict__ xt) {
+  constexpr int PX = 64;                       // pixels per block
+  constexpr int EV = 16 / (int)sizeof(TL);     // elements per 16-byte store
+  __shared__ TL tile[kXtFrames][PX + 4 / (int)sizeof(TL)];
+  const int64_t p0 = (int64_t)blockIdx.x * PX;
+  const int t = threadIdx.x;
+  if (vec_ok && sizeof(TIn) == 2 && sizeof(TL) == 2) {
+    // 16-byte loads: thread -> (frame t/4, 16 pixels as two vectors)
+    const int f = t >> 2, q0 = (t & 3) * 16;
+    u4v a = 0u, b = 0u;
+    if (f < n_frames) {
+      const u4v *src = reinterpret_cast<const u4v *>(frames + (int64_t)f * frame_stride + p0 + q0);
+      a = src[0];
+      b = src[1];
+    }
+    uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      tile[f][q0 + 2 * q] = (TL)(w[q] & 0xffffu);
+      tile[f][q0 + 2 * q + 1] = (TL)(w[q] >> 16);
+    }
+  } else {
+    const int f = t >> 2, q0 = (t & 3) * 16;
+    const TIn *src = frames + (int64_t)f * frame_stride + p0 + q0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      TL v = static_cast<TL>(0);
+      if (f < n_frames && p0 + q0 + q < npx) v = static_cast<TL>(src[q]);
+      tile[f][q0 + q] = v;
+    }
+  }
+  __syncthreads();
+  // 16-byte stores: thread -> (pixel, EV frames); 64 / EV stores per pixel.
+  constexpr int SPP = kXtFrames / EV;
+#pragma unroll
+  for (int it = 0; it < PX * SPP / 256; ++it) {
+    const int job = t + 256 * it;
+    const int p = job / SPP, c = job - p * SPP;
+    if (p0 + p >= npx) continue;
+    u4v out;
+    if constexpr (sizeof(TL) == 2) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        out[k] = (uint32_t)(uint16_t)tile[c * EV + 2 * k][p] |
+                 ((uint32_t)(uint16_t)tile[c * EV + 2 * k + 1][p] << 16);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) out[k] = __float_as_uint((float)tile[c * EV + k][p]);
+    }
+    *reinterpret_cast<u4v *>(xt + (p0 + p) * kXtFrames + c * EV) = out;
+  }
+}
+
+// Four frames of one pixel row: uint16 x4 in 8 bytes or float x4 in 16.
+template <typename TL> struct Quad;
+template <> struct Quad<uint16_t> {
+  typedef u2v type;
+  __device__ __forceinline__ static void unpack(const u2v v, float o[4]) {
+    o[0] = (float)(v.x & 0xffffu);
+    o[1] = (float)(v.x >> 16);
+    o[2] = (float)(v.y & 0xffffu);
+    o[3] = (float)(v.y >> 16);
+  }
+};
+template <> struct Quad<float> {
+  typedef f4v type;
+  __device__ __forceinline__ static void unpack(const f4v v, float o[4]) {
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+  }
+};
+
+template <int LG> struct GranVec;
+template <> struct GranVec<16> { typedef float type __attribute__((ext_vector_type(16))); };
+
+// Stores a row slot's granule for this lane's frame f (16 consecutive bins).
 template <int LG, typename VT>
 __device__ __forceinline__ void flush_granule(const VT &v, int idx, uint32_t mask, int f,
                                               float *__restrict__ cakes, int64_t nb,
@@ -258,19 +340,9 @@ __device__ __forceinline__ void flush_granule(const VT &v, int idx, uint32_t mas
   }
 }
 
-// Registers of one tile's prefetch: this thread's share of the frame
-// boxes (kStageVecs staging vectors), of the bin records and entries.
-typedef int i4v __attribute__((ext_vector_type(4)));
-typedef unsigned u2v __attribute__((ext_vector_type(2)));
-
-template <typename TIn, int V> struct TilePrefetch {
-  VecLoad<TIn, V> v[kStageVecs];
-  i4v r0, r1;  // one 32-byte bin record
-  u2v e[kTileEnts / (64 * kWaves)];
-};
-
 struct TileHdr {
-  int y0, z0, bwp, bh, ks, S, b_beg, nbt, e_beg, net, ra, nc, pv;
+  int y0, z0, bw, bh, b_beg, nbt, e_beg, net, ra, nc, vpr;
+  uint32_t magic;
 };
 
 __device__ __forceinline__ TileHdr tile_hdr(const int32_t *__restrict__ tiles, int t) {
@@ -278,30 +350,32 @@ __device__ __forceinline__ TileHdr tile_hdr(const int32_t *__restrict__ tiles, i
   TileHdr h;
   h.y0 = T[0];
   h.z0 = T[1];
-  h.bwp = T[2];
+  h.bw = T[2];
   h.bh = T[3];
-  h.ks = T[4];
-  h.S = T[5];
-  h.b_beg = T[6];
-  h.nbt = T[7] - T[6];
-  h.e_beg = T[8];
-  h.net = T[9] - T[8];
-  h.ra = T[10];
-  h.nc = T[11];
-  h.pv = T[12];
+  h.b_beg = T[4];
+  h.nbt = T[5] - T[4];
+  h.e_beg = T[6];
+  h.net = T[7] - T[6];
+  h.ra = T[8];
+  h.nc = T[9];
+  h.vpr = T[10];
+  h.magic = (uint32_t)T[11];
   return h;
 }
 
-// Staging slot u of a thread: frame wave + kWaves * (u >> spf_log) (the
-// wave's frames), vector q = lane + 64 * (u & (spf - 1)) of that frame's
-// box.  Boxes are at most 64 vectors wide, so a lane keeps its column.
-template <typename TIn, int V>
-__device__ __forceinline__ void tile_issue(TilePrefetch<TIn, V> &P, const TileHdr &h, int tid,
-                                           int lane, int wave, int spf_log,
+// Registers of one tile's prefetch: this thread's share of the box
+// (kStageVecs 16-byte vectors), of the bin records and of the entries.
+struct TilePrefetch {
+  u4v v[kStageVecs];
+  i4v r0, r1;
+  u2v e[kTileEnts / (64 * kWaves)];
+};
+
+template <typename TL>
+__device__ __forceinline__ void tile_issue(TilePrefetch &P, const TileHdr &h, int tid,
                                            const BinRec *__restrict__ recs,
                                            const uint2 *__restrict__ ents,
-                                           const TIn *__restrict__ frames, int64_t frame_stride,
-                                           int n_frames, int ny) {
+                                           const TL *__restrict__ xt, int ny) {
   if (tid < h.nbt) {
     const i4v *rp = reinterpret_cast<const i4v *>(recs + h.b_beg + tid);
     P.r0 = rp[0];
@@ -312,24 +386,25 @@ __device__ __forceinline__ void tile_issue(TilePrefetch<TIn, V> &P, const TileHd
     const int k = tid + 64 * kWaves * q;
     if (k < h.net) P.e[q] = reinterpret_cast<const u2v *>(ents)[h.e_beg + k];
   }
-  if (h.bwp > 0) {
-    const int cv = lane & ((1 << h.ks) - 1), r0 = lane >> h.ks, rstep = 64 >> h.ks;
-    const TIn *col = frames + (int64_t)(h.z0 + r0) * ny + h.y0 + cv * V;
+  const int jobs = h.bh * h.vpr;
 #pragma unroll
-    for (int u = 0; u < kStageVecs; ++u) {
-      const int f = wave + kWaves * (u >> spf_log);
-      const int s = u & ((1 << spf_log) - 1);
-      if (f < n_frames && lane + 64 * s < h.pv)
-        P.v[u] = *reinterpret_cast<const VecLoad<TIn, V> *>(
-            col + (int64_t)f * frame_stride + (int64_t)(s * rstep) * ny);
+  for (int u = 0; u < kStageVecs; ++u) {
+    const int j = tid + 64 * kWaves * u;
+    if (j < jobs) {
+      const int row = h.vpr == 1 ? j : (int)__umulhi((uint32_t)j, h.magic);
+      const int v = j - row * h.vpr;
+      const char *src = reinterpret_cast<const char *>(
+                            xt + ((int64_t)(h.z0 + row) * ny + h.y0) * kXtFrames) +
+                        v * 16;
+      P.v[u] = *reinterpret_cast<const u4v *>(src);
     }
   }
 }
 
-template <typename TIn, typename TL, int V>
-__device__ __forceinline__ void tile_commit(const TilePrefetch<TIn, V> &P, const TileHdr &h,
-                                            int tid, int lane, int wave, int spf_log,
-                                            int n_frames, BinRec *lbin, uint2 *lent, TL *plane) {
+template <typename TL>
+__device__ __forceinline__ void tile_commit(const TilePrefetch &P, const TileHdr &h, int tid,
+                                            BinRec *lbin, uint2 *lent, char *plane) {
+  constexpr int RB = kXtFrames * (int)sizeof(TL);  // bytes per pixel row
   if (tid < h.nbt) {
     i4v *rp = reinterpret_cast<i4v *>(lbin + tid);
     rp[0] = P.r0;
@@ -340,36 +415,37 @@ __device__ __forceinline__ void tile_commit(const TilePrefetch<TIn, V> &P, const
     const int k = tid + 64 * kWaves * q;
     if (k < h.net) reinterpret_cast<u2v *>(lent)[k] = P.e[q];
   }
-  // Zero pad element S-1 of every plane: the padding entries of each bin
-  // (weight 0) point at it.
-  if (tid < n_frames) plane[tid * h.S + h.S - 1] = static_cast<TL>(0);
-  if (h.bwp > 0) {
-    const int cv = lane & ((1 << h.ks) - 1), r0 = lane >> h.ks, rstep = 64 >> h.ks;
-    const int colo = r0 * h.bwp + cv * V;
+  // The zero pixel row after the box: the padding entries point at it.
+  if (tid < RB / 16)
+    reinterpret_cast<u4v *>(plane + (size_t)h.bw * h.bh * RB)[tid] = 0u;
+  const int jobs = h.bh * h.vpr;
 #pragma unroll
-    for (int u = 0; u < kStageVecs; ++u) {
-      const int f = wave + kWaves * (u >> spf_log);
-      const int s = u & ((1 << spf_log) - 1);
-      if (f < n_frames && lane + 64 * s < h.pv) {
-        const int dst = f * h.S + colo + s * rstep * h.bwp;
-        if constexpr (sizeof(TL) == 2 && V % 2 == 0) {
-          // pairs of 16-bit values as 32-bit words (dst is even)
-#pragma unroll
-          for (int q = 0; q < V; q += 2)
-            *reinterpret_cast<uint32_t *>(plane + dst + q) =
-                (uint32_t)(uint16_t)P.v[u].v[q] | ((uint32_t)(uint16_t)P.v[u].v[q + 1] << 16);
-        } else {
-#pragma unroll
-          for (int q = 0; q < V; ++q) plane[dst + q] = static_cast<TL>(P.v[u].v[q]);
-        }
-      }
+  for (int u = 0; u < kStageVecs; ++u) {
+    const int j = tid + 64 * kWaves * u;
+    if (j < jobs) {
+      const int row = h.vpr == 1 ? j : (int)__umulhi((uint32_t)j, h.magic);
+      const int v = j - row * h.vpr;
+      *reinterpret_cast<u4v *>(plane + (size_t)row * h.bw * RB + v * 16) = P.v[u];
     }
   }
 }
 
-// Compensated sum: products are added in blocks of 8 with plain FMAs and
-// each block sum enters a Kahan accumulator (relative error about 8 ulp of
-// a block plus 2 ulp, independent of the entry count).
+// Sum over the four 16-lane groups of a wave at the same position in the
+// group: lane l of every group gets x[l] + x[l+16] + x[l+32] + x[l+48],
+// added in the same order everywhere (gfx950 lane swaps, no LDS).
+__device__ __forceinline__ float sum_groups(float v) {
+  const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false,
+                                                  false);
+  v = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+  const auto b = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false,
+                                                  false);
+  return __uint_as_float(b[0]) + __uint_as_float(b[1]);
+}
+
+// Compensated sum: products are added with plain FMAs over blocks of up to
+// 32 entries; a bin with more entries adds its blocks through a Kahan
+// accumulator (relative error about 32 ulp of a block plus 2 ulp,
+// independent of the entry count).
 __device__ __forceinline__ void kahan_add(Kahan &k, float x) {
   const float y = x - k.c;
   const float t = k.s + y;
@@ -377,70 +453,60 @@ __device__ __forceinline__ void kahan_add(Kahan &k, float x) {
   k.s = t;
 }
 
-template <typename TIn, typename TL, int FPL, int V, int G>
+template <typename TL, int G>
 __global__ __launch_bounds__(64 * kWaves) void k_batched(
-    const int32_t *__restrict__ chunks, int n_chunks, const int32_t *__restrict__ tiles,
-    const BinRec *__restrict__ recs, const uint2 *__restrict__ ents,
-    const TIn *__restrict__ frames, int64_t frame_stride, int n_frames, int spf_log, int ny,
-    int64_t nb, float *__restrict__ cakes, float *__restrict__ part) {
+    const int32_t *__restrict__ chunks, const int32_t *__restrict__ xcd,
+    const int32_t *__restrict__ tiles, const BinRec *__restrict__ recs,
+    const uint2 *__restrict__ ents, const TL *__restrict__ xt,
+    int n_frames, int ny, int64_t nb, float *__restrict__ cakes, float *__restrict__ part) {
   extern __shared__ __attribute__((aligned(16))) char lds_raw[];
-  constexpr int NP = 64 * FPL;  // planes (frames) per launch
-  constexpr int RB = kWaves * G;
+  typedef typename Quad<TL>::type QT;
+  constexpr int RB = kXtFrames * (int)sizeof(TL);
+  constexpr int QB = 4 * (int)sizeof(TL);  // bytes of one lane's 4 frames
+  constexpr int RBAND = kWaves * G;
+  constexpr int LG = 16;
+  typedef typename GranVec<LG>::type VT;
   BinRec *lbin = reinterpret_cast<BinRec *>(lds_raw);
   uint2 *lent = reinterpret_cast<uint2 *>(lds_raw + kTileBins * sizeof(BinRec));
-  TL *plane = reinterpret_cast<TL *>(lds_raw + kTileBins * sizeof(BinRec) +
-                                     kTileEnts * sizeof(uint2));
+  char *plane = lds_raw + kTileBins * sizeof(BinRec) + kTileEnts * sizeof(uint2);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
+  const int grp = lane >> 4, l16 = lane & 15;
+  const int frame = 4 * l16 + grp;  // the frame this lane keeps
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int per = (n_chunks + 7) / 8;
-  const int c = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  if (c >= n_chunks) return;
+  const int x = blockIdx.x & 7;
+  const int c = xcd[x] + (int)(blockIdx.x >> 3);
+  if (c >= xcd[x + 1]) return;
   const int t_beg = chunks[4 * c], t_end = chunks[4 * c + 1];
   const bool want_lo = part != nullptr;
-  constexpr int LG = 16 / FPL;
-  typedef typename GranVec<LG>::type VT;
-  float lo[G][4][FPL];
-  VT ob[G][FPL];
+  float lo[G][4];
+  VT ob[G];
   int og[G];
   uint32_t om[G];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     og[g] = -1;
     om[g] = 0u;
+    ob[g] = 0.f;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-#pragma unroll
-      for (int j = 0; j < FPL; ++j) lo[g][k][j] = 0.f;
-#pragma unroll
-    for (int j = 0; j < FPL; ++j) ob[g][j] = 0.f;
+    for (int k = 0; k < 4; ++k) lo[g][k] = 0.f;
   }
   const bool vec_ok = ((nb & 3) == 0) && (((uintptr_t)cakes & 15) == 0);
+  const char *lpl = plane + l16 * QB;  // this lane's 4 frames in a pixel row
 
-  // Software pipeline: the loads of tile t+1 are in flight while the waves
-  // reduce tile t.
-  TilePrefetch<TIn, V> P;
+  TilePrefetch P;
   TileHdr hn = tile_hdr(tiles, t_beg);
-  tile_issue(P, hn, tid, lane, wave, spf_log, recs, ents, frames, frame_stride, n_frames, ny);
+  tile_issue(P, hn, tid, recs, ents, xt, ny);
   for (int t = t_beg; t < t_end; ++t) {
     const TileHdr h = hn;
     __syncthreads();  // every wave is done with the previous tile's LDS
-    tile_commit<TIn, TL, V>(P, h, tid, lane, wave, spf_log, n_frames, lbin, lent, plane);
+    tile_commit<TL>(P, h, tid, lbin, lent, plane);
     __syncthreads();
     if (t + 1 < t_end) {
       hn = tile_hdr(tiles, t + 1);
-      tile_issue(P, hn, tid, lane, wave, spf_log, recs, ents, frames, frame_stride, n_frames,
-                 ny);
+      tile_issue(P, hn, tid, recs, ents, xt, ny);
     }
     const int nrows = h.nbt / h.nc;
-    // Byte address of this lane's plane(s); entries hold byte offsets.
-    // Lanes past the batch read the last staged plane; their sums are
-    // dropped.
-    const char *pb[FPL];
-#pragma unroll
-    for (int j = 0; j < FPL; ++j)
-      pb[j] = reinterpret_cast<const char *>(plane) +
-              (size_t)min(lane + 64 * j, n_frames - 1) * h.S * sizeof(TL);
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const int jr = kWaves * g + wave - h.ra;  // record row inside the tile
@@ -450,91 +516,94 @@ __global__ __launch_bounds__(64 * kWaves) void k_batched(
       for (int e = 0; e < h.nc; ++e) {
         const BinRec r = rn;
         if (e + 1 < h.nc) rn = rrow[e + 1];  // next record early: hides its LDS latency
-        Kahan acc[FPL];
+        float tot[4];
         if (!r.tb.w) {
-          // Staged entries come in whole groups of 8 (zero-weight padding on
-          // the plane's zero pad element), 16-byte aligned in LDS.
           const int kb = r.tb.y - h.e_beg, ke = r.tb.z - h.e_beg;
+          float gs[4] = {0.f, 0.f, 0.f, 0.f};
+          Kahan acc[4];
+          int nblk = 0;
+          bool comp = false;
           for (int k = kb; k < ke; k += 8) {
-            uint2 en[8];
-            const uint4 *ep = reinterpret_cast<const uint4 *>(lent + k);
+            const u4v two = *reinterpret_cast<const u4v *>(lent + k + 2 * grp);
+            const QT da = *reinterpret_cast<const QT *>(lpl + two.x);
+            const QT db = *reinterpret_cast<const QT *>(lpl + two.z);
+            float fa[4], fb[4];
+            Quad<TL>::unpack(da, fa);
+            Quad<TL>::unpack(db, fb);
+            const float wa = __uint_as_float(two.y), wb = __uint_as_float(two.w);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const uint4 two = ep[q];
-              en[2 * q] = make_uint2(two.x, two.y);
-              en[2 * q + 1] = make_uint2(two.z, two.w);
+            for (int j = 0; j < 4; ++j) gs[j] = fmaf(fb[j], wb, fmaf(fa[j], wa, gs[j]));
+            if (++nblk == 4 && k + 8 < ke) {  // more than 32 entries: compensate
+#pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                kahan_add(acc[j], gs[j]);
+                gs[j] = 0.f;
+              }
+              nblk = 0;
+              comp = true;
             }
-            TL pv[8][FPL];
+          }
+          if (comp) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
+            for (int j = 0; j < 4; ++j) {
+              kahan_add(acc[j], gs[j]);
+              tot[j] = acc[j].value();
+            }
+          } else {
 #pragma unroll
-              for (int j = 0; j < FPL; ++j)
-                pv[q][j] = *reinterpret_cast<const TL *>(pb[j] + en[q].x);
-            float gs[FPL];
-#pragma unroll
-            for (int j = 0; j < FPL; ++j) gs[j] = to_f(pv[0][j]) * __uint_as_float(en[0].y);
-#pragma unroll
-            for (int q = 1; q < 8; ++q)
-#pragma unroll
-              for (int j = 0; j < FPL; ++j)
-                gs[j] = fmaf(to_f(pv[q][j]), __uint_as_float(en[q].y), gs[j]);
-#pragma unroll
-            for (int j = 0; j < FPL; ++j) kahan_add(acc[j], gs[j]);
+            for (int j = 0; j < 4; ++j) tot[j] = gs[j];
           }
         } else {
-          for (int k = r.tb.y; k < r.tb.z; ++k) {  // entries stay in global memory
+          Kahan acc[4];
+          // Entries stay in global memory: group g takes entries g, g+4, ...
+          for (int k = r.tb.y + grp; k < r.tb.z; k += 4) {
             const uint2 en = ents[k];
+            const QT d = *reinterpret_cast<const QT *>(
+                reinterpret_cast<const char *>(xt + (int64_t)en.x * kXtFrames) + l16 * QB);
+            float fv[4];
+            Quad<TL>::unpack(d, fv);
 #pragma unroll
-            for (int j = 0; j < FPL; ++j) {
-              const int f = lane + 64 * j;
-              if (f < n_frames)
-                kahan_add(acc[j],
-                          to_f(frames[(int64_t)f * frame_stride + en.x]) * __uint_as_float(en.y));
-            }
+            for (int j = 0; j < 4; ++j) kahan_add(acc[j], fv[j] * __uint_as_float(en.y));
           }
+#pragma unroll
+          for (int j = 0; j < 4; ++j) tot[j] = acc[j].value();
         }
+        // The four groups hold partial sums of the same frames: add them.
+        float sum[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sum[j] = sum_groups(tot[j]);
+        const float mine = grp == 0 ? sum[0] : grp == 1 ? sum[1] : grp == 2 ? sum[2] : sum[3];
         const int gr = r.tb.x / LG, gk = r.tb.x - gr * LG;
         if (gr != og[g]) {
-          if (om[g])
-#pragma unroll
-            for (int j = 0; j < FPL; ++j)
-              if (lane + 64 * j < n_frames)
-                flush_granule<LG>(ob[g][j], og[g], om[g], lane + 64 * j, cakes, nb, vec_ok);
+          if (om[g] && frame < n_frames) flush_granule<LG>(ob[g], og[g], om[g], frame, cakes, nb,
+                                                            vec_ok);
           og[g] = gr;
           om[g] = 0u;
         }
         om[g] |= 1u << gk;
-#pragma unroll
-        for (int j = 0; j < FPL; ++j) {
-          const float val = (acc[j].value() - r.info.y) * fabsf(r.info.x);
-          const float o = flagged(r.info.x) ? NAN : val;
-          ob[g][j][gk] = o;
-          if (o != 0.f && !isnan(o)) {
-            lo[g][0][j] += o;
-            lo[g][1][j] += 1.f;
-          }
-          if (r.a > 1e-9f && !isnan(o)) {
-            lo[g][2][j] += o * r.a;
-            lo[g][3][j] += o;
-          }
+        const float val = (mine - r.info.y) * fabsf(r.info.x);
+        const float o = flagged(r.info.x) ? NAN : val;
+        ob[g][gk] = o;
+        if (o != 0.f && !isnan(o)) {
+          lo[g][0] += o;
+          lo[g][1] += 1.f;
+        }
+        if (r.a > 1e-9f && !isnan(o)) {
+          lo[g][2] += o * r.a;
+          lo[g][3] += o;
         }
       }
     }
   }
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    if (om[g])
-#pragma unroll
-      for (int j = 0; j < FPL; ++j)
-        if (lane + 64 * j < n_frames)
-          flush_granule<LG>(ob[g][j], og[g], om[g], lane + 64 * j, cakes, nb, vec_ok);
+    if (om[g] && frame < n_frames)
+      flush_granule<LG>(ob[g], og[g], om[g], frame, cakes, nb, vec_ok);
     if (want_lo) {
-      // part[c][row][k][NP] for the band row this slot holds.
-      float *Pp = part + ((int64_t)c * RB + kWaves * g + wave) * 4 * NP;
+      // part[c][row][k][64] for the band row this slot holds.
+      float *Pp = part + ((int64_t)c * RBAND + kWaves * g + wave) * 4 * kXtFrames;
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
-#pragma unroll
-        for (int j = 0; j < FPL; ++j) Pp[k * NP + lane + 64 * j] = lo[g][k][j];
+      for (int k = 0; k < 4; ++k) Pp[k * kXtFrames + frame] = lo[g][k];
     }
   }
 }
@@ -718,91 +787,69 @@ int launch_tile_recs(hipStream_t s, const Map &m, Tiles &t, const float2 *binfo)
   return MC_OK;
 }
 
-template <typename TIn, int FPL, int V, int G>
-static int launch_batched_v(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
-                            int mask_nan, const void *frames, int n_frames, float *cakes,
-                            float *lineouts) {
-  using TL = typename std::conditional<sizeof(TIn) <= 2, uint16_t, float>::type;
-  auto kern = k_batched<TIn, TL, FPL, V, G>;
-  if (V > 1 && ((uintptr_t)frames % (sizeof(TIn) * V)) != 0) {
-    set_error("frame buffer not %d-byte aligned for vector staging", (int)(sizeof(TIn) * V));
-    return MC_ERR_ARG;
-  }
-  if (n_frames < 1 || n_frames > t.batch) {
-    set_error("batch of %d frames outside 1..%d", n_frames, t.batch);
-    return MC_ERR_ARG;
-  }
+size_t batch_xt_bytes(const Map &m, int dtype) {
+  return (size_t)m.npx() * kXtFrames * (dtype_size(dtype) <= 2 ? 2 : 4);
+}
+
+template <typename TIn, typename TL, int G>
+static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mask_nan,
+                            const void *frames, int n_frames, float *cakes, float *lineouts,
+                            void *xt) {
+  // 1. pixel-major batch
+  const int64_t npx = m.npx();
+  const int vec_ok = (npx % 64 == 0) && (((uintptr_t)frames & 15) == 0) ? 1 : 0;
+  hipLaunchKernelGGL((k_transpose<TIn, TL>), dim3((unsigned)((npx + 63) / 64)), dim3(256), 0, s,
+                     (const TIn *)frames, npx, n_frames, npx, vec_ok, (TL *)xt);
+  MC_HIP(hipGetLastError());
+  // 2. tiles
+  auto kern = k_batched<TL, G>;
   const size_t lds = t.lds_needed;
   if (lds > 65536)
     MC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
   float *part = lineouts ? t.part.as<float>() : nullptr;
-  const unsigned grid = 8u * (unsigned)((t.n_chunks + 7) / 8);
-  (void)binfo;  // folded into the tile records (launch_tile_recs)
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kWaves), lds, s, t.chunks.as<int32_t>(),
-                     t.n_chunks, t.tiles.as<int32_t>(), t.recs.as<BinRec>(), t.ents.as<uint2>(),
-                     (const TIn *)frames, m.npx(), n_frames, t.spf_log, m.ny, m.nb(), cakes, part);
+  hipLaunchKernelGGL(kern, dim3((unsigned)t.grid), dim3(64 * kWaves), lds, s,
+                     t.chunks.as<int32_t>(), t.xcd.as<int32_t>(), t.tiles.as<int32_t>(),
+                     t.recs.as<BinRec>(), t.ents.as<uint2>(), (const TL *)xt, n_frames, m.ny,
+                     m.nb(), cakes, part);
   MC_HIP(hipGetLastError());
+  // 3. lineouts
   if (lineouts) {
     const int64_t n = (int64_t)n_frames * m.n_r;
     hipLaunchKernelGGL(k_lineout_finalize, dim3(blocks_for(n, 256)), dim3(256), 0, s, part,
                        t.band_chunk.as<int32_t>(), m.d_row_static.as<float4>(), n_frames,
-                       m.n_r, t.rows, 64 * FPL, mask_nan, lineouts);
+                       m.n_r, t.rows, kXtFrames, mask_nan, lineouts);
     MC_HIP(hipGetLastError());
   }
   return MC_OK;
 }
 
-template <typename TIn, int FPL, int V>
-static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
-                            int mask_nan, const void *frames, int n_frames, float *cakes,
-                            float *lineouts) {
-  switch (t.rows / kWaves) {
-    case 1:
-      return launch_batched_v<TIn, FPL, V, 1>(s, m, t, binfo, mask_nan, frames, n_frames, cakes,
-                                              lineouts);
-    case 2:
-      return launch_batched_v<TIn, FPL, V, 2>(s, m, t, binfo, mask_nan, frames, n_frames, cakes,
-                                              lineouts);
-    default:
-      set_error("rows per band %d not in {16, 32}", t.rows);
-      return MC_ERR_ARG;
-  }
-}
-
-template <typename TIn, int FPL>
-static int launch_batched_t(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
-                            int mask_nan, const void *frames, int n_frames, float *cakes,
-                            float *lineouts) {
-  constexpr int VV = sizeof(TIn) < 8 ? 8 / (int)sizeof(TIn) : 1;
-  if (t.vec == VV)
-    return launch_batched_g<TIn, FPL, VV>(s, m, t, binfo, mask_nan, frames, n_frames, cakes,
-                                          lineouts);
-  if constexpr (VV != 1) {
-    if (t.vec != 1) {
-      set_error("staging vector %d not supported", t.vec);
-      return MC_ERR_STATE;
-    }
-    return launch_batched_g<TIn, FPL, 1>(s, m, t, binfo, mask_nan, frames, n_frames, cakes,
-                                         lineouts);
-  }
-  set_error("staging vector %d not supported", t.vec);
-  return MC_ERR_STATE;
-}
-
 int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
                    int mask_nan, const void *frames, int dtype, int n_frames, float *cakes,
-                   float *lineouts) {
+                   float *lineouts, void *xt) {
+  (void)binfo;  // folded into the tile records (launch_tile_recs)
+  if (n_frames < 1 || n_frames > kXtFrames) {
+    set_error("batched pass of %d frames outside 1..%d", n_frames, kXtFrames);
+    return MC_ERR_ARG;
+  }
+  if ((dtype_size(dtype) <= 2 ? 2 : 4) != t.in_size) {
+    set_error("tile layout built for %d-byte batch elements", t.in_size);
+    return MC_ERR_STATE;
+  }
   return dispatch_dtype(dtype, [&](auto tag) -> int {
     using T = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
-    if (t.fpl == 1)
-      return launch_batched_t<T, 1>(s, m, t, binfo, mask_nan, frames, n_frames, cakes,
-                                    lineouts);
-    if (t.fpl == 2)
-      return launch_batched_t<T, 2>(s, m, t, binfo, mask_nan, frames, n_frames, cakes,
-                                    lineouts);
-    set_error("frames per lane %d not in {1, 2}", t.fpl);
-    return MC_ERR_STATE;
+    using TL = typename std::conditional<sizeof(T) <= 2, uint16_t, float>::type;
+    switch (t.rows / kWaves) {
+      case 1:
+        return launch_batched_g<T, TL, 1>(s, m, t, mask_nan, frames, n_frames, cakes, lineouts,
+                                          xt);
+      case 2:
+        return launch_batched_g<T, TL, 2>(s, m, t, mask_nan, frames, n_frames, cakes, lineouts,
+                                          xt);
+      default:
+        set_error("rows per band %d not in {16, 32}", t.rows);
+        return MC_ERR_ARG;
+    }
   });
 }
 

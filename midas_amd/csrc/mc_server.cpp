@@ -54,6 +54,7 @@ struct Slot {
   hipStream_t s = nullptr;
   hipEvent_t done = nullptr;
   DBuf frames, cakes, lo;
+  DBuf xt;  // pixel-major batch of the slot's passes
   float *h_cakes = nullptr, *h_lo = nullptr;  // pinned
   size_t cap_frames = 0, cap_cakes = 0, cap_lo = 0;
   std::vector<Req> reqs;
@@ -99,16 +100,17 @@ struct mc_server {
         mc_map_info_get(g.m, &info);
         b += info.hbm_bytes;
       }
-    for (auto &s : slots) b += s.frames.bytes + s.cakes.bytes + s.lo.bytes;
+    for (auto &s : slots) b += s.frames.bytes + s.cakes.bytes + s.lo.bytes + s.xt.bytes;
     return b;
   }
 };
 
 namespace {
 
-int grow_slots(mc_server *sv, size_t fb, size_t cf, size_t lf) {
+int grow_slots(mc_server *sv, size_t fb, size_t cf, size_t lf, size_t xb) {
   for (auto &s : sv->slots) {
     int rc;
+    if (s.xt.bytes < xb && (rc = s.xt.alloc(xb))) return rc;
     if (s.cap_frames < fb) {
       if ((rc = s.frames.alloc(fb))) return rc;
       s.cap_frames = fb;
@@ -173,7 +175,7 @@ void scheduler(mc_server *sv) {
     if (!rc) {
       std::lock_guard<std::mutex> mlk(G.m->mu);
       rc = run_integrate(G.m, S.s, S.frames.p, G.dtype, nf, G.mode, G.flags, S.cakes.as<float>(),
-                         S.lo.as<float>(), false, false);
+                         S.lo.as<float>(), false, false, &S.xt);
     }
     if (!rc && hipMemcpyAsync(S.h_cakes, S.cakes.p, (size_t)nf * G.cake_floats * 4,
                               hipMemcpyDeviceToHost, S.s) != hipSuccess)
@@ -286,11 +288,12 @@ int mc_server_add_geometry(mc_server *sv, mc_map *m, int32_t dtype, int32_t mode
   G.frame_bytes = (size_t)m->npx() * dtype_size(dtype);
   G.cake_floats = (size_t)m->nb();
   G.lo_floats = (size_t)3 * m->n_r;
-  size_t fb = 0, cf = 0, lf = 0;
+  size_t fb = 0, cf = 0, lf = 0, xb = batch_xt_bytes(*m, dtype);
   for (auto &g : sv->geos) {
     fb = std::max(fb, g.frame_bytes);
     cf = std::max(cf, g.cake_floats);
     lf = std::max(lf, g.lo_floats);
+    xb = std::max(xb, batch_xt_bytes(*g.m, g.dtype));
   }
   fb = std::max(fb, G.frame_bytes) * sv->max_batch;
   cf = std::max(cf, G.cake_floats) * sv->max_batch;
@@ -307,7 +310,8 @@ int mc_server_add_geometry(mc_server *sv, mc_map *m, int32_t dtype, int32_t mode
   for (auto &s : sv->slots)
     need += (int64_t)(std::max(fb, s.cap_frames) - s.frames.bytes) +
             (int64_t)(std::max(cf, s.cap_cakes) * 4 - s.cakes.bytes) +
-            (int64_t)(std::max(lf, s.cap_lo) * 4 - s.lo.bytes);
+            (int64_t)(std::max(lf, s.cap_lo) * 4 - s.lo.bytes) +
+            (int64_t)(std::max(xb, s.xt.bytes) - s.xt.bytes);
   if (need > sv->budget) {
     sv->rejected++;
     set_error("geometry needs %lld bytes resident, budget is %lld", (long long)need,
@@ -315,7 +319,7 @@ int mc_server_add_geometry(mc_server *sv, mc_map *m, int32_t dtype, int32_t mode
     return MC_ERR_BUDGET;
   }
   MC_HIP(hipSetDevice(sv->ctx->device));
-  if ((rc = grow_slots(sv, fb, cf, lf))) return rc;
+  if ((rc = grow_slots(sv, fb, cf, lf, xb))) return rc;
   sv->geos.push_back(G);
   sv->free_host.emplace_back();
   // Pinned staging for the first frames of this geometry, so that early

@@ -101,7 +101,7 @@ def test_small_lds_budget_uses_direct_bins():
     rng = np.random.default_rng(11)
     frames = rng.poisson(200.0, (9, P.NrPixelsZ, P.NrPixelsY)).astype(np.uint16)
     g = ma.build_map(P)
-    g.set_tuning(batch=8, rows_per_group=16, lds_bytes=24576 + 8 * 2 * 64)
+    g.set_tuning(batch=8, rows_per_group=16, lds_bytes=24576 + 128 * 48)
     tight = g.layout_info()
     assert tight["direct_bins"] > 0
     cb, lb = ma.integrate_batch(frames, g, mode="bilinear")
