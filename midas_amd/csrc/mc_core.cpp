@@ -239,7 +239,7 @@ struct BBox {
 
 // LDS of the batched kernel: bin records, staged entries, the box of the
 // pixel-major batch (one row of 64 frames per pixel) and a zero pixel row.
-constexpr int kTileBins = 256, kTileEnts = 2048, kBinRecBytes = 32, kWaves = 16, kTileInts = 12,
+constexpr int kTileBins = 256, kTileEnts = 2048, kBinRecBytes = 32, kWaves = 16, kTileInts = 16,
               kStageVecs = 4, kXtFrames = 64;
 
 // Tiles of the batched kernel.  Bands of H cake rows; inside a band, tiles
@@ -433,12 +433,14 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
           recs[(size_t)j * T.nc + e] = make_int4((int32_t)b, eb, (int32_t)ents.size(), 1);
           n_direct++;
         }
+      // Every tile's entries start at a multiple of 8 (16-byte staging).
+      while (ents.size() % 8) ents.push_back(make_uint2(0u, 0u));
       tbins.insert(tbins.end(), recs.begin(), recs.end());
       const int vpr = bw * RB / 16;  // staging vectors per box row
       // floor(j / vpr) = umulhi(j, magic) for j < 2^32 / vpr (j < 4096 here)
       const uint32_t magic = vpr > 1 ? (uint32_t)(((uint64_t)1 << 32) / (uint64_t)vpr + 1) : 0u;
       tiles.insert(tiles.end(), {bx0, bz0, bw, bh, b_beg, (int32_t)tbins.size(), e_beg, e_end,
-                                 T.ra, T.nc, vpr, (int32_t)magic});
+                                 T.ra, T.nc, vpr, (int32_t)magic, T.nrows, 0, 0, 0});
       chunks[chunks.size() - 3] = (int32_t)(tiles.size() / kTileInts);
       const int64_t tcost = all_ents + kTileCost + kBinCost * T.nrows * T.nc;
       acc += tcost;

@@ -101,7 +101,7 @@ struct Tiles {
   int n_chunks = 0, n_bands = 0;
   size_t lds_needed = 0;
   DBuf chunks;       // int32 x4 per chunk: tile_begin tile_end band_row0 band_rows
-  DBuf tiles;        // int32 x12 per tile (see mc_kernels.hip)
+  DBuf tiles;        // int32 x16 per tile (see mc_kernels.hip)
   DBuf tbins;        // int32 x4 per tile bin: cake bin, ent_begin, ent_end, direct flag
   DBuf recs;         // 32-byte records: tbins + scale, dark term, area of one binfo table
   const void *recs_binfo = nullptr;

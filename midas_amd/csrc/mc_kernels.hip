@@ -195,19 +195,22 @@ __global__ __launch_bounds__(256) void k_profile(
 // tile t.  In the reduction the four 16-lane groups of a wave take the
 // entries of a bin in turn (group g the entries 2g, 2g+1 of each group of
 // 8) and lane l of a group holds frames 4l..4l+3: one 8-byte LDS read
-// serves 4 frames of one entry.  The groups' partial sums meet by lane
-// exchange at the end of the bin; the lane (g, l) then keeps frame 4l+g.
+// serves 4 frames of one entry.  The staged entries of a band row are one
+// contiguous run (records are row-major), so the entries of the next
+// group of 8 are read while the current one is reduced, across bin
+// boundaries.  The groups' partial sums meet by a lane-exchange
+// reduce-scatter at the end of the bin, after which lane (g, l) holds the
+// total of frame 4l+g.
 // Wave w owns the band rows j = w (mod 16), so it keeps the lineout sums
 // of its G rows in registers across the chunk; k_lineout_finalize adds the
 // chunks of each band.  Cake values are stored per frame in whole 16-bin
 // granules.  Bins whose own box exceeds the budget ("direct" bins) gather
 // from Xt in global memory.
 //   chunk: tile_begin tile_end band_row0 band_rows
-//   tile : y0 z0 bw bh b_beg b_end e_beg e_end ra nc vpr magic
-//          (box of bw x bh pixels at (y0, z0), band rows ra ..
-//          ra + (b_end - b_beg) / nc - 1, nc columns, records row-major;
-//          vpr = 16-byte staging vectors per box row, magic = the
-//          multiplier of the division by vpr)
+//   tile : y0 z0 bw bh b_beg b_end e_beg e_end ra nc vpr magic nrows 0 0 0
+//          (box of bw x bh pixels at (y0, z0), band rows ra .. ra + nrows - 1,
+//          nc columns, records row-major; vpr = 16-byte staging vectors per
+//          box row, magic = the multiplier of the division by vpr)
 //   rec  : cake bin, ent_begin, ent_end, direct flag, scale, dark term,
 //          in-bounds area
 //   ent  : (byte offset of the pixel row in the box, or global pixel if
@@ -220,7 +223,7 @@ __global__ __launch_bounds__(256) void k_profile(
 constexpr int kWaves = 16;
 constexpr int kTileBins = 256;    // bin records per tile (LDS)
 constexpr int kTileEnts = 2048;   // staged map entries per tile (LDS)
-constexpr int kTileInts = 12;
+constexpr int kTileInts = 16;
 constexpr int kStageVecs = 4;     // 16-byte staging vectors per thread and tile
 constexpr int kXtFrames = 64;     // frames per pixel row of the batch
 
@@ -343,6 +346,7 @@ __device__ __forceinline__ void flush_granule(const VT &v, int idx, uint32_t mas
 struct TileHdr {
   int y0, z0, bw, bh, b_beg, nbt, e_beg, net, ra, nc, vpr;
   uint32_t magic;
+  int nrows;
 };
 
 __device__ __forceinline__ TileHdr tile_hdr(const int32_t *__restrict__ tiles, int t) {
@@ -360,32 +364,28 @@ __device__ __forceinline__ TileHdr tile_hdr(const int32_t *__restrict__ tiles, i
   h.nc = T[9];
   h.vpr = T[10];
   h.magic = (uint32_t)T[11];
+  h.nrows = T[12];
   return h;
 }
 
 // Registers of one tile's prefetch: this thread's share of the box
-// (kStageVecs 16-byte vectors), of the bin records and of the entries.
+// (kStageVecs 16-byte vectors), of the bin records (512 vectors of 16
+// bytes) and of the entries (1024 vectors; a tile's entries start at a
+// multiple of 8).
 struct TilePrefetch {
   u4v v[kStageVecs];
-  i4v r0, r1;
-  u2v e[kTileEnts / (64 * kWaves)];
+  u4v rv, ev;
 };
+static_assert(kTileBins * sizeof(BinRec) / 16 <= 64 * kWaves, "record vectors per thread");
+static_assert(kTileEnts * sizeof(uint2) / 16 == 64 * kWaves, "entry vectors per thread");
 
 template <typename TL>
 __device__ __forceinline__ void tile_issue(TilePrefetch &P, const TileHdr &h, int tid,
                                            const BinRec *__restrict__ recs,
                                            const uint2 *__restrict__ ents,
                                            const TL *__restrict__ xt, int ny) {
-  if (tid < h.nbt) {
-    const i4v *rp = reinterpret_cast<const i4v *>(recs + h.b_beg + tid);
-    P.r0 = rp[0];
-    P.r1 = rp[1];
-  }
-#pragma unroll
-  for (int q = 0; q < kTileEnts / (64 * kWaves); ++q) {
-    const int k = tid + 64 * kWaves * q;
-    if (k < h.net) P.e[q] = reinterpret_cast<const u2v *>(ents)[h.e_beg + k];
-  }
+  if (tid < 2 * h.nbt) P.rv = reinterpret_cast<const u4v *>(recs + h.b_beg)[tid];
+  if (2 * tid < h.net) P.ev = reinterpret_cast<const u4v *>(ents + h.e_beg)[tid];
   const int jobs = h.bh * h.vpr;
 #pragma unroll
   for (int u = 0; u < kStageVecs; ++u) {
@@ -405,16 +405,8 @@ template <typename TL>
 __device__ __forceinline__ void tile_commit(const TilePrefetch &P, const TileHdr &h, int tid,
                                             BinRec *lbin, uint2 *lent, char *plane) {
   constexpr int RB = kXtFrames * (int)sizeof(TL);  // bytes per pixel row
-  if (tid < h.nbt) {
-    i4v *rp = reinterpret_cast<i4v *>(lbin + tid);
-    rp[0] = P.r0;
-    rp[1] = P.r1;
-  }
-#pragma unroll
-  for (int q = 0; q < kTileEnts / (64 * kWaves); ++q) {
-    const int k = tid + 64 * kWaves * q;
-    if (k < h.net) reinterpret_cast<u2v *>(lent)[k] = P.e[q];
-  }
+  if (tid < 2 * h.nbt) reinterpret_cast<u4v *>(lbin)[tid] = P.rv;
+  if (2 * tid < h.net) reinterpret_cast<u4v *>(lent)[tid] = P.ev;
   // The zero pixel row after the box: the padding entries point at it.
   if (tid < RB / 16)
     reinterpret_cast<u4v *>(plane + (size_t)h.bw * h.bh * RB)[tid] = 0u;
@@ -430,16 +422,21 @@ __device__ __forceinline__ void tile_commit(const TilePrefetch &P, const TileHdr
   }
 }
 
-// Sum over the four 16-lane groups of a wave at the same position in the
-// group: lane l of every group gets x[l] + x[l+16] + x[l+32] + x[l+48],
-// added in the same order everywhere (gfx950 lane swaps, no LDS).
-__device__ __forceinline__ float sum_groups(float v) {
-  const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false,
-                                                  false);
-  v = __uint_as_float(a[0]) + __uint_as_float(a[1]);
-  const auto b = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false,
-                                                  false);
-  return __uint_as_float(b[0]) + __uint_as_float(b[1]);
+// Reduce-scatter over the four 16-lane groups of a wave: t[j] holds a
+// group's partial sum for frame 4l+j (l = lane in the group); the result in
+// group g is the total over the groups of frame 4l+g.  Two gfx950 lane-half
+// swaps pair groups (0,2), (1,3), one 16-lane swap pairs (0,1), (2,3).
+__device__ __forceinline__ float group_scatter(const float t[4]) {
+  const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(t[0]), __float_as_uint(t[2]),
+                                                  false, false);
+  const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(t[1]), __float_as_uint(t[3]),
+                                                  false, false);
+  // groups 0,1: frames +0 / +1; groups 2,3: frames +2 / +3
+  const float s02 = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+  const float s13 = __uint_as_float(b[0]) + __uint_as_float(b[1]);
+  const auto c = __builtin_amdgcn_permlane16_swap(__float_as_uint(s02), __float_as_uint(s13),
+                                                  false, false);
+  return __uint_as_float(c[0]) + __uint_as_float(c[1]);
 }
 
 // Compensated sum: products are added with plain FMAs over blocks of up to
@@ -506,44 +503,55 @@ __global__ __launch_bounds__(64 * kWaves) void k_batched(
       hn = tile_hdr(tiles, t + 1);
       tile_issue(P, hn, tid, recs, ents, xt, ny);
     }
-    const int nrows = h.nbt / h.nc;
+    const int nrows = h.nrows;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const int jr = kWaves * g + wave - h.ra;  // record row inside the tile
       if (jr < 0 || jr >= nrows) continue;
       const BinRec *rrow = lbin + jr * h.nc;
       BinRec rn = rrow[0];
+      u4v tw = 0u;  // entries of the row's next group of 8 (read ahead)
+      bool primed = false;
       for (int e = 0; e < h.nc; ++e) {
         const BinRec r = rn;
         if (e + 1 < h.nc) rn = rrow[e + 1];  // next record early: hides its LDS latency
         float tot[4];
         if (!r.tb.w) {
-          const int kb = r.tb.y - h.e_beg, ke = r.tb.z - h.e_beg;
+          int k = r.tb.y - h.e_beg;
+          const int nblk = (r.tb.z - r.tb.y) >> 3;
+          if (!primed) {
+            tw = *reinterpret_cast<const u4v *>(lent + k + 2 * grp);
+            primed = true;
+          }
           float gs[4] = {0.f, 0.f, 0.f, 0.f};
-          Kahan acc[4];
-          int nblk = 0;
-          bool comp = false;
-          for (int k = kb; k < ke; k += 8) {
-            const u4v two = *reinterpret_cast<const u4v *>(lent + k + 2 * grp);
+          Kahan acc[4];  // set at the fourth group of 8, used only past it
+          for (int blk = 0; blk < nblk; ++blk, k += 8) {
+            const u4v two = tw;
             const QT da = *reinterpret_cast<const QT *>(lpl + two.x);
             const QT db = *reinterpret_cast<const QT *>(lpl + two.z);
+            // The run's next group (past the run's end this reads unused
+            // LDS words inside the allocation).
+            tw = *reinterpret_cast<const u4v *>(lent + k + 8 + 2 * grp);
             float fa[4], fb[4];
             Quad<TL>::unpack(da, fa);
             Quad<TL>::unpack(db, fb);
             const float wa = __uint_as_float(two.y), wb = __uint_as_float(two.w);
 #pragma unroll
             for (int j = 0; j < 4; ++j) gs[j] = fmaf(fb[j], wb, fmaf(fa[j], wa, gs[j]));
-            if (++nblk == 4 && k + 8 < ke) {  // more than 32 entries: compensate
+            if ((blk & 3) == 3 && blk + 1 < nblk) {  // more than 32 entries: compensate
 #pragma unroll
               for (int j = 0; j < 4; ++j) {
-                kahan_add(acc[j], gs[j]);
+                if (blk == 3) {
+                  acc[j].s = gs[j];
+                  acc[j].c = 0.f;
+                } else {
+                  kahan_add(acc[j], gs[j]);
+                }
                 gs[j] = 0.f;
               }
-              nblk = 0;
-              comp = true;
             }
           }
-          if (comp) {
+          if (nblk > 4) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
               kahan_add(acc[j], gs[j]);
@@ -568,11 +576,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_batched(
 #pragma unroll
           for (int j = 0; j < 4; ++j) tot[j] = acc[j].value();
         }
-        // The four groups hold partial sums of the same frames: add them.
-        float sum[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) sum[j] = sum_groups(tot[j]);
-        const float mine = grp == 0 ? sum[0] : grp == 1 ? sum[1] : grp == 2 ? sum[2] : sum[3];
+        // The four groups hold partial sums of the same frames.
+        const float mine = group_scatter(tot);
         const int gr = r.tb.x / LG, gk = r.tb.x - gr * LG;
         if (gr != og[g]) {
           if (om[g] && frame < n_frames) flush_granule<LG>(ob[g], og[g], om[g], frame, cakes, nb,
