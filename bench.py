@@ -15,11 +15,12 @@ percentiles (host clock around a synchronous one-frame call), the HBM
 roofline of the batched kernel and the FP64 CPU oracle (all host cores) on
 a bounded sample of the same frames.
 
-`roofline.traffic` is the measured HBM traffic per launch of the batched
-kernel (FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc runs of this script
-with --copy-calibration, scaled by the counters' reading for a 1 GiB device
-copy), read from profiles/traffic_batch_2048.json when that file was
-measured for the same tuning; otherwise null.
+`roofline.traffic` is the measured HBM traffic per batched launch (the
+transpose, k_batched and the lineout finalisation, as timed; FETCH_SIZE +
+WRITE_SIZE from rocprofv3 --pmc runs of this script with
+--copy-calibration, scaled by the counters' reading for a 1 GiB
+elementwise kernel), read from profiles/traffic_batch_2048.json when that
+file was measured for the same tuning and tile layout; otherwise null.
 """
 from __future__ import annotations
 
@@ -263,7 +264,9 @@ def main():
     if tf.exists():
         try:
             tj = json.loads(tf.read_text())
-            if (tj.get("batch"), tj.get("rows_per_group"), tj.get("lds_bytes")) == (B, H, L):
+            same_layout = all(layout.get(k) == v for k, v in tj.get("layout", {}).items())
+            if (tj.get("batch"), tj.get("rows_per_group"), tj.get("lds_bytes")) == (B, H, L) \
+                    and same_layout:
                 traffic = tj["hbm_bytes_per_launch"]
         except (ValueError, KeyError):
             traffic = None
