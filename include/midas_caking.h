@@ -32,6 +32,15 @@
  *                       PeakFit.h:114-116)
  *   mc_server_*         the frame server of IntegratorFitPeaksGPUStream.cu
  *                       :567-734, 2567-2883 and server.py FrameServer
+ *   mc_out_*            the result files: <data>.caked.hdf
+ *                       (IntegratorZarrOMP.c:1671-1687, 2084-2168),
+ *                       <prefix>_lineout.bin/.xy (:1514-1527, 1829-1863),
+ *                       <prefix>_fit.bin/_fit_per_eta.csv (:1530-1549,
+ *                       1879-1883, 2038-2058), <prefix>_caked_peaks.h5
+ *                       (PeakFitIO.c:52-267, IntegratorZarrOMP.c:2176-2195),
+ *                       and the GPU stream's lineout.bin,
+ *                       lineout_simple_mean.bin, Int2D.bin and its writer
+ *                       thread (IntegratorFitPeaksGPUStream.cu:506-565)
  */
 #ifndef MIDAS_CAKING_H
 #define MIDAS_CAKING_H
@@ -242,6 +251,69 @@ int mc_fit_cake_rings(mc_map *m, const float *cakes_dev, int32_t n_frames,
                       const double *r_centers, const double *ring_r,
                       int32_t n_rings, int32_t pad, int32_t model,
                       int32_t max_iter, double *out, int32_t out_on_device);
+
+/* ---- result files ---- */
+typedef struct mc_out mc_out;
+
+#define MC_OUT_CAKED_HDF 0x01u  /* <dir>/<name>.caked.hdf: /REtaMap, /InstrumentParameters */
+#define MC_OUT_PER_FRAME 0x02u  /* .caked.hdf /IntegrationResult/FrameNr_<i> (IndividualSave) */
+#define MC_OUT_SUM_FRAMES 0x04u /* .caked.hdf /SumFrames (SumImages) */
+#define MC_OUT_OMEGAS 0x08u     /* .caked.hdf /Omegas */
+#define MC_OUT_LINEOUT 0x10u    /* <dir>/<prefix>_lineout.bin and _lineout.xy */
+#define MC_OUT_STREAM 0x20u     /* <dir>/lineout.bin and lineout_simple_mean.bin */
+#define MC_OUT_INT2D 0x40u      /* <dir>/Int2D.bin */
+#define MC_OUT_PEAKS 0x80u      /* <prefix>_fit.bin, _fit_per_eta.csv, _caked_peaks.h5 */
+
+/* An output set.  <name> is the data file's name: the .caked.hdf file is
+ * <dir>/<name>.caked.hdf and <prefix> is <name> without its extension.
+ * n_r x n_eta cakes; omega_sum_frames > 0 adds /OmegaSumFrame sums of that
+ * many frames.  px, lsd (um) and wavelength (A) give 2-theta, Q and
+ * d-spacing.  q_mode selects Q (q_min + q_bin_size * (r + 0.5)) instead of
+ * 2-theta as the abscissa of _lineout.xy.  reta is the [5][n_r][n_eta]
+ * REtaMap of mc_map_reta (NULL: no /REtaMap); r_centers [n_r] (pixels) and
+ * eta_lo/eta_hi [n_eta] (degrees) are the bin axes; instrument [10] holds
+ * Polariz, Lam, SH_L, U, V, W, X, Y, Z, Distance (NULL: no
+ * /InstrumentParameters); source is the zarr_source attribute of
+ * _caked_peaks.h5.  The arrays are copied. */
+typedef struct mc_out_config {
+  int32_t n_r, n_eta;
+  uint32_t files;
+  int32_t omega_sum_frames;
+  double px, lsd, wavelength;
+  int32_t q_mode;
+  double q_min, q_bin_size;
+  const double *reta;
+  const double *r_centers;
+  const double *eta_lo, *eta_hi;
+  const double *instrument;
+  const char *source;
+} mc_out_config;
+
+/* One per-eta peak fit (IntegratorZarrOMP.c:2038-2058): the 7 PeakFit
+ * parameters (PeakFit.h:40-48) [area, center (px), sig, gam, FWHM, eta,
+ * chi2 or SNR] of peak `peak` in eta bin `eta_bin` of frame `frame`. */
+typedef struct mc_peak_row {
+  int32_t frame, eta_bin, peak, reserved;
+  double p[7];
+} mc_peak_row;
+
+int mc_out_open(const char *dir, const char *name, const mc_out_config *cfg, mc_out **out);
+/* Queues one frame: cake [n_r][n_eta] float32 (host), lineouts
+ * [3][n_r] float32 (rows MC_LINEOUT_*; needed by MC_OUT_STREAM, else may be
+ * NULL), omega in degrees, meta [4] = Temperature, Pressure, I, I0 (or
+ * NULL).  The arrays are copied and the files are written by the output
+ * set's writer thread; frames are written in call order.  Blocks while 64
+ * frames are queued. */
+int mc_out_frame(mc_out *o, int32_t frame, double omega, const float *cake,
+                 const float *lineouts, const double *meta);
+/* Fits of one frame's lineout: n_peaks x 7 PeakFit parameters
+ * (<prefix>_fit.bin, IntegratorZarrOMP.c:1879-1883). */
+int mc_out_lineout_fits(mc_out *o, int32_t frame, int32_t n_peaks, const double *params);
+/* Per-eta fits: rows with center == 0 are skipped, as the reference does. */
+int mc_out_peaks(mc_out *o, int32_t n_rows, const mc_peak_row *rows);
+/* Drains the queue, writes the closing datasets and closes every file.
+ * Returns the first error of the output set; the handle is freed. */
+int mc_out_close(mc_out *o);
 
 /* ---- continuous-batching frame server ---- */
 typedef struct mc_server mc_server;

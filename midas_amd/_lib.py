@@ -73,6 +73,27 @@ class ServerStats(C.Structure):
                 ("peak_bytes", C.c_int64), ("rejected", C.c_int64)]
 
 
+class OutConfig(C.Structure):
+    """mc_out_config: an output set's shape, files and axes."""
+    _fields_ = [("n_r", C.c_int32), ("n_eta", C.c_int32), ("files", C.c_uint32),
+                ("omega_sum_frames", C.c_int32), ("px", C.c_double), ("lsd", C.c_double),
+                ("wavelength", C.c_double), ("q_mode", C.c_int32), ("q_min", C.c_double),
+                ("q_bin_size", C.c_double), ("reta", C.POINTER(C.c_double)),
+                ("r_centers", C.POINTER(C.c_double)), ("eta_lo", C.POINTER(C.c_double)),
+                ("eta_hi", C.POINTER(C.c_double)), ("instrument", C.POINTER(C.c_double)),
+                ("source", C.c_char_p)]
+
+
+class PeakRow(C.Structure):
+    """mc_peak_row: one per-eta fit (7 PeakFit parameters)."""
+    _fields_ = [("frame", C.c_int32), ("eta_bin", C.c_int32), ("peak", C.c_int32),
+                ("reserved", C.c_int32), ("p", C.c_double * 7)]
+
+
+MC_OUT_CAKED_HDF, MC_OUT_PER_FRAME, MC_OUT_SUM_FRAMES, MC_OUT_OMEGAS = 0x01, 0x02, 0x04, 0x08
+MC_OUT_LINEOUT, MC_OUT_STREAM, MC_OUT_INT2D, MC_OUT_PEAKS = 0x10, 0x20, 0x40, 0x80
+
+
 RESULT_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.c_int32,
                         C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32)
 
@@ -120,6 +141,11 @@ SIGNATURES = {
     "mc_server_drain": (_I, [_V, _I32]),
     "mc_server_stats_get": (_I, [_V, _P(ServerStats)]),
     "mc_server_destroy": (_I, [_V]),
+    "mc_out_open": (_I, [C.c_char_p, C.c_char_p, _P(OutConfig), _P(_V)]),
+    "mc_out_frame": (_I, [_V, _I32, _D, _PF, _PF, _PD]),
+    "mc_out_lineout_fits": (_I, [_V, _I32, _I32, _PD]),
+    "mc_out_peaks": (_I, [_V, _I32, _P(PeakRow)]),
+    "mc_out_close": (_I, [_V]),
 }
 
 _lib = None
