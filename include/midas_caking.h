@@ -205,6 +205,21 @@ int mc_integrate(mc_map *m, const void *frames, int32_t dtype,
                  uint32_t flags, float *cakes, int32_t cakes_on_device,
                  float *lineouts, int32_t lineouts_on_device);
 
+/* Cakes with propagated variance (integrate_with_variance,
+ * packages/midas_integrate/midas_integrate/kernels.py:476-561): per bin,
+ * besides the value mc_integrate returns for (mode, flags), the variance
+ * sum(w^2 * Var(pixel)) over the same merged (bin, pixel) weights, scaled by
+ * the square of the value's own scale (1/sum A when MC_F_NORMALIZE), NaN
+ * where the value is NaN.  variance = [n_frames][NZ][NY] float32 variance
+ * images, or NULL for Poisson statistics: Var = the frame minus the dark
+ * (the dark's sum w^2 * dark is subtracted).  cakes and variances are
+ * [n_frames][n_r][n_eta] float32, host or device (out_on_device).  The
+ * hybrid dtype is not accepted here. */
+int mc_integrate_variance(mc_map *m, const void *frames, int32_t dtype, int32_t n_frames,
+                          int32_t frames_on_device, const float *variance,
+                          int32_t variance_on_device, int32_t mode, uint32_t flags,
+                          float *cakes, float *variances, int32_t out_on_device);
+
 /* 1-D lineouts of cakes already computed ([n_frames][n_r][n_eta] float32)
  * into [n_frames][3][n_r] (rows MC_LINEOUT_*); the profile kernels of
  * IntegratorFitPeaksGPUStream.cu:1162-1334 and the CPU lineout of

@@ -369,6 +369,60 @@ def integrate(image, geom: CSRGeometry, *, mode: str = "bilinear", normalize: bo
     return cakes[0]
 
 
+def integrate_with_variance(image, geom: CSRGeometry, *, mode: str = "bilinear",
+                            normalize: bool = True, variance_image=None,
+                            mask_nan: bool = False):
+    """Cake and propagated variance of one or more frames
+    (integrate_with_variance, kernels.py:476-561).  ``variance_image``
+    (same shape as ``image``) gives the per-pixel variance; None assumes
+    Poisson statistics (the image minus the dark is its own variance).
+    Per bin the variance is sum(w^2 * var) over the merged (bin, pixel)
+    weights, divided by (sum A)^2 when normalising.  Returns
+    (intensity, variance), each (n_r, n_eta) for one frame or
+    (F, n_r, n_eta)."""
+    if mode not in _lib.MODES:
+        raise ValueError(f"unknown integration mode {mode!r}; expected floor|bilinear|gradient")
+    npx = geom.n_pixels_y * geom.n_pixels_z
+    code = _dtype_code(image)
+    fl = _flags(normalize, mask_nan)
+    single = (image.ndim == 2)
+    if _is_cuda(image):
+        t = _torch()
+        fr = image.contiguous()
+        F = fr.numel() // npx
+        if fr.numel() != F * npx:
+            raise ValueError(f"image numel {fr.numel()} is not a multiple of {npx}")
+        v = None
+        if variance_image is not None:
+            v = variance_image.to(device=fr.device, dtype=t.float32).contiguous()
+            if v.numel() != fr.numel():
+                raise ValueError("variance_image must match image")
+        cakes = t.empty((F, geom.n_r, geom.n_eta), dtype=t.float32, device=fr.device)
+        var = t.empty_like(cakes)
+        t.cuda.current_stream(fr.device).synchronize()
+        check(lib().mc_integrate_variance(
+            geom.handle, C.c_void_p(fr.data_ptr()), code, F, 1,
+            C.c_void_p(v.data_ptr()) if v is not None else None, 1, _lib.MODES[mode], fl,
+            C.c_void_p(cakes.data_ptr()), C.c_void_p(var.data_ptr()), 1))
+    else:
+        a = np.ascontiguousarray(image)
+        F = a.size // npx
+        if a.size != F * npx:
+            raise ValueError(f"image size {a.size} is not a multiple of {npx}")
+        v = None
+        if variance_image is not None:
+            v = np.ascontiguousarray(variance_image, dtype=np.float32)
+            if v.size != a.size:
+                raise ValueError("variance_image must match image")
+        cakes = np.empty((F, geom.n_r, geom.n_eta), dtype=np.float32)
+        var = np.empty_like(cakes)
+        check(lib().mc_integrate_variance(geom.handle, a.ctypes.data, code, F, 0,
+                                          v.ctypes.data if v is not None else None, 0,
+                                          _lib.MODES[mode], fl, cakes.ctypes.data,
+                                          var.ctypes.data, 0))
+    return (cakes[0], var[0]) if single else (cakes, var)
+
+
 _PROFILE_ROWS = {"cpu_mean": 0, "area_weighted": 1, "simple_mean": 2}
 
 

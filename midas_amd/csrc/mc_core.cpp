@@ -208,6 +208,7 @@ int ensure_csr(Map *m, int mode) {
   c.bc_z = m->bc_z;
   c.built = true;
   c.dterm_valid = false;
+  c.d2_valid = false;
   m->tiles[mode].clear();
   for (int f = 0; f < 4; f++) m->binfo_ok[mode * 4 + f] = false;
   return MC_OK;
@@ -889,7 +890,7 @@ int mc_map_set_dark(mc_map *m, const void *dark, int32_t dtype, int32_t n_frames
   std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
   std::lock_guard<std::mutex> lk2(m->mu);
   MC_HIP(hipSetDevice(m->ctx->device));
-  for (auto &c : m->csr) c.dterm_valid = false;
+  for (auto &c : m->csr) c.dterm_valid = c.d2_valid = false;
   for (bool &ok : m->binfo_ok) ok = false;
   if (!dark) {
     m->has_dark = false;
@@ -1007,6 +1008,64 @@ int mc_integrate(mc_map *m, const void *frames, int32_t dtype, int32_t n_frames,
   if (!cakes_on_device) MC_HIP(hipMemcpyAsync(cakes, cd, cbytes, hipMemcpyDeviceToHost, s));
   if (lineouts && !lineouts_on_device)
     MC_HIP(hipMemcpyAsync(lineouts, ld, lbytes, hipMemcpyDeviceToHost, s));
+  MC_HIP(hipStreamSynchronize(s));
+  return MC_OK;
+}
+
+int mc_integrate_variance(mc_map *m, const void *frames, int32_t dtype, int32_t n_frames,
+                          int32_t frames_on_device, const float *variance,
+                          int32_t variance_on_device, int32_t mode, uint32_t flags,
+                          float *cakes, float *variances, int32_t out_on_device) {
+  MC_CHECK(m && frames && cakes && variances && n_frames >= 1, MC_ERR_ARG, "bad arguments");
+  MC_CHECK(mode >= 0 && mode <= 2, MC_ERR_ARG, "mode %d", mode);
+  const size_t es = dtype_size(dtype);
+  MC_CHECK(es > 0 && dtype != MC_DTYPE_HYBRID, MC_ERR_ARG, "unsupported frame dtype %d", dtype);
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  std::lock_guard<std::mutex> lk2(m->mu);
+  MC_HIP(hipSetDevice(m->ctx->device));
+  hipStream_t s = m->ctx->stream;
+  const float2 *binfo = nullptr;
+  int rc = ensure_binfo(m, mode, flags, &binfo);
+  if (rc) return rc;
+  Csr &c = m->csr[mode];
+  const float *d2 = nullptr;
+  if (!variance && m->has_dark) {
+    if (!c.d2_valid) {
+      if (!c.d2term.p && (rc = c.d2term.alloc(m->nb() * 4))) return rc;
+      if ((rc = launch_dark_term2(s, c.row.as<int32_t>(), c.pix.as<int32_t>(), c.w.as<float>(),
+                                  m->d_dark.as<double>(), m->nb(), c.d2term.as<float>())))
+        return rc;
+      c.d2_valid = true;
+    }
+    d2 = c.d2term.as<float>();
+  }
+  const size_t fbytes = (size_t)n_frames * m->npx() * es;
+  const size_t vbytes = (size_t)n_frames * m->npx() * 4;
+  const size_t cbytes = (size_t)n_frames * m->nb() * 4;
+  DBuf df, dv, dc, dvar;
+  const void *fd = frames;
+  if (!frames_on_device) {
+    if ((rc = df.alloc(fbytes))) return rc;
+    MC_HIP(hipMemcpyAsync(df.p, frames, fbytes, hipMemcpyHostToDevice, s));
+    fd = df.p;
+  }
+  const float *vd = variance;
+  if (variance && !variance_on_device) {
+    if ((rc = dv.alloc(vbytes))) return rc;
+    MC_HIP(hipMemcpyAsync(dv.p, variance, vbytes, hipMemcpyHostToDevice, s));
+    vd = dv.as<float>();
+  }
+  float *cd = cakes, *vard = variances;
+  if (!out_on_device) {
+    if ((rc = dc.alloc(cbytes)) || (rc = dvar.alloc(cbytes))) return rc;
+    cd = dc.as<float>();
+    vard = dvar.as<float>();
+  }
+  if ((rc = launch_plain_var(s, *m, c, binfo, d2, fd, dtype, vd, cd, vard, n_frames))) return rc;
+  if (!out_on_device) {
+    MC_HIP(hipMemcpyAsync(cakes, cd, cbytes, hipMemcpyDeviceToHost, s));
+    MC_HIP(hipMemcpyAsync(variances, vard, cbytes, hipMemcpyDeviceToHost, s));
+  }
   MC_HIP(hipStreamSynchronize(s));
   return MC_OK;
 }

@@ -90,6 +90,8 @@ struct Csr {
   std::vector<float> h_w;
   DBuf dterm;                         // float [nb], dark term of this CSR
   bool dterm_valid = false;
+  DBuf d2term;                        // float [nb], sum w^2 * dark (variance)
+  bool d2_valid = false;
 };
 
 // Row-block/tile layout of the batched kernel for one (mode, batch, rows,
@@ -163,6 +165,13 @@ int launch_dark_term(hipStream_t s, const int32_t *row, const int32_t *pix,
 int launch_binfo(hipStream_t s, const float *inv_area, const int32_t *flags,
                  const float *dterm, int64_t nb, int normalize, int mask_nan,
                  float2 *binfo);
+int launch_dark_term2(hipStream_t s, const int32_t *row, const int32_t *pix, const float *w,
+                      const double *dark, int64_t nb, float *d2term);
+// One-frame kernel with variance propagation; vimg NULL = Poisson (the
+// frame itself, minus the dark's d2term when that is given).
+int launch_plain_var(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo,
+                     const float *d2term, const void *frames, int dtype, const float *vimg,
+                     float *cake, float *var, int n_frames);
 int launch_plain(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo,
                  int mask_nan, const void *frame, int dtype, float *cake,
                  float *lineouts, int n_frames, int64_t frame_stride);

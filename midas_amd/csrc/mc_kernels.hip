@@ -146,6 +146,49 @@ __global__ __launch_bounds__(256) void k_plain(
 }
 
 // ---------------------------------------------------------------------
+// One-frame kernel with variance propagation (integrate_with_variance,
+// kernels.py:476-561): beside each cake value, the sum of w^2 * var over
+// the same merged (bin, pixel) weights, scaled by the square of the value's
+// own scale.  var is the frame itself (Poisson: Var(I) = I, with the dark's
+// sum w^2 * dark subtracted as the dark is from the value) or a variance
+// image.  grid = (n_r, n_frames).
+// ---------------------------------------------------------------------
+template <typename T, bool POISSON>
+__global__ __launch_bounds__(256) void k_plain_var(
+    const int32_t *__restrict__ row, const int32_t *__restrict__ pix,
+    const float *__restrict__ w, const float2 *__restrict__ binfo,
+    const float *__restrict__ d2term, const T *__restrict__ frames, const float *__restrict__ vimg,
+    int64_t frame_stride, int n_eta, int64_t nb, float *__restrict__ cake,
+    float *__restrict__ var) {
+  constexpr int LPB = 16, SLOTS = 256 / LPB;
+  const int r = blockIdx.x, f = blockIdx.y;
+  const T *__restrict__ img = frames + (int64_t)f * frame_stride;
+  const float *__restrict__ vi = POISSON ? nullptr : vimg + (int64_t)f * frame_stride;
+  const int slot = threadIdx.x / LPB, sub = threadIdx.x % LPB;
+  for (int e = slot; e < n_eta; e += SLOTS) {
+    const int64_t b = (int64_t)r * n_eta + e;
+    const int beg = row[b], end = row[b + 1];
+    Kahan acc, acc2;
+    for (int k = beg + sub; k < end; k += LPB) {
+      const int p = pix[k];
+      const float x = to_f(img[p]), ww = w[k];
+      acc.add(x * ww);
+      acc2.add((POISSON ? x : vi[p]) * ww * ww);
+    }
+    const float v = dpp_sum16(acc.value());
+    const float v2 = dpp_sum16(acc2.value());
+    if (sub == 0) {
+      const float2 bi = binfo[b];
+      const float sc = fabsf(bi.x);
+      const bool fl = flagged(bi.x);
+      cake[(int64_t)f * nb + b] = fl ? NAN : (v - bi.y) * sc;
+      const float raw = POISSON && d2term ? v2 - d2term[b] : v2;
+      var[(int64_t)f * nb + b] = fl ? NAN : raw * sc * sc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
 // Lineouts of existing cakes: workgroup = one R row of one frame.
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_profile(
@@ -680,6 +723,15 @@ __global__ void k_dark_term(const int32_t *__restrict__ row, const int32_t *__re
   dterm[b] = (float)s;
 }
 
+__global__ void k_dark_term2(const int32_t *__restrict__ row, const int32_t *__restrict__ pix,
+                             const float *__restrict__ w, const double *__restrict__ dark,
+                             int64_t nb, float *__restrict__ d2term) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double s = 0;
+  for (int k = row[b]; k < row[b + 1]; ++k) s += dark[pix[k]] * (double)w[k] * (double)w[k];
+  d2term[b] = (float)s;
+}
 __global__ void k_binfo(const float *__restrict__ inv_area, const int32_t *__restrict__ flags,
                         const float *__restrict__ dterm, int64_t nb, int normalize,
                         int mask_nan, float2 *__restrict__ binfo) {
@@ -766,6 +818,33 @@ int launch_plain(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo,
                        m.d_area_inb.as<float>(), m.d_row_static.as<float4>(),
                        (const T *)frames, frame_stride, m.n_eta, m.n_r, m.nb(), mask_nan,
                        cake, lineouts);
+    MC_HIP(hipGetLastError());
+    return MC_OK;
+  });
+}
+
+int launch_dark_term2(hipStream_t s, const int32_t *row, const int32_t *pix, const float *w,
+                      const double *dark, int64_t nb, float *d2term) {
+  hipLaunchKernelGGL(k_dark_term2, dim3(blocks_for(nb, 256)), dim3(256), 0, s, row, pix, w,
+                     dark, nb, d2term);
+  MC_HIP(hipGetLastError());
+  return MC_OK;
+}
+
+int launch_plain_var(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo,
+                     const float *d2term, const void *frames, int dtype, const float *vimg,
+                     float *cake, float *var, int n_frames) {
+  return dispatch_dtype(dtype, [&](auto tag) -> int {
+    using T = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
+    const dim3 grid(m.n_r, n_frames);
+    if (vimg)
+      hipLaunchKernelGGL((k_plain_var<T, false>), grid, dim3(256), 0, s, c.row.as<int32_t>(),
+                         c.pix.as<int32_t>(), c.w.as<float>(), binfo, d2term, (const T *)frames,
+                         vimg, m.npx(), m.n_eta, m.nb(), cake, var);
+    else
+      hipLaunchKernelGGL((k_plain_var<T, true>), grid, dim3(256), 0, s, c.row.as<int32_t>(),
+                         c.pix.as<int32_t>(), c.w.as<float>(), binfo, d2term, (const T *)frames,
+                         vimg, m.npx(), m.n_eta, m.nb(), cake, var);
     MC_HIP(hipGetLastError());
     return MC_OK;
   });

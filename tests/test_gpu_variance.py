@@ -1,0 +1,89 @@
+"""Variance propagation (mc_integrate_variance) vs the FP64 restatement of
+integrate_with_variance (kernels.py:476-561).
+
+For maps whose entries sit on whole pixels (SubPixelLevel 1) every map
+entry is one (bin, pixel) weight, so the oracle is direct:
+Var_b = s_b^2 * sum_k frac_k^2 * V[pixel_k], with s_b the value's scale
+(1 / sum areaWeight when normalising).  With merged weights (bilinear
+corners) the result must be linear in the variance image.  Tolerance
+1e-5 relative (positive terms, FP32 Kahan sums vs FP64)."""
+import numpy as np
+import pytest
+
+import midas_amd as ma
+from helpers import small_cases, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_var(om, V, NY, normalize=True):
+    e = om.entries
+    cnt, off = om.counts, om.offsets
+    nb = cnt.size
+    out = np.zeros(nb)
+    for b in range(nb):
+        s = slice(off[b], off[b] + cnt[b])
+        y = e["y"][s].astype(np.int64)
+        z = e["z"][s].astype(np.int64)
+        fr = e["frac"][s]
+        raw = np.sum(fr * fr * V[z, y])
+        if normalize:
+            a = np.sum(e["areaWeight"][s].astype(np.float64))
+            raw = raw / (a * a) if a > 0 else 0.0
+        out[b] = raw
+    return out.reshape(om.n_r, om.n_eta)
+
+
+def test_poisson_and_given_variance_match_oracle(oracle_lib):
+    P = small_cases()["flat"]
+    assert P.SubPixelLevel == 1
+    rng = np.random.default_rng(3)
+    frames = rng.poisson(200.0, (2, P.NrPixelsZ, P.NrPixelsY)).astype(np.uint16)
+    dark = rng.poisson(20.0, (2, P.NrPixelsZ, P.NrPixelsY)).astype(np.uint16)
+    print("[variance] flat, floor, Poisson with dark", flush=True)
+    g = ma.build_map(P)
+    g.set_dark(dark)
+    om = oracle_lib.build_map(to_oracle(P))
+    dmean = dark.astype(np.float64).mean(axis=0)
+    cakes, var = ma.integrate_with_variance(frames, g, mode="floor")
+    for i in range(len(frames)):
+        # the one-frame kernel's values, bit for bit (same summation order)
+        one, _ = ma.integrate_batch(frames[i], g, mode="floor", mask_nan=False, lineouts=False)
+        np.testing.assert_array_equal(cakes[i], one[0])
+    for i in range(len(frames)):
+        want = _oracle_var(om, frames[i].astype(np.float64) - dmean, P.NrPixelsY)
+        np.testing.assert_allclose(var[i], want, rtol=1e-5, atol=1e-6)
+    # an explicit variance image, unnormalised
+    V = rng.random((2, P.NrPixelsZ, P.NrPixelsY)).astype(np.float32) * 50
+    _, var2 = ma.integrate_with_variance(frames, g, mode="floor", variance_image=V,
+                                         normalize=False)
+    for i in range(2):
+        want = _oracle_var(om, V[i].astype(np.float64), P.NrPixelsY, normalize=False)
+        np.testing.assert_allclose(var2[i], want, rtol=1e-5, atol=1e-6)
+
+
+def test_linear_in_variance_image_with_merged_weights():
+    """Bilinear corners on a sub-pixel map: the weights are merged per
+    (bin, pixel); the propagated variance is linear in the variance image,
+    non-negative for a non-negative image, and NaN exactly where the value
+    is NaN."""
+    P = small_cases()["subpix"]
+    rng = np.random.default_rng(4)
+    mask = (rng.random((P.NrPixelsZ, P.NrPixelsY)) < 0.01).astype(np.float64)
+    frames = rng.poisson(100.0, (1, P.NrPixelsZ, P.NrPixelsY)).astype(np.uint16)
+    print("[variance] subpix, bilinear, linearity", flush=True)
+    g = ma.build_map(P, mask=mask)
+    V1 = rng.random(frames.shape).astype(np.float32) * 10
+    V2 = rng.random(frames.shape).astype(np.float32) * 10
+    c, a = ma.integrate_with_variance(frames, g, variance_image=V1, mask_nan=True)
+    _, b = ma.integrate_with_variance(frames, g, variance_image=V2, mask_nan=True)
+    _, ab = ma.integrate_with_variance(frames, g, variance_image=V1 + 2 * V2, mask_nan=True)
+    assert np.array_equal(np.isnan(a), np.isnan(c)) and np.isnan(c).any()
+    ok = ~np.isnan(a)
+    np.testing.assert_allclose(ab[ok], (a + 2 * b)[ok], rtol=2e-5, atol=1e-7)
+    assert (a[ok] >= 0).all()
+    # Poisson without a dark: the variance image is the frame itself
+    _, p = ma.integrate_with_variance(frames, g, mask_nan=True)
+    _, q = ma.integrate_with_variance(frames, g, variance_image=frames.astype(np.float32),
+                                      mask_nan=True)
+    np.testing.assert_allclose(p[ok], q[ok], rtol=1e-6)
