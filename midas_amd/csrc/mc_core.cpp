@@ -241,7 +241,7 @@ struct BBox {
 // LDS of the batched kernel: bin records, staged entries, the box of the
 // pixel-major batch (one row of 64 frames per pixel) and a zero pixel row.
 constexpr int kTileBins = 256, kTileEnts = 2048, kBinRecBytes = 32, kWaves = 16, kTileInts = 16,
-              kStageVecs = 4, kXtFrames = 64;
+              kXtFrames = 64;
 
 // Tiles of the batched kernel.  Bands of H cake rows; inside a band, tiles
 // are rectangles of rows x eta columns grown column by column while the
@@ -268,9 +268,9 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   const int64_t nb = m->nb();
   const int RB = kXtFrames * in_size;  // bytes per pixel row
   const int64_t fixed = (int64_t)kTileBins * kBinRecBytes + (int64_t)kTileEnts * 8;
-  // Box pixels: LDS (plus the zero row) and one register pass of staging.
-  const int64_t cap_px = std::min<int64_t>((m->lds_bytes - fixed) / RB - 1,
-                                           (int64_t)64 * kWaves * kStageVecs * 16 / RB);
+  // Box pixels: what the LDS budget holds after the records, the entries
+  // and the zero row.
+  const int64_t cap_px = (m->lds_bytes - fixed) / RB - 1;
   MC_CHECK(cap_px >= 16, MC_ERR_ARG, "LDS budget %d too small", m->lds_bytes);
   auto box_ok = [&](const BBox &bx) {
     return bx.empty() || (int64_t)(bx.y1 - bx.y0 + 1) * (bx.z1 - bx.z0 + 1) <= cap_px;

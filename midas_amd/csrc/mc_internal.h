@@ -147,7 +147,7 @@ struct Map {
   bool binfo_ok[12] = {};
   uint64_t binfo_version[12] = {};  // bumped whenever a table is rebuilt
   // tuning
-  int batch = 64, rows_per_group = 32, lds_bytes = 98304;
+  int batch = 64, rows_per_group = 16, lds_bytes = 81920;
   DBuf xt;  // pixel-major batch of the calls made through the map's context
   int64_t nb() const { return (int64_t)n_r * n_eta; }
   int64_t npx() const { return (int64_t)ny * nz; }

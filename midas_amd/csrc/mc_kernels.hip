@@ -234,8 +234,9 @@ __global__ __launch_bounds__(256) void k_profile(
 //   * the box of the batch: box row after box row, each pixel's 64 frames
 //     contiguous -- whole 128-byte lines of Xt, so staging reads only what
 //     it keeps.
-// The loads of tile t+1 are issued into registers before the waves reduce
-// tile t.  In the reduction the four 16-lane groups of a wave take the
+// Staging uses global->LDS copies (no registers), so a workgroup needs at
+// most 64 VGPRs per lane and two workgroups share a CU: one stages its
+// next tile while the other reduces.  In the reduction the four 16-lane groups of a wave take the
 // entries of a bin in turn (group g the entries 2g, 2g+1 of each group of
 // 8) and lane l of a group holds frames 4l..4l+3: one 8-byte LDS read
 // serves 4 frames of one entry.  The staged entries of a band row are one
@@ -246,8 +247,8 @@ __global__ __launch_bounds__(256) void k_profile(
 // total of frame 4l+g.
 // Wave w owns the band rows j = w (mod 16), so it keeps the lineout sums
 // of its G rows in registers across the chunk; k_lineout_finalize adds the
-// chunks of each band.  Cake values are stored per frame in whole 16-bin
-// granules.  Bins whose own box exceeds the budget ("direct" bins) gather
+// chunks of each band.  Cake values are stored per frame in whole 8-bin
+// granules (32-byte stores).  Bins whose own box exceeds the budget ("direct" bins) gather
 // from Xt in global memory.
 //   chunk: tile_begin tile_end band_row0 band_rows
 //   tile : y0 z0 bw bh b_beg b_end e_beg e_end ra nc vpr magic nrows 0 0 0
@@ -267,7 +268,6 @@ constexpr int kWaves = 16;
 constexpr int kTileBins = 256;    // bin records per tile (LDS)
 constexpr int kTileEnts = 2048;   // staged map entries per tile (LDS)
 constexpr int kTileInts = 16;
-constexpr int kStageVecs = 4;     // 16-byte staging vectors per thread and tile
 constexpr int kXtFrames = 64;     // frames per pixel row of the batch
 
 struct BinRec {
@@ -361,7 +361,7 @@ template <> struct Quad<float> {
 };
 
 template <int LG> struct GranVec;
-template <> struct GranVec<16> { typedef float type __attribute__((ext_vector_type(16))); };
+template <> struct GranVec<8> { typedef float type __attribute__((ext_vector_type(8))); };
 
 // Stores a row slot's granule for this lane's frame f (16 consecutive bins).
 template <int LG, typename VT>
@@ -411,58 +411,50 @@ __device__ __forceinline__ TileHdr tile_hdr(const int32_t *__restrict__ tiles, i
   return h;
 }
 
-// Registers of one tile's prefetch: this thread's share of the box
-// (kStageVecs 16-byte vectors), of the bin records (512 vectors of 16
-// bytes) and of the entries (1024 vectors; a tile's entries start at a
-// multiple of 8).
-struct TilePrefetch {
-  u4v v[kStageVecs];
-  u4v rv, ev;
-};
-static_assert(kTileBins * sizeof(BinRec) / 16 <= 64 * kWaves, "record vectors per thread");
-static_assert(kTileEnts * sizeof(uint2) / 16 == 64 * kWaves, "entry vectors per thread");
+typedef __attribute__((address_space(3))) void lds_void;
 
+// Stages one tile into LDS with gfx950 global->LDS copies (16 bytes per
+// lane; a wave fills 1 KB of consecutive LDS, so the copies need no
+// registers): the bin records, the entries (a tile's entries start at a
+// multiple of 8), then the box row after row, and the zero pixel row after
+// the box (plain stores; the padding entries point at it).  The caller
+// waits for the copies (s_waitcnt) and then for every wave (barrier).
 template <typename TL>
-__device__ __forceinline__ void tile_issue(TilePrefetch &P, const TileHdr &h, int tid,
+__device__ __forceinline__ void tile_stage(const TileHdr &h, int wave, int lane, char *lds,
                                            const BinRec *__restrict__ recs,
                                            const uint2 *__restrict__ ents,
                                            const TL *__restrict__ xt, int ny) {
-  if (tid < 2 * h.nbt) P.rv = reinterpret_cast<const u4v *>(recs + h.b_beg)[tid];
-  if (2 * tid < h.net) P.ev = reinterpret_cast<const u4v *>(ents + h.e_beg)[tid];
-  const int jobs = h.bh * h.vpr;
-#pragma unroll
-  for (int u = 0; u < kStageVecs; ++u) {
-    const int j = tid + 64 * kWaves * u;
+  constexpr int RB = kXtFrames * (int)sizeof(TL);  // bytes per pixel row
+  char *lent = lds + kTileBins * sizeof(BinRec);
+  char *plane = lent + kTileEnts * sizeof(uint2);
+  const int nrv = 2 * h.nbt;  // 16-byte record vectors
+  for (int ch = wave; ch * 64 < nrv; ch += kWaves) {
+    const int j = ch * 64 + lane;
+    if (j < nrv)
+      __builtin_amdgcn_global_load_lds(reinterpret_cast<const u4v *>(recs + h.b_beg) + j,
+                                       (lds_void *)(lds + ch * 1024), 16, 0, 0);
+  }
+  const int nev = h.net >> 1;  // 16-byte entry vectors
+  for (int ch = wave; ch * 64 < nev; ch += kWaves) {
+    const int j = ch * 64 + lane;
+    if (j < nev)
+      __builtin_amdgcn_global_load_lds(reinterpret_cast<const u4v *>(ents + h.e_beg) + j,
+                                       (lds_void *)(lent + ch * 1024), 16, 0, 0);
+  }
+  const int jobs = h.bh * h.vpr;  // 16-byte box vectors
+  for (int ch = wave; ch * 64 < jobs; ch += kWaves) {
+    const int j = ch * 64 + lane;
     if (j < jobs) {
       const int row = h.vpr == 1 ? j : (int)__umulhi((uint32_t)j, h.magic);
       const int v = j - row * h.vpr;
       const char *src = reinterpret_cast<const char *>(
                             xt + ((int64_t)(h.z0 + row) * ny + h.y0) * kXtFrames) +
                         v * 16;
-      P.v[u] = *reinterpret_cast<const u4v *>(src);
+      __builtin_amdgcn_global_load_lds(src, (lds_void *)(plane + ch * 1024), 16, 0, 0);
     }
   }
-}
-
-template <typename TL>
-__device__ __forceinline__ void tile_commit(const TilePrefetch &P, const TileHdr &h, int tid,
-                                            BinRec *lbin, uint2 *lent, char *plane) {
-  constexpr int RB = kXtFrames * (int)sizeof(TL);  // bytes per pixel row
-  if (tid < 2 * h.nbt) reinterpret_cast<u4v *>(lbin)[tid] = P.rv;
-  if (2 * tid < h.net) reinterpret_cast<u4v *>(lent)[tid] = P.ev;
-  // The zero pixel row after the box: the padding entries point at it.
-  if (tid < RB / 16)
-    reinterpret_cast<u4v *>(plane + (size_t)h.bw * h.bh * RB)[tid] = 0u;
-  const int jobs = h.bh * h.vpr;
-#pragma unroll
-  for (int u = 0; u < kStageVecs; ++u) {
-    const int j = tid + 64 * kWaves * u;
-    if (j < jobs) {
-      const int row = h.vpr == 1 ? j : (int)__umulhi((uint32_t)j, h.magic);
-      const int v = j - row * h.vpr;
-      *reinterpret_cast<u4v *>(plane + (size_t)row * h.bw * RB + v * 16) = P.v[u];
-    }
-  }
+  if (wave == kWaves - 1 && lane < RB / 16)
+    reinterpret_cast<u4v *>(plane + (size_t)h.bw * h.bh * RB)[lane] = 0u;
 }
 
 // Reduce-scatter over the four 16-lane groups of a wave: t[j] holds a
@@ -493,8 +485,10 @@ __device__ __forceinline__ void kahan_add(Kahan &k, float x) {
   k.s = t;
 }
 
+// Two workgroups share a CU (8 waves per SIMD, at most 64 VGPRs and half
+// the LDS each): while one waits for its tile's copies, the other reduces.
 template <typename TL, int G>
-__global__ __launch_bounds__(64 * kWaves) void k_batched(
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batched(
     const int32_t *__restrict__ chunks, const int32_t *__restrict__ xcd,
     const int32_t *__restrict__ tiles, const BinRec *__restrict__ recs,
     const uint2 *__restrict__ ents, const TL *__restrict__ xt,
@@ -504,7 +498,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_batched(
   constexpr int RB = kXtFrames * (int)sizeof(TL);
   constexpr int QB = 4 * (int)sizeof(TL);  // bytes of one lane's 4 frames
   constexpr int RBAND = kWaves * G;
-  constexpr int LG = 16;
+  constexpr int LG = 8;  // cake bins per granule: 32-byte stores
   typedef typename GranVec<LG>::type VT;
   BinRec *lbin = reinterpret_cast<BinRec *>(lds_raw);
   uint2 *lent = reinterpret_cast<uint2 *>(lds_raw + kTileBins * sizeof(BinRec));
@@ -534,18 +528,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_batched(
   const bool vec_ok = ((nb & 3) == 0) && (((uintptr_t)cakes & 15) == 0);
   const char *lpl = plane + l16 * QB;  // this lane's 4 frames in a pixel row
 
-  TilePrefetch P;
-  TileHdr hn = tile_hdr(tiles, t_beg);
-  tile_issue(P, hn, tid, recs, ents, xt, ny);
   for (int t = t_beg; t < t_end; ++t) {
-    const TileHdr h = hn;
-    __syncthreads();  // every wave is done with the previous tile's LDS
-    tile_commit<TL>(P, h, tid, lbin, lent, plane);
+    const TileHdr h = tile_hdr(tiles, t);
+    if (t > t_beg) __syncthreads();  // every wave is done with the previous tile's LDS
+    tile_stage<TL>(h, wave, lane, lds_raw, recs, ents, xt, ny);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
     __syncthreads();
-    if (t + 1 < t_end) {
-      hn = tile_hdr(tiles, t + 1);
-      tile_issue(P, hn, tid, recs, ents, xt, ny);
-    }
     const int nrows = h.nrows;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
