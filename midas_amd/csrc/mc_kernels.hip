@@ -339,6 +339,45 @@ __global__ __launch_bounds__(256) void k_transpose(const TIn *__restrict__ frame
   }
 }
 
+// uint16 frames, frame base 16-byte aligned, npx a multiple of 128: blocks
+// of 128 pixels x 64 frames with 16-byte loads and LDS writes.  LDS holds
+// [frame][16 chunks of 8 pixels]; chunk c of frame f sits at slot
+// c ^ (f / 8), so the 2-byte gathers of the store phase (8 frame groups x
+// 8 pixels per wave) fall on 32 different banks.
+__global__ __launch_bounds__(256) void k_transpose16(const uint16_t *__restrict__ frames,
+                                                     int64_t frame_stride, int n_frames,
+                                                     int64_t npx, uint16_t *__restrict__ xt) {
+  __shared__ __attribute__((aligned(16))) uint16_t tile[kXtFrames * 128];
+  const int64_t p0 = (int64_t)blockIdx.x * 128;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int j = t + 256 * u;
+    const int f = j >> 4, c = j & 15;  // 16 consecutive threads read 256 bytes of a frame
+    u4v v = 0u;
+    if (f < n_frames)
+      v = *reinterpret_cast<const u4v *>(frames + (int64_t)f * frame_stride + p0 + c * 8);
+    *reinterpret_cast<u4v *>(tile + f * 128 + (c ^ (f >> 3)) * 8) = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int k = t + 256 * u;
+    const int p = k >> 3, g = k & 7;  // pixel, group of 8 frames
+    if (p0 + p >= npx) continue;
+    const int ch = p >> 3, w = p & 7;
+    u4v out;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int fa = 8 * g + 2 * q, fb = fa + 1;  // fa / 8 = fb / 8 = g
+      const uint32_t a = tile[fa * 128 + (ch ^ g) * 8 + w];
+      const uint32_t b = tile[fb * 128 + (ch ^ g) * 8 + w];
+      out[q] = a | (b << 16);
+    }
+    *reinterpret_cast<u4v *>(xt + (p0 + p) * kXtFrames + 8 * g) = out;
+  }
+}
+
 // Four frames of one pixel row: uint16 x4 in 8 bytes or float x4 in 16.
 template <typename TL> struct Quad;
 template <> struct Quad<uint16_t> {
@@ -870,8 +909,12 @@ static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mas
   // 1. pixel-major batch
   const int64_t npx = m.npx();
   const int vec_ok = (npx % 64 == 0) && (((uintptr_t)frames & 15) == 0) ? 1 : 0;
-  hipLaunchKernelGGL((k_transpose<TIn, TL>), dim3((unsigned)((npx + 63) / 64)), dim3(256), 0, s,
-                     (const TIn *)frames, npx, n_frames, npx, vec_ok, (TL *)xt);
+  if (sizeof(TIn) == 2 && sizeof(TL) == 2 && vec_ok && npx % 128 == 0)
+    hipLaunchKernelGGL(k_transpose16, dim3((unsigned)(npx / 128)), dim3(256), 0, s,
+                       (const uint16_t *)frames, npx, n_frames, npx, (uint16_t *)xt);
+  else
+    hipLaunchKernelGGL((k_transpose<TIn, TL>), dim3((unsigned)((npx + 63) / 64)), dim3(256), 0,
+                       s, (const TIn *)frames, npx, n_frames, npx, vec_ok, (TL *)xt);
   MC_HIP(hipGetLastError());
   // 2. tiles
   auto kern = k_batched<TL, G>;

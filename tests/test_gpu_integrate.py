@@ -182,3 +182,23 @@ def test_torch_device_tensors(oracle_lib):
     hc, hl = ma.integrate_batch(frames, g, mode="bilinear")
     np.testing.assert_array_equal(dc.cpu().numpy(), hc)
     np.testing.assert_array_equal(dl.cpu().numpy(), hl)
+
+
+@pytest.mark.parametrize("ny,nz", [(96, 80), (96, 66), (97, 66)])
+def test_transpose_paths(ny, nz):
+    """The pixel-major batch through each transpose path: 128-pixel blocks
+    (pixel count a multiple of 128), 64-pixel blocks with 16-byte loads (a
+    multiple of 64 only) and the element-wise path; the batched results
+    match the one-frame kernel's, which reads the frames directly."""
+    P = IntegrationParams(NrPixelsY=ny, NrPixelsZ=nz, pxY=200.0, pxZ=200.0, Lsd=1e6,
+                          BC_y=ny / 2 + 0.3, BC_z=nz / 2 - 0.2, RMin=2.0, RMax=40.0,
+                          RBinSize=0.5, EtaBinSize=10.0)
+    rng = np.random.default_rng(12)
+    frames = rng.poisson(300.0, (5, nz, ny)).astype(np.uint16)
+    print(f"[integrate] transpose path {ny}x{nz}", flush=True)
+    g = ma.build_map(P)
+    cb, lb = ma.integrate_batch(frames, g, mode="bilinear", mask_nan=True)
+    for i in range(len(frames)):
+        c1, l1 = ma.integrate_batch(frames[i], g, mode="bilinear", mask_nan=True)
+        np.testing.assert_allclose(cb[i], c1[0], rtol=3e-6, atol=1e-4, equal_nan=True)
+        np.testing.assert_allclose(lb[i], l1[0], rtol=3e-6, atol=1e-4)
