@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <iterator>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -217,40 +218,21 @@ int ensure_csr(Map *m, int mode) {
 // ------------------------------------------------------------------
 // Tile planner of the batched kernel.
 // ------------------------------------------------------------------
-namespace {
-struct BBox {
-  int y0 = INT32_MAX, z0 = INT32_MAX, y1 = -1, z1 = -1;  // inclusive
-  bool empty() const { return y1 < 0; }
-  void add(int y, int z) {
-    y0 = std::min(y0, y);
-    y1 = std::max(y1, y);
-    z0 = std::min(z0, z);
-    z1 = std::max(z1, z);
-  }
-  void add(const BBox &o) {
-    if (o.empty()) return;
-    y0 = std::min(y0, o.y0);
-    y1 = std::max(y1, o.y1);
-    z0 = std::min(z0, o.z0);
-    z1 = std::max(z1, o.z1);
-  }
-  int64_t area() const { return empty() ? 0 : (int64_t)(y1 - y0 + 1) * (z1 - z0 + 1); }
-};
-}  // namespace
-
-// LDS of the batched kernel: bin records, staged entries, the box of the
-// pixel-major batch (one row of 64 frames per pixel) and a zero pixel row.
+// LDS of the batched kernel: bin records, staged entries, the staged pixel
+// rows of the pixel-major batch (64 frames per pixel), a zero pixel row, and
+// two pixel-list buffers (the current tile's and the next one's).
 constexpr int kTileBins = 256, kTileEnts = 2048, kBinRecBytes = 32, kWaves = 16, kTileInts = 16,
               kXtFrames = 64;
 
 // Tiles of the batched kernel.  Bands of H cake rows; inside a band, tiles
 // are rectangles of rows x eta columns grown column by column while the
-// bounding box of their entries' pixels fits the budget (LDS and the
-// staging registers); a column that does not fit whole is split into row
-// ranges, and a bin that does not fit alone is "direct" (gathered from
-// global memory).  Chunks are runs of consecutive tiles of one band of
-// about `target` entries: the work of one workgroup.  in_size is the
-// element size of the pixel-major batch (2 or 4 bytes).
+// footprint of their entries (the distinct pixels) fits the LDS budget; a
+// tile stages exactly its footprint, one pixel row of the batch per pixel.
+// A column that does not fit whole is split into row ranges, and a bin
+// that does not fit alone is "direct" (gathered from global memory).
+// Chunks are runs of consecutive tiles of one band of about `target`
+// entries: the work of one workgroup.  in_size is the element size of the
+// pixel-major batch (2 or 4 bytes).
 int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   int rc = ensure_csr(m, mode);
   if (rc) return rc;
@@ -268,31 +250,51 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   const int64_t nb = m->nb();
   const int RB = kXtFrames * in_size;  // bytes per pixel row
   const int64_t fixed = (int64_t)kTileBins * kBinRecBytes + (int64_t)kTileEnts * 8;
-  // Box pixels: what the LDS budget holds after the records, the entries
-  // and the zero row.
-  const int64_t cap_px = (m->lds_bytes - fixed) / RB - 1;
+  // Pixels a tile can stage: what the LDS budget holds after the records,
+  // the entries, the zero row and the two pixel lists (4 bytes a pixel,
+  // rounded up to 16 bytes).
+  const int64_t cap_px = (m->lds_bytes - fixed - RB - 24) / (RB + 8);
   MC_CHECK(cap_px >= 16, MC_ERR_ARG, "LDS budget %d too small", m->lds_bytes);
-  auto box_ok = [&](const BBox &bx) {
-    return bx.empty() || (int64_t)(bx.y1 - bx.y0 + 1) * (bx.z1 - bx.z0 + 1) <= cap_px;
-  };
-  // Per-bin pixel boxes; a bin that does not fit alone is direct.
-  std::vector<BBox> bb(nb);
+  // Per-bin footprints: the sorted distinct pixels of its entries; a bin
+  // whose footprint or entries do not fit a tile alone is direct.
+  std::vector<std::vector<int32_t>> fp(nb);
   std::vector<char> big(nb, 0);
   parallel_for(nb, [&](int64_t a, int64_t b) {
     for (int64_t q = a; q < b; q++) {
-      for (int32_t k = c.h_row[q]; k < c.h_row[q + 1]; k++)
-        bb[q].add(c.h_pix[k] % NY, c.h_pix[k] / NY);
-      big[q] = !box_ok(bb[q]) || ((c.h_row[q + 1] - c.h_row[q] + 7) & ~7) > kTileEnts;
+      auto &v = fp[q];
+      v.assign(c.h_pix.begin() + c.h_row[q], c.h_pix.begin() + c.h_row[q + 1]);
+      std::sort(v.begin(), v.end());
+      v.erase(std::unique(v.begin(), v.end()), v.end());
+      big[q] = (int64_t)v.size() > cap_px ||
+               ((c.h_row[q + 1] - c.h_row[q] + 7) & ~7) > kTileEnts;
+      if (big[q]) std::vector<int32_t>().swap(v);
     }
   });
   auto cnt = [&](int64_t b) { return (int64_t)(c.h_row[b + 1] - c.h_row[b]); };
   // Staged entries of a bin are padded to whole groups of 8 (zero-weight
   // entries on the zero pixel row).
   auto cntp = [&](int64_t b) { return (cnt(b) + 7) & ~(int64_t)7; };
+  // Size of the union of two sorted pixel sets.
+  auto union_size = [](const std::vector<int32_t> &x, const std::vector<int32_t> &y) {
+    size_t i = 0, j = 0, n = 0;
+    while (i < x.size() && j < y.size()) {
+      if (x[i] < y[j]) i++;
+      else if (y[j] < x[i]) j++;
+      else { i++; j++; }
+      n++;
+    }
+    return (int64_t)(n + (x.size() - i) + (y.size() - j));
+  };
+  auto merge_into = [](std::vector<int32_t> &x, const std::vector<int32_t> &y) {
+    std::vector<int32_t> u;
+    u.reserve(x.size() + y.size());
+    std::set_union(x.begin(), x.end(), y.begin(), y.end(), std::back_inserter(u));
+    x.swap(u);
+  };
   struct TileH {
     int ra = 0, nrows = 0, e0 = 0, nc = 0;
-    BBox box;
-    int64_t ents = 0;  // staged entries
+    std::vector<int32_t> px;  // footprint, sorted
+    int64_t ents = 0;         // staged entries
   };
   const int n_bands = (nR + H - 1) / H;
   std::vector<std::vector<TileH>> bands(n_bands);
@@ -300,28 +302,28 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
     for (int64_t gi = a; gi < bq; gi++) {
       auto &tv = bands[gi];
       const int row0 = (int)gi * H, nrows = std::min(H, nR - row0);
-      auto room = [&](const TileH &T, const BBox &add, int64_t ents, int nbins) {
-        BBox u = T.box;
-        u.add(add);
+      auto room = [&](const TileH &T, const std::vector<int32_t> &add, int64_t ents,
+                      int nbins) {
         return (int64_t)T.nrows * T.nc + nbins <= kTileBins && T.ents + ents <= kTileEnts &&
-               box_ok(u);
+               union_size(T.px, add) <= cap_px;
       };
       TileH cur;
       auto flush = [&]() {
-        if (cur.nc > 0) tv.push_back(cur);
+        if (cur.nc > 0) tv.push_back(std::move(cur));
         cur = TileH();
       };
+      std::vector<int32_t> col;
       for (int e = 0; e < nE; e++) {
-        BBox col;
+        col.clear();
         int64_t ce = 0;
         for (int j = 0; j < nrows; j++) {
           const int64_t b = (int64_t)(row0 + j) * nE + e;
           if (big[b]) continue;
-          col.add(bb[b]);
+          merge_into(col, fp[b]);
           ce += cntp(b);
         }
         if (cur.nc > 0 && cur.nrows == nrows && room(cur, col, ce, nrows)) {
-          cur.box.add(col);
+          merge_into(cur.px, col);
           cur.ents += ce;
           cur.nc++;
           continue;
@@ -330,10 +332,10 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
         TileH fresh;
         fresh.nrows = nrows;
         if (room(fresh, col, ce, nrows)) {
-          cur = fresh;
+          cur = std::move(fresh);
           cur.e0 = e;
           cur.nc = 1;
-          cur.box = col;
+          cur.px = col;
           cur.ents = ce;
           continue;
         }
@@ -341,22 +343,23 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
         TileH piece;
         piece.e0 = e;
         piece.nc = 1;
+        static const std::vector<int32_t> none;
         for (int j = 0; j < nrows; j++) {
           const int64_t b = (int64_t)(row0 + j) * nE + e;
-          const BBox add = big[b] ? BBox() : bb[b];
+          const std::vector<int32_t> &add = big[b] ? none : fp[b];
           const int64_t ae = big[b] ? 0 : cntp(b);
           if (piece.nrows > 0 && !room(piece, add, ae, 1)) {
-            tv.push_back(piece);
+            tv.push_back(std::move(piece));
             piece = TileH();
             piece.e0 = e;
             piece.nc = 1;
             piece.ra = j;
           }
-          piece.box.add(add);
+          merge_into(piece.px, add);
           piece.ents += ae;
           piece.nrows++;
         }
-        tv.push_back(piece);
+        tv.push_back(std::move(piece));
       }
       flush();
     }
@@ -375,9 +378,10 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   std::vector<int32_t> chunks, band_chunk(n_bands + 1, 0), tiles;
   std::vector<int4> tbins;
   std::vector<uint2> ents;
+  std::vector<int32_t> tpix;  // staged pixels of every tile, tile after tile
   tbins.reserve(nb);
   ents.reserve(c.nnz + c.nnz / 4);
-  int64_t n_direct = 0, max_lds = 0, staged = 0, real_ents = 0;
+  int64_t n_direct = 0, max_npx = 0, staged = 0, real_ents = 0;
   for (int gi = 0; gi < n_bands; gi++) {
     const int row0 = gi * H, nrows_band = std::min(H, nR - row0);
     int64_t acc = target;  // forces a new chunk at the band start
@@ -388,22 +392,22 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
         chunk_cost.push_back(0);
         acc = 0;
       }
-      int bx0 = 0, bz0 = 0, bw = 0, bh = 0;
-      if (!T.box.empty()) {
-        bx0 = T.box.y0;
-        bz0 = T.box.z0;
-        bw = T.box.y1 - T.box.y0 + 1;
-        bh = T.box.z1 - T.box.z0 + 1;
-      }
-      max_lds = std::max<int64_t>(max_lds, ((int64_t)bw * bh + 1) * RB);
-      staged += (int64_t)bw * bh;
+      const int32_t npx = (int32_t)T.px.size();
+      const int32_t px_beg = (int32_t)tpix.size();
+      tpix.insert(tpix.end(), T.px.begin(), T.px.end());
+      while (tpix.size() % 4) tpix.push_back(0);  // lists start at 16-byte boundaries
+      max_npx = std::max<int64_t>(max_npx, npx);
+      staged += npx;
       const int32_t b_beg = (int32_t)tbins.size(), e_beg = (int32_t)ents.size();
-      const uint32_t zero_row = (uint32_t)((int64_t)bw * bh * RB);
+      const uint32_t zero_row = (uint32_t)((int64_t)npx * RB);
       auto push_entries = [&](int64_t b, bool dir) {
         for (int32_t k = c.h_row[b]; k < c.h_row[b + 1]; k++) {
           const int p = c.h_pix[k];
-          const uint32_t off =
-              dir ? (uint32_t)p : (uint32_t)(((p / NY - bz0) * bw + (p % NY - bx0)) * RB);
+          uint32_t off = (uint32_t)p;
+          if (!dir) {  // byte offset of the pixel's row in the tile's staged list
+            const int64_t slot = std::lower_bound(T.px.begin(), T.px.end(), p) - T.px.begin();
+            off = (uint32_t)(slot * RB);
+          }
           uint32_t wb;
           memcpy(&wb, &c.h_w[k], 4);
           ents.push_back(make_uint2(off, wb));
@@ -437,11 +441,8 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
       // Every tile's entries start at a multiple of 8 (16-byte staging).
       while (ents.size() % 8) ents.push_back(make_uint2(0u, 0u));
       tbins.insert(tbins.end(), recs.begin(), recs.end());
-      const int vpr = bw * RB / 16;  // staging vectors per box row
-      // floor(j / vpr) = umulhi(j, magic) for j < 2^32 / vpr (j < 4096 here)
-      const uint32_t magic = vpr > 1 ? (uint32_t)(((uint64_t)1 << 32) / (uint64_t)vpr + 1) : 0u;
-      tiles.insert(tiles.end(), {bx0, bz0, bw, bh, b_beg, (int32_t)tbins.size(), e_beg, e_end,
-                                 T.ra, T.nc, vpr, (int32_t)magic, T.nrows, 0, 0, 0});
+      tiles.insert(tiles.end(), {px_beg, npx, 0, 0, b_beg, (int32_t)tbins.size(), e_beg, e_end,
+                                 T.ra, T.nc, 0, 0, T.nrows, 0, 0, 0});
       chunks[chunks.size() - 3] = (int32_t)(tiles.size() / kTileInts);
       const int64_t tcost = all_ents + kTileCost + kBinCost * T.nrows * T.nc;
       acc += tcost;
@@ -477,7 +478,11 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   t->in_size = in_size;
   t->n_chunks = (int)(chunks.size() / 4);
   t->n_bands = n_bands;
-  t->lds_needed = ((size_t)fixed + (size_t)max_lds + 15) & ~(size_t)15;
+  t->tp_off = (int)(fixed + (max_npx + 1) * RB);
+  t->tp_len = (int)(((max_npx + 3) & ~(int64_t)3) * 4);
+  t->lds_needed = (size_t)t->tp_off + 2 * (size_t)t->tp_len;
+  MC_CHECK((int64_t)t->lds_needed <= m->lds_bytes, MC_ERR_STATE, "tile LDS %zu over budget %d",
+           t->lds_needed, m->lds_bytes);
   t->n_tiles = (int64_t)tiles.size() / kTileInts;
   t->n_tbins = (int64_t)tbins.size();
   t->n_ents = (int64_t)ents.size();
@@ -486,8 +491,11 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   if ((rc = t->chunks.alloc(chunks.size() * 4)) || (rc = t->tiles.alloc(tiles.size() * 4)) ||
       (rc = t->tbins.alloc(tbins.size() * 16)) || (rc = t->ents.alloc(ents.size() * 8)) ||
       (rc = t->band_chunk.alloc(band_chunk.size() * 4)) || (rc = t->xcd.alloc(9 * 4)) ||
-      (rc = t->part.alloc((size_t)t->n_chunks * H * 4 * kXtFrames * 4)))
+      (rc = t->part.alloc((size_t)t->n_chunks * H * 4 * kXtFrames * 4)) ||
+      (rc = t->tpix.alloc(std::max<size_t>(tpix.size(), 1) * 4)))
     return rc;
+  if (!tpix.empty())
+    MC_HIP(hipMemcpy(t->tpix.p, tpix.data(), tpix.size() * 4, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->chunks.p, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->tiles.p, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->tbins.p, tbins.data(), tbins.size() * 16, hipMemcpyHostToDevice));

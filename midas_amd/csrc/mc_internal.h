@@ -102,14 +102,16 @@ struct Tiles {
   int in_size = 0;   // bytes per element of the pixel-major batch (2 or 4)
   int n_chunks = 0, n_bands = 0;
   size_t lds_needed = 0;
+  int tp_off = 0, tp_len = 0;  // LDS bytes: the two pixel-list buffers (offset, length of one)
   DBuf chunks;       // int32 x4 per chunk: tile_begin tile_end band_row0 band_rows
   DBuf tiles;        // int32 x16 per tile (see mc_kernels.hip)
   DBuf tbins;        // int32 x4 per tile bin: cake bin, ent_begin, ent_end, direct flag
   DBuf recs;         // 32-byte records: tbins + scale, dark term, area of one binfo table
   const void *recs_binfo = nullptr;
   uint64_t recs_version = 0;
-  DBuf ents;         // uint32 x2: byte offset in the tile box (or global pixel), weight bits
+  DBuf ents;         // uint32 x2: byte offset of the staged pixel row (or global pixel), weight bits
   DBuf band_chunk;   // int32 [n_bands+1]: chunk range of each band
+  DBuf tpix;         // int32: staged pixels of every tile (tile t at px_beg .. + npx)
   DBuf xcd;          // int32 [9]: chunk range of each XCD (equal cost)
   int grid = 0;      // workgroups of the launch: 8 x the largest XCD range
   DBuf part;         // float [n_chunks][rows][4][64]: lineout partials

@@ -426,46 +426,66 @@ __device__ __forceinline__ void flush_granule(const VT &v, int idx, uint32_t mas
 }
 
 struct TileHdr {
-  int y0, z0, bw, bh, b_beg, nbt, e_beg, net, ra, nc, vpr;
-  uint32_t magic;
-  int nrows;
+  int px_beg, npx, b_beg, nbt, e_beg, net, ra, nc, nrows;
 };
 
 __device__ __forceinline__ TileHdr tile_hdr(const int32_t *__restrict__ tiles, int t) {
   const int32_t *T = tiles + kTileInts * t;
   TileHdr h;
-  h.y0 = T[0];
-  h.z0 = T[1];
-  h.bw = T[2];
-  h.bh = T[3];
+  h.px_beg = T[0];
+  h.npx = T[1];
   h.b_beg = T[4];
   h.nbt = T[5] - T[4];
   h.e_beg = T[6];
   h.net = T[7] - T[6];
   h.ra = T[8];
   h.nc = T[9];
-  h.vpr = T[10];
-  h.magic = (uint32_t)T[11];
   h.nrows = T[12];
   return h;
 }
 
 typedef __attribute__((address_space(3))) void lds_void;
 
+// Copies a tile's pixel list (npx int32 at a multiple of 4) to LDS with
+// global->LDS copies (16 bytes per lane).
+__device__ __forceinline__ void stage_pixels(const int32_t *__restrict__ tpix, int px_beg,
+                                             int npx, int wave, int lane, char *dst) {
+  const int nv = (npx + 3) >> 2;
+  for (int ch = kWaves - 1 - wave; ch * 64 < nv; ch += kWaves) {
+    const int j = ch * 64 + lane;
+    if (j < nv)
+      __builtin_amdgcn_global_load_lds(reinterpret_cast<const u4v *>(tpix + px_beg) + j,
+                                       (lds_void *)(dst + ch * 1024), 16, 0, 0);
+  }
+}
+
 // Stages one tile into LDS with gfx950 global->LDS copies (16 bytes per
 // lane; a wave fills 1 KB of consecutive LDS, so the copies need no
-// registers): the bin records, the entries (a tile's entries start at a
-// multiple of 8), then the box row after row, and the zero pixel row after
-// the box (plain stores; the padding entries point at it).  The caller
-// waits for the copies (s_waitcnt) and then for every wave (barrier).
+// registers): the rows of the tile's pixels (its footprint, listed in LDS
+// at ltp), the bin records, the entries (a tile's entries start at a
+// multiple of 8), and the zero pixel row after the staged rows (plain
+// stores; the padding entries point at it).  The caller waits for the
+// copies (s_waitcnt) and then for every wave (barrier).
 template <typename TL>
 __device__ __forceinline__ void tile_stage(const TileHdr &h, int wave, int lane, char *lds,
                                            const BinRec *__restrict__ recs,
                                            const uint2 *__restrict__ ents,
-                                           const TL *__restrict__ xt, int ny) {
+                                           const TL *__restrict__ xt,
+                                           const int32_t *__restrict__ ltp) {
   constexpr int RB = kXtFrames * (int)sizeof(TL);  // bytes per pixel row
+  constexpr int VPP = RB / 16;                     // 16-byte vectors per pixel row
   char *lent = lds + kTileBins * sizeof(BinRec);
   char *plane = lent + kTileEnts * sizeof(uint2);
+  const int jobs = h.npx * VPP;
+  for (int ch = wave; ch * 64 < jobs; ch += kWaves) {
+    const int j = ch * 64 + lane;
+    if (j < jobs) {
+      const int p = ltp[j / VPP];
+      const char *src = reinterpret_cast<const char *>(xt + (int64_t)p * kXtFrames) +
+                        (j % VPP) * 16;
+      __builtin_amdgcn_global_load_lds(src, (lds_void *)(plane + ch * 1024), 16, 0, 0);
+    }
+  }
   const int nrv = 2 * h.nbt;  // 16-byte record vectors
   for (int ch = wave; ch * 64 < nrv; ch += kWaves) {
     const int j = ch * 64 + lane;
@@ -480,20 +500,8 @@ __device__ __forceinline__ void tile_stage(const TileHdr &h, int wave, int lane,
       __builtin_amdgcn_global_load_lds(reinterpret_cast<const u4v *>(ents + h.e_beg) + j,
                                        (lds_void *)(lent + ch * 1024), 16, 0, 0);
   }
-  const int jobs = h.bh * h.vpr;  // 16-byte box vectors
-  for (int ch = wave; ch * 64 < jobs; ch += kWaves) {
-    const int j = ch * 64 + lane;
-    if (j < jobs) {
-      const int row = h.vpr == 1 ? j : (int)__umulhi((uint32_t)j, h.magic);
-      const int v = j - row * h.vpr;
-      const char *src = reinterpret_cast<const char *>(
-                            xt + ((int64_t)(h.z0 + row) * ny + h.y0) * kXtFrames) +
-                        v * 16;
-      __builtin_amdgcn_global_load_lds(src, (lds_void *)(plane + ch * 1024), 16, 0, 0);
-    }
-  }
-  if (wave == kWaves - 1 && lane < RB / 16)
-    reinterpret_cast<u4v *>(plane + (size_t)h.bw * h.bh * RB)[lane] = 0u;
+  if (wave == kWaves - 1 && lane < VPP)
+    reinterpret_cast<u4v *>(plane + (size_t)h.npx * RB)[lane] = 0u;
 }
 
 // Reduce-scatter over the four 16-lane groups of a wave: t[j] holds a
@@ -530,8 +538,9 @@ template <typename TL, int G>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batched(
     const int32_t *__restrict__ chunks, const int32_t *__restrict__ xcd,
     const int32_t *__restrict__ tiles, const BinRec *__restrict__ recs,
-    const uint2 *__restrict__ ents, const TL *__restrict__ xt,
-    int n_frames, int ny, int64_t nb, float *__restrict__ cakes, float *__restrict__ part) {
+    const uint2 *__restrict__ ents, const int32_t *__restrict__ tpix, const TL *__restrict__ xt,
+    int n_frames, int tp_off, int tp_len, int64_t nb, float *__restrict__ cakes,
+    float *__restrict__ part) {
   extern __shared__ __attribute__((aligned(16))) char lds_raw[];
   typedef typename Quad<TL>::type QT;
   constexpr int RB = kXtFrames * (int)sizeof(TL);
@@ -567,10 +576,24 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
   const bool vec_ok = ((nb & 3) == 0) && (((uintptr_t)cakes & 15) == 0);
   const char *lpl = plane + l16 * QB;  // this lane's 4 frames in a pixel row
 
+  // Pixel lists of tiles t (buffer (t - t_beg) & 1) and t + 1 (the other):
+  // a tile's list is in LDS before its staging starts.
+  {
+    const TileHdr h0 = tile_hdr(tiles, t_beg);
+    stage_pixels(tpix, h0.px_beg, h0.npx, wave, lane, lds_raw + tp_off);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
   for (int t = t_beg; t < t_end; ++t) {
     const TileHdr h = tile_hdr(tiles, t);
+    const int cur = (t - t_beg) & 1;
     if (t > t_beg) __syncthreads();  // every wave is done with the previous tile's LDS
-    tile_stage<TL>(h, wave, lane, lds_raw, recs, ents, xt, ny);
+    tile_stage<TL>(h, wave, lane, lds_raw, recs, ents, xt,
+                   reinterpret_cast<const int32_t *>(lds_raw + tp_off + cur * tp_len));
+    if (t + 1 < t_end) {
+      const int32_t *Tn = tiles + kTileInts * (t + 1);
+      stage_pixels(tpix, Tn[0], Tn[1], wave, lane, lds_raw + tp_off + (cur ^ 1) * tp_len);
+    }
     __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
     __syncthreads();
     const int nrows = h.nrows;
@@ -925,8 +948,8 @@ static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mas
   float *part = lineouts ? t.part.as<float>() : nullptr;
   hipLaunchKernelGGL(kern, dim3((unsigned)t.grid), dim3(64 * kWaves), lds, s,
                      t.chunks.as<int32_t>(), t.xcd.as<int32_t>(), t.tiles.as<int32_t>(),
-                     t.recs.as<BinRec>(), t.ents.as<uint2>(), (const TL *)xt, n_frames, m.ny,
-                     m.nb(), cakes, part);
+                     t.recs.as<BinRec>(), t.ents.as<uint2>(), t.tpix.as<int32_t>(),
+                     (const TL *)xt, n_frames, t.tp_off, t.tp_len, m.nb(), cakes, part);
   MC_HIP(hipGetLastError());
   // 3. lineouts
   if (lineouts) {
