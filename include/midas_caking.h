@@ -41,6 +41,8 @@
  *                       and the GPU stream's lineout.bin,
  *                       lineout_simple_mean.bin, Int2D.bin and its writer
  *                       thread (IntegratorFitPeaksGPUStream.cu:506-565)
+ *   mc_smooth_eta       gauss_smooth_eta, packages/midas_integrate/
+ *   mc_median_eta       midas_integrate/postprocess.py:36-142
  */
 #ifndef MIDAS_CAKING_H
 #define MIDAS_CAKING_H
@@ -228,6 +230,33 @@ int mc_integrate_variance(mc_map *m, const void *frames, int32_t dtype, int32_t 
 int mc_profile(mc_map *m, const float *cakes, int32_t n_frames,
                int32_t cakes_on_device, float *lineouts,
                int32_t lineouts_on_device);
+
+/* ---- post-process filters on cakes along eta ---- */
+
+/* Gaussian smoothing along eta of n_cakes cakes [n_cakes][n_r][n_eta]
+ * (replaces gauss_smooth_eta, midas_integrate/postprocess.py:36-103).
+ * Weights w_k = exp(-k^2 / (2 sigma^2)) normalised to sum 1 over
+ * k = -h..h, h = max(1, round-half-even(4 sigma)).  area ([n_r][n_eta],
+ * optional) makes the smoothing area-weighted: out = sum w I A / sum w A,
+ * which keeps each ring's sum of I*A when the axis is periodic; with
+ * area NULL every bin has area 1.  periodic 1 wraps the eta axis, 0
+ * reflects it at the ends (numpy "reflect").  FP64 arithmetic in the
+ * reference's order.  cakes, area and out are device pointers when
+ * on_device is 1, host pointers otherwise; out must not alias cakes.
+ * Errors: sigma <= 0, h > 1024, or h > n_eta when periodic. */
+int mc_smooth_eta(mc_ctx *ctx, const float *cakes, int32_t n_cakes, int32_t n_r,
+                  int32_t n_eta, double sigma_bins, const float *area,
+                  int32_t periodic, int32_t on_device, float *out);
+
+/* Median filter along eta over an odd window (replaces median_filter_eta,
+ * postprocess.py:106-142): the window's median, NaN where the window holds
+ * a NaN; with preserve_ring_flux each row is then scaled by
+ * sum(cake row) / sum(filtered row) (float32 sums in numpy's pairwise
+ * order; factor 1 where the filtered sum is 0).  window 1..63.  Pointers
+ * as in mc_smooth_eta. */
+int mc_median_eta(mc_ctx *ctx, const float *cakes, int32_t n_cakes, int32_t n_r,
+                  int32_t n_eta, int32_t window, int32_t periodic,
+                  int32_t preserve_ring_flux, int32_t on_device, float *out);
 
 /* Launch timing of the kernels on device-resident frames, with HIP events
  * on the context's stream.  kind 0 = one-frame kernel (n_frames launches),

@@ -16,6 +16,7 @@ from .caking import (AREA_THRESHOLD, CSRGeometry, Context, build_csr, build_map,
                      default_context, integrate, integrate_batch, integrate_with_variance,
                      load_map_gpu, profile_1d, time_integrate)
 from .outputs import ResultFiles
+from .postprocess import gauss_smooth_eta, median_filter_eta
 
 __all__ = [
     "__version__", "MAP_HEADER_MAGIC", "MAP_HEADER_SIZE", "PXLIST_DTYPE", "MapHeader",
@@ -23,5 +24,6 @@ __all__ = [
     "IntegrationParams", "bin_edges", "parse_integrator_params", "parse_mapper_params",
     "parse_params", "AREA_THRESHOLD", "CSRGeometry", "Context", "build_csr", "build_map",
     "default_context", "integrate", "integrate_batch", "integrate_with_variance",
-    "load_map_gpu", "profile_1d", "time_integrate", "ResultFiles",
+    "load_map_gpu", "profile_1d", "time_integrate", "ResultFiles", "gauss_smooth_eta",
+    "median_filter_eta",
 ]

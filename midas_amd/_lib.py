@@ -136,6 +136,8 @@ SIGNATURES = {
     "mc_time_integrate": (_I, [_V, _V, _I32, _I32, _I32, _U32, _I32, _I32, _I32, _V, _V,
                                _PD, _PD]),
     "mc_fit_windows": (_I, [_V, _I32, _I32, _I32, _V, _V, _V, _D, _I32, _V, _I32]),
+    "mc_smooth_eta": (_I, [_V, _V, _I32, _I32, _I32, _D, _V, _I32, _I32, _V]),
+    "mc_median_eta": (_I, [_V, _V, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _V]),
     "mc_fit_cake_rings": (_I, [_V, _V, _I32, _PD, _PD, _I32, _I32, _I32, _I32, _V, _I32]),
     "mc_server_create": (_I, [_V, _I64, _I32, _I32, RESULT_CB, _V, _P(_V)]),
     "mc_server_add_geometry": (_I, [_V, _V, _I32, _I32, _U32, _P(_I32)]),
