@@ -41,6 +41,8 @@
  *                       and the GPU stream's lineout.bin,
  *                       lineout_simple_mean.bin, Int2D.bin and its writer
  *                       thread (IntegratorFitPeaksGPUStream.cu:506-565)
+ *   mc_map_fuse         build_fused_geometry, packages/midas_integrate/
+ *                       midas_integrate/fused_csr.py:126-260
  *   mc_smooth_eta       gauss_smooth_eta, packages/midas_integrate/
  *   mc_median_eta       midas_integrate/postprocess.py:36-142
  */
@@ -166,6 +168,23 @@ int mc_map_info_get(mc_map *m, mc_map_info *info);
 /* Copies the map back: records (nnz*24 B), pairs (2*n_bins), flags. */
 int mc_map_download(mc_map *m, void *records, int32_t *pairs,
                     int32_t *flags);
+/* Multi-map averaging fused into one map (replaces build_fused_geometry,
+ * packages/midas_integrate/midas_integrate/fused_csr.py:126-260): maps[k]
+ * are built for the same detector and eta bins with R edges shifted by
+ * k * dR / n_maps (their R bin counts may differ by the shift); r_axes
+ * holds each map's R bin centres, map after map (n_r of map 0, then of
+ * map 1, ...).  The fused map has map 0's bins.
+ * The result holds M = (1/N) sum_k P_k D_k M_k, with M_k the map's
+ * bilinear or gradient matrix (mode), D_k = diag(1/A) over its in-bounds
+ * bin areas A > 1e-9 (0 elsewhere) and P_k the linear interpolation from
+ * its R axis onto map 0's, clamped at the ends.  FP64 products and sums in
+ * the reference's order, stored as FP32 weights.  The fused map integrates
+ * in bilinear or gradient mode (both use M) with unit areas, so normalise
+ * is a no-op, as in the reference; it has no Map.bin records (mc_map_write
+ * and mc_map_download refuse it).  Every map needs n_r >= 2. */
+int mc_map_fuse(mc_map *const *maps, int32_t n_maps, const double *r_axes,
+                int32_t mode, mc_map **out);
+
 /* Static per-bin axes (REtaMap, IntegratorZarrOMP.c:1754-1763): 5 rows
  * R, 2theta, eta, sum areaWeight, Q; needs the bin edges. */
 int mc_map_reta(mc_map *m, const double *r_lo, const double *r_hi,

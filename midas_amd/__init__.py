@@ -17,6 +17,7 @@ from .caking import (AREA_THRESHOLD, CSRGeometry, Context, build_csr, build_map,
                      load_map_gpu, profile_1d, time_integrate)
 from .outputs import ResultFiles
 from .postprocess import gauss_smooth_eta, median_filter_eta
+from .fused_csr import build_fused_geometry
 
 __all__ = [
     "__version__", "MAP_HEADER_MAGIC", "MAP_HEADER_SIZE", "PXLIST_DTYPE", "MapHeader",
@@ -25,5 +26,5 @@ __all__ = [
     "parse_params", "AREA_THRESHOLD", "CSRGeometry", "Context", "build_csr", "build_map",
     "default_context", "integrate", "integrate_batch", "integrate_with_variance",
     "load_map_gpu", "profile_1d", "time_integrate", "ResultFiles", "gauss_smooth_eta",
-    "median_filter_eta",
+    "median_filter_eta", "build_fused_geometry",
 ]

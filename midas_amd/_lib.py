@@ -136,6 +136,7 @@ SIGNATURES = {
     "mc_time_integrate": (_I, [_V, _V, _I32, _I32, _I32, _U32, _I32, _I32, _I32, _V, _V,
                                _PD, _PD]),
     "mc_fit_windows": (_I, [_V, _I32, _I32, _I32, _V, _V, _V, _D, _I32, _V, _I32]),
+    "mc_map_fuse": (_I, [_V, _I32, _V, _I32, _P(_V)]),
     "mc_smooth_eta": (_I, [_V, _V, _I32, _I32, _I32, _D, _V, _I32, _I32, _V]),
     "mc_median_eta": (_I, [_V, _V, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _V]),
     "mc_fit_cake_rings": (_I, [_V, _V, _I32, _PD, _PD, _I32, _I32, _I32, _I32, _V, _I32]),

@@ -160,6 +160,7 @@ class CSRGeometry:
         self.n_r, self.n_eta = n_r, n_eta
         self.n_pixels_y, self.n_pixels_z = ny, nz
         self.params = params
+        self.fused = False  # set by build_fused_geometry
         self._lock = threading.Lock()
 
     # -- info -----------------------------------------------------------------

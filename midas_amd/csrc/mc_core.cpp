@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <iterator>
+#include <memory>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -156,6 +157,10 @@ template <class F> void parallel_for(int64_t n, F &&f) {
 int ensure_csr(Map *m, int mode) {
   MC_CHECK(mode >= 0 && mode <= 2, MC_ERR_ARG, "mode %d", mode);
   Csr &c = m->csr[mode];
+  if (m->fused) {
+    MC_CHECK(c.built, MC_ERR_ARG, "a fused map integrates in bilinear or gradient mode only");
+    return MC_OK;
+  }
   if (c.built && (mode != MC_MODE_GRADIENT || (c.bc_y == m->bc_y && c.bc_z == m->bc_z)))
     return MC_OK;
   const int64_t nb = m->nb();
@@ -808,6 +813,7 @@ int mc_map_load(mc_ctx *ctx, const char *map_path, const char *nmap_path,
 
 int mc_map_write(mc_map *m, const char *dir, const uint8_t *header64) {
   MC_CHECK(m && dir, MC_ERR_ARG, "null argument");
+  MC_CHECK(!m->fused, MC_ERR_ARG, "a fused map has no Map.bin records to write");
   std::string d(dir);
   auto wr = [&](const std::string &name, const void *p, size_t n) -> int {
     std::string path = d + "/" + name;
@@ -848,7 +854,7 @@ int mc_map_info_get(mc_map *m, mc_map_info *info) {
     hb += c.row.bytes + c.pix.bytes + c.w.bytes + c.dterm.bytes;
     for (auto &t : m->tiles[q])
       hb += t->chunks.bytes + t->tiles.bytes + t->tbins.bytes + t->ents.bytes +
-            t->band_chunk.bytes + t->part.bytes + t->recs.bytes;
+            t->band_chunk.bytes + t->part.bytes + t->recs.bytes + t->tpix.bytes;
   }
   info->hbm_bytes = hb;
   for (int q = 0; q < 3; q++) info->skipped[q] = m->skipped[q];
@@ -857,6 +863,7 @@ int mc_map_info_get(mc_map *m, mc_map_info *info) {
 
 int mc_map_download(mc_map *m, void *records, int32_t *pairs, int32_t *flags) {
   MC_CHECK(m, MC_ERR_ARG, "null map");
+  MC_CHECK(!m->fused, MC_ERR_ARG, "a fused map has no Map.bin records");
   if (records) memcpy(records, m->h_rec.data(), m->nnz * sizeof(Rec));
   if (pairs) memcpy(pairs, m->h_pairs.data(), m->h_pairs.size() * 4);
   if (flags) memcpy(flags, m->h_flags.data(), m->h_flags.size() * 4);
@@ -879,6 +886,190 @@ int mc_map_reta(mc_map *m, const double *r_lo, const double *r_hi, const double 
       out5[4 * nb + b] = lambda > 0 ? (4.0 * M_PI / lambda) * sin(tth / 2.0) : 0.0;
     }
   }
+  return MC_OK;
+}
+
+// Linear interpolation of the source R axis at r (fused_csr.py:64-99):
+// (j, w_lo, w_hi) with r = w_lo * src[j] + w_hi * src[j + 1], clamped to the
+// first / last pair.
+static void r_interp(const double *src, int n, double r, int *j, double *wl, double *wh) {
+  if (r <= src[0]) {
+    *j = 0;
+    *wl = 1.0;
+    *wh = 0.0;
+  } else if (r >= src[n - 1]) {
+    *j = n - 2;
+    *wl = 0.0;
+    *wh = 1.0;
+  } else {
+    int k = (int)(std::upper_bound(src, src + n, r) - src) - 1;
+    k = std::max(0, std::min(k, n - 2));
+    const double den = src[k + 1] - src[k];
+    const double t = den > 0 ? (r - src[k]) / den : 0.0;
+    *j = k;
+    *wl = 1.0 - t;
+    *wh = t;
+  }
+}
+
+int mc_map_fuse(mc_map *const *maps, int32_t n_maps, const double *r_axes, int32_t mode,
+                mc_map **out) {
+  MC_CHECK(maps && r_axes && out && n_maps >= 1, MC_ERR_ARG, "bad arguments");
+  MC_CHECK(mode == MC_MODE_BILINEAR || mode == MC_MODE_GRADIENT, MC_ERR_ARG,
+           "mode must be bilinear or gradient, got %d", mode);
+  Map *m0 = maps[0];
+  MC_CHECK(m0, MC_ERR_ARG, "null map");
+  for (int k = 0; k < n_maps; k++) {
+    const Map *mk = maps[k];
+    MC_CHECK(mk && mk->ctx == m0->ctx && mk->ny == m0->ny && mk->nz == m0->nz &&
+                 mk->n_eta == m0->n_eta && !mk->fused,
+             MC_ERR_ARG, "map %d does not match map 0 (context, detector, eta bins) or is fused",
+             k);
+    MC_CHECK(mk->n_r >= 2, MC_ERR_ARG, "R interpolation needs at least 2 R bins (map %d)", k);
+  }
+  const int nR = m0->n_r, nE = m0->n_eta;
+  std::vector<size_t> roff(n_maps + 1, 0);  // each map's R axis in r_axes
+  for (int k = 0; k < n_maps; k++) roff[k + 1] = roff[k] + maps[k]->n_r;
+  Ctx *ctx = m0->ctx;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  MC_HIP(hipSetDevice(ctx->device));
+  int rc;
+  std::vector<std::vector<double>> inv_a(n_maps);
+  for (int k = 0; k < n_maps; k++) {
+    std::lock_guard<std::mutex> lk2(maps[k]->mu);
+    if ((rc = ensure_csr(maps[k], mode))) return rc;
+    const auto &A = maps[k]->h_area_inb;
+    inv_a[k].resize(A.size());
+    for (size_t b = 0; b < A.size(); b++)
+      inv_a[k][b] = A[b] > 1e-9 ? 1.0 / std::max(A[b], 1e-9) : 0.0;
+  }
+  // Per output R row: the source rows and weights of every map.
+  std::vector<int> jj((size_t)n_maps * nR);
+  std::vector<double> wlo((size_t)n_maps * nR), whi((size_t)n_maps * nR);
+  for (int k = 0; k < n_maps; k++)
+    for (int i = 0; i < nR; i++)
+      r_interp(r_axes + roff[k], maps[k]->n_r, r_axes[i], &jj[(size_t)k * nR + i],
+               &wlo[(size_t)k * nR + i], &whi[(size_t)k * nR + i]);
+  // Row (i, e) of M = (1/N) sum_k (P_k D_k) M_k, in the order the
+  // reference's sparse products take: per map, a pixel's contribution is
+  // (w_lo / A_j) * w(j) + (w_hi / A_j+1) * w(j+1); maps add in order; the
+  // sum is divided by N; exact zeros drop.
+  const int64_t nb = (int64_t)nR * nE;
+  std::vector<int32_t> cnt(nb, 0);
+  std::vector<std::vector<int32_t>> rp(nb);
+  std::vector<std::vector<float>> rw(nb);
+  parallel_for(nb, [&](int64_t a, int64_t bq) {
+    struct T {
+      int32_t pix, k, side;
+      double v;
+    };
+    std::vector<T> v;
+    for (int64_t ob = a; ob < bq; ob++) {
+      const int i = (int)(ob / nE), e = (int)(ob % nE);
+      v.clear();
+      for (int k = 0; k < n_maps; k++) {
+        const Csr &c = maps[k]->csr[mode];
+        const int j = jj[(size_t)k * nR + i];
+        for (int side = 0; side < 2; side++) {
+          const int64_t sb = (int64_t)(j + side) * nE + e;
+          const double pw = side ? whi[(size_t)k * nR + i] : wlo[(size_t)k * nR + i];
+          const double ia = inv_a[k][sb];
+          for (int32_t q = c.h_row[sb]; q < c.h_row[sb + 1]; q++)
+            v.push_back({c.h_pix[q], k, side, (pw * ia) * (double)c.h_w[q]});
+        }
+      }
+      std::sort(v.begin(), v.end(), [](const T &x, const T &y) {
+        return x.pix != y.pix ? x.pix < y.pix : (x.k != y.k ? x.k < y.k : x.side < y.side);
+      });
+      auto &P = rp[ob];
+      auto &W = rw[ob];
+      size_t q = 0;
+      while (q < v.size()) {
+        const int32_t pix = v[q].pix;
+        double tot = 0.0;
+        bool first = true;
+        while (q < v.size() && v[q].pix == pix) {
+          const int k = v[q].k;
+          double ck = 0.0;
+          while (q < v.size() && v[q].pix == pix && v[q].k == k) ck += v[q++].v;
+          tot = first ? ck : tot + ck;
+          first = false;
+        }
+        const double val = tot / n_maps;
+        if (val != 0.0) {
+          P.push_back(pix);
+          W.push_back((float)val);
+        }
+      }
+      cnt[ob] = (int32_t)P.size();
+    }
+  });
+  auto *m = new mc_map();
+  std::unique_ptr<mc_map> guard(m);
+  m->ctx = ctx;
+  m->fused = true;
+  m->ny = m0->ny;
+  m->nz = m0->nz;
+  m->n_r = nR;
+  m->n_eta = nE;
+  m->nnz = 0;
+  m->q_mode = m0->q_mode;
+  m->bc_y = m0->bc_y;
+  m->bc_z = m0->bc_z;
+  m->h_flags.assign(nb, 0);
+  m->h_area_all.assign(nb, 1.0);
+  m->h_area_inb.assign(nb, 1.0);
+  Csr &c = m->csr[MC_MODE_BILINEAR];
+  c.h_row.assign(nb + 1, 0);
+  for (int64_t b = 0; b < nb; b++) c.h_row[b + 1] = c.h_row[b] + cnt[b];
+  const int64_t nnz = c.h_row[nb];
+  c.h_pix.resize(nnz);
+  c.h_w.resize(nnz);
+  for (int64_t b = 0; b < nb; b++) {
+    std::copy(rp[b].begin(), rp[b].end(), c.h_pix.begin() + c.h_row[b]);
+    std::copy(rw[b].begin(), rw[b].end(), c.h_w.begin() + c.h_row[b]);
+  }
+  c.nnz = nnz;
+  c.bc_y = m->bc_y;
+  c.bc_z = m->bc_z;
+  if ((rc = c.row.alloc((nb + 1) * 4)) || (rc = c.pix.alloc(nnz * 4)) ||
+      (rc = c.w.alloc(nnz * 4)) || (rc = c.dterm.alloc(nb * 4)))
+    return rc;
+  MC_HIP(hipMemcpy(c.row.p, c.h_row.data(), (nb + 1) * 4, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(c.pix.p, c.h_pix.data(), nnz * 4, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(c.w.p, c.h_w.data(), nnz * 4, hipMemcpyHostToDevice));
+  c.built = true;
+  // The gradient slot holds the same matrix (the reference exposes the
+  // fused matrix under both names).
+  Csr &g = m->csr[MC_MODE_GRADIENT];
+  g.h_row = c.h_row;
+  g.h_pix = c.h_pix;
+  g.h_w = c.h_w;
+  g.nnz = nnz;
+  g.bc_y = c.bc_y;
+  g.bc_z = c.bc_z;
+  if ((rc = g.row.alloc((nb + 1) * 4)) || (rc = g.pix.alloc(nnz * 4)) ||
+      (rc = g.w.alloc(nnz * 4)) || (rc = g.dterm.alloc(nb * 4)))
+    return rc;
+  MC_HIP(hipMemcpy(g.row.p, c.row.p, (nb + 1) * 4, hipMemcpyDeviceToDevice));
+  MC_HIP(hipMemcpy(g.pix.p, c.pix.p, nnz * 4, hipMemcpyDeviceToDevice));
+  MC_HIP(hipMemcpy(g.w.p, c.w.p, nnz * 4, hipMemcpyDeviceToDevice));
+  g.built = true;
+  // Unit areas: scale 1, every bin valid for the lineouts.
+  std::vector<float> ones(nb, 1.f);
+  std::vector<float4> rs(nR, make_float4((float)nE, (float)nE, (float)nE, (float)nE));
+  std::vector<int32_t> pairs0(2 * nb, 0);
+  if ((rc = m->d_rec.alloc(16)) || (rc = m->d_pairs.alloc(2 * nb * 4)) ||
+      (rc = m->d_flags.alloc(nb * 4)) || (rc = m->d_area_inb.alloc(nb * 4)) ||
+      (rc = m->d_row_static.alloc(nR * sizeof(float4))) || (rc = m->inv_area.alloc(nb * 4)))
+    return rc;
+  MC_HIP(hipMemcpy(m->d_pairs.p, pairs0.data(), 2 * nb * 4, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(m->d_flags.p, m->h_flags.data(), nb * 4, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(m->d_area_inb.p, ones.data(), nb * 4, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(m->inv_area.p, ones.data(), nb * 4, hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(m->d_row_static.p, rs.data(), nR * sizeof(float4), hipMemcpyHostToDevice));
+  m->h_pairs = std::move(pairs0);
+  *out = guard.release();
   return MC_OK;
 }
 

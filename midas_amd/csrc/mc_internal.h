@@ -121,6 +121,9 @@ struct Tiles {
 struct Map {
   Ctx *ctx = nullptr;
   std::mutex mu;
+  // A fused map (mc_map_fuse) holds one combined matrix as its bilinear and
+  // gradient CSR, unit areas and no Map.bin records.
+  bool fused = false;
   int ny = 0, nz = 0, n_r = 0, n_eta = 0;
   int64_t nnz = 0;
   int q_mode = 0;
