@@ -41,6 +41,9 @@
  *                       and the GPU stream's lineout.bin,
  *                       lineout_simple_mean.bin, Int2D.bin and its writer
  *                       thread (IntegratorFitPeaksGPUStream.cu:506-565)
+ *   mc_blosc_*          blosc1_decompress of the Zarr-zip chunks,
+ *                       IntegratorZarrOMP.c:1416-1454, 1646-1669 and
+ *                       ZarrReader.c:7-62
  *   mc_map_fuse         build_fused_geometry, packages/midas_integrate/
  *                       midas_integrate/fused_csr.py:126-260
  *   mc_smooth_eta       gauss_smooth_eta, packages/midas_integrate/
@@ -249,6 +252,25 @@ int mc_integrate_variance(mc_map *m, const void *frames, int32_t dtype, int32_t 
 int mc_profile(mc_map *m, const float *cakes, int32_t n_frames,
                int32_t cakes_on_device, float *lineouts,
                int32_t lineouts_on_device);
+
+/* ---- Zarr-zip input: Blosc1 chunks ---- */
+
+/* Decompresses one Blosc1 chunk (the chunk format of the reference's
+ * Zarr zips, read there with blosc1_decompress: IntegratorZarrOMP.c:1440,
+ * 1658; ZarrReader.c:7-62).  Handles stored chunks, split and unsplit
+ * blocks, byte shuffle and bitshuffle (format versions 2 and 3), and the
+ * zstd, lz4/lz4hc and zlib codecs through the system's libzstd.so.1,
+ * liblz4.so.1 and libz.so.1 (loaded on first use); blosclz and snappy
+ * chunks return MC_ERR_FORMAT.  *out_size = decompressed bytes; an error
+ * if they exceed dst_capacity.  Host memory only; no GPU is used. */
+int mc_blosc_decompress(const void *src, int64_t src_size, void *dst,
+                        int64_t dst_capacity, int64_t *out_size);
+
+/* n chunks at once on `threads` host threads (0 = all cores); out_sizes[i]
+ * is -1 for a chunk that failed, and the error names the first one. */
+int mc_blosc_decompress_many(const void *const *srcs, const int64_t *src_sizes,
+                             void *const *dsts, int64_t dst_capacity, int32_t n,
+                             int32_t threads, int64_t *out_sizes);
 
 /* ---- post-process filters on cakes along eta ---- */
 

@@ -137,6 +137,8 @@ SIGNATURES = {
                                _PD, _PD]),
     "mc_fit_windows": (_I, [_V, _I32, _I32, _I32, _V, _V, _V, _D, _I32, _V, _I32]),
     "mc_map_fuse": (_I, [_V, _I32, _V, _I32, _P(_V)]),
+    "mc_blosc_decompress": (_I, [_V, _I64, _V, _I64, _P(_I64)]),
+    "mc_blosc_decompress_many": (_I, [_V, _V, _V, _I64, _I32, _I32, _V]),
     "mc_smooth_eta": (_I, [_V, _V, _I32, _I32, _I32, _D, _V, _I32, _I32, _V]),
     "mc_median_eta": (_I, [_V, _V, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _V]),
     "mc_fit_cake_rings": (_I, [_V, _V, _I32, _PD, _PD, _I32, _I32, _I32, _I32, _V, _I32]),
