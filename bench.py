@@ -97,6 +97,11 @@ def serving_bench(rates, n_frames, log):
             t_next = t0
             for i in range(n_frames):
                 t_next += gaps[i]
+                # sleep (releasing the interpreter lock) until shortly before
+                # the arrival, then spin to it
+                dt = t_next - time.perf_counter()
+                if dt > 3e-4:
+                    time.sleep(dt - 2e-4)
                 while time.perf_counter() < t_next:
                     pass
                 k = int(which[i])

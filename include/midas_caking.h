@@ -427,6 +427,16 @@ int mc_server_add_geometry(mc_server *s, mc_map *m, int32_t dtype,
  * Blocks while the queue is full (backpressure) up to timeout_ms. */
 int mc_server_submit(mc_server *s, int32_t geometry, const void *frame,
                      int64_t request_id, int32_t timeout_ms);
+/* Zero-copy submission: a pinned host buffer of the geometry's frame size
+ * (from the server's pool, allocated when the pool is empty).  The caller
+ * fills it (for instance with a socket receive) and either submits it --
+ * the server owns it again and returns it to the pool after the frame's
+ * batch -- or releases it unused.  Same queueing and backpressure as
+ * mc_server_submit. */
+int mc_server_acquire_buffer(mc_server *s, int32_t geometry, void **buf);
+int mc_server_submit_buffer(mc_server *s, int32_t geometry, void *buf,
+                            int64_t request_id, int32_t timeout_ms);
+int mc_server_release_buffer(mc_server *s, int32_t geometry, void *buf);
 int mc_server_drain(mc_server *s, int32_t timeout_ms);
 int mc_server_stats_get(mc_server *s, mc_server_stats *st);
 int mc_server_destroy(mc_server *s);

@@ -145,6 +145,9 @@ SIGNATURES = {
     "mc_server_create": (_I, [_V, _I64, _I32, _I32, RESULT_CB, _V, _P(_V)]),
     "mc_server_add_geometry": (_I, [_V, _V, _I32, _I32, _U32, _P(_I32)]),
     "mc_server_submit": (_I, [_V, _I32, _V, _I64, _I32]),
+    "mc_server_acquire_buffer": (_I, [_V, _I32, _P(_V)]),
+    "mc_server_submit_buffer": (_I, [_V, _I32, _V, _I64, _I32]),
+    "mc_server_release_buffer": (_I, [_V, _I32, _V]),
     "mc_server_drain": (_I, [_V, _I32]),
     "mc_server_stats_get": (_I, [_V, _P(ServerStats)]),
     "mc_server_destroy": (_I, [_V]),

@@ -49,7 +49,11 @@ class FrameServer:
         self.results: dict[int, Result] = {}
         self._shapes: dict[int, tuple] = {}
         self._lock = threading.Lock()
-        self._cb = _lib.RESULT_CB(self._callback)
+        # Without a consumer of the results no Python callback is
+        # registered: the server's completion thread then never waits for
+        # the interpreter lock.
+        self._cb = (_lib.RESULT_CB(self._callback) if (keep_results or on_result is not None)
+                    else _lib.RESULT_CB())  # a NULL function pointer
         h = C.c_void_p()
         check(lib().mc_server_create(self.ctx.handle, hbm_budget_bytes, max_batch, queue_depth,
                                      self._cb, None, C.byref(h)))
@@ -87,6 +91,26 @@ class FrameServer:
             raise ValueError("frame size does not match the geometry")
         check(lib().mc_server_submit(self.handle, geometry, a.ctypes.data, request_id,
                                      timeout_ms))
+
+    def acquire(self, geometry: int) -> np.ndarray:
+        """A pinned frame buffer of the geometry (shape (NZ, NY), its dtype)
+        to fill in place and hand to :meth:`submit_buffer` (zero copy), or
+        give back with :meth:`release`."""
+        geom, dt = self._geoms[geometry]
+        p = C.c_void_p()
+        check(lib().mc_server_acquire_buffer(self.handle, geometry, C.byref(p)))
+        n = geom.n_pixels_y * geom.n_pixels_z
+        raw = (C.c_char * (n * dt.itemsize)).from_address(p.value)
+        return np.frombuffer(raw, dtype=dt).reshape(geom.n_pixels_z, geom.n_pixels_y)
+
+    def submit_buffer(self, geometry: int, buf: np.ndarray, request_id: int,
+                      timeout_ms: int = 10000):
+        """Queues an acquired buffer; the server owns it again."""
+        check(lib().mc_server_submit_buffer(self.handle, geometry, buf.ctypes.data, request_id,
+                                            timeout_ms))
+
+    def release(self, geometry: int, buf: np.ndarray):
+        check(lib().mc_server_release_buffer(self.handle, geometry, buf.ctypes.data))
 
     def drain(self, timeout_ms: int = 60000):
         check(lib().mc_server_drain(self.handle, timeout_ms))
@@ -136,6 +160,16 @@ class FrameServer:
                         if body is None:
                             return
                         frame = decode_hybrid(body, npx, cnt)
+                    elif dtype in WIRE_BYTES and np.dtype(WIRE_NUMPY[dtype]) == dt:
+                        # the wire carries the geometry's own pixel type:
+                        # receive straight into a pinned buffer (zero copy)
+                        buf = self.acquire(g)
+                        if not _recv_into(conn, memoryview(buf).cast("B")):
+                            self.release(g, buf)
+                            return  # disconnect mid-payload: the frame is dropped
+                        self.submit_buffer(g, buf, (cid << 40) + n)
+                        n += 1
+                        continue
                     elif dtype in WIRE_BYTES:
                         body = _recv_exact(conn, npx * WIRE_BYTES[dtype])
                         if body is None:
@@ -163,6 +197,16 @@ class FrameServer:
         self._tcp = None
         if s is not None:
             s.close()
+
+
+def _recv_into(conn, view) -> bool:
+    got = 0
+    while got < len(view):
+        k = conn.recv_into(view[got:])
+        if k == 0:
+            return False
+        got += k
+    return True
 
 
 def _recv_exact(conn, n):

@@ -89,3 +89,32 @@ def test_tcp_wire_protocol():
             alone, _ = ma.integrate_batch(frames[i], g, mode="bilinear", mask_nan=True)
             np.testing.assert_allclose(srv.results[rid].cake, alone[0], rtol=1e-5, atol=1e-3,
                                        equal_nan=True)
+
+
+def test_zero_copy_buffers_and_large_frames():
+    """Frames of 4.5 MiB (copied on 4 threads by submit) and frames written
+    straight into acquired pinned buffers give the same cakes as
+    integrating them alone; a released buffer is reused."""
+    P = IntegrationParams(NrPixelsY=1536, NrPixelsZ=1536, pxY=200.0, pxZ=200.0, Lsd=1e6,
+                          BC_y=768.2, BC_z=767.9, RMin=5.0, RMax=700.0, RBinSize=1.0,
+                          EtaBinSize=5.0)
+    g = ma.build_map(P)
+    rng = np.random.default_rng(2)
+    frames = rng.poisson(50, (6, P.NrPixelsZ, P.NrPixelsY)).astype(np.uint16)
+    print("[server] zero-copy and 4.5 MiB frames", flush=True)
+    with FrameServer(hbm_budget_bytes=16 << 30, max_batch=4) as srv:
+        gid = srv.add_geometry(g, np.uint16)
+        spare = srv.acquire(gid)
+        assert spare.shape == (P.NrPixelsZ, P.NrPixelsY) and spare.dtype == np.uint16
+        srv.release(gid, spare)
+        for i in range(3):
+            srv.submit(gid, frames[i], i)
+        for i in range(3, 6):
+            buf = srv.acquire(gid)
+            buf[...] = frames[i]
+            srv.submit_buffer(gid, buf, i)
+        srv.drain()
+        for i in range(6):
+            alone, lo = ma.integrate_batch(frames[i], g, mode="bilinear", mask_nan=True)
+            np.testing.assert_allclose(srv.results[i].cake, alone[0], rtol=1e-5, atol=1e-3,
+                                       equal_nan=True)
