@@ -159,7 +159,9 @@ int blosc_decode(const uint8_t *src, int64_t n_src, uint8_t *dst, int64_t cap, i
   const bool shuf = (flags & kShuffle) != 0, bshuf = (flags & kBitShuffle) != 0;
   const int64_t nblocks = (nbytes + blocksize - 1) / blocksize;
   MC_CHECK(16 + 4 * nblocks <= n_src, MC_ERR_FORMAT, "blosc block table is truncated");
-  std::vector<uint8_t> tmp((size_t)blocksize);
+  // (a block is never longer than the chunk: a larger stated block size
+  // means one short block)
+  std::vector<uint8_t> tmp((size_t)std::min(blocksize, nbytes));
   for (int64_t b = 0; b < nblocks; b++) {
     const int64_t bsize = std::min(blocksize, nbytes - b * blocksize);
     const bool leftover = bsize < blocksize;
