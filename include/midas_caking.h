@@ -210,9 +210,10 @@ int mc_map_set_tuning(mc_map *m, int32_t batch, int32_t rows_per_group,
 int mc_map_get_tuning(mc_map *m, int32_t *batch, int32_t *rows_per_group,
                       int32_t *lds_bytes);
 /* Layout of the batched kernel for (mode, dtype) under the current tuning
- * (built if needed): out[8] = tiles, chunks (workgroups), direct bins,
+ * (built if needed): out[9] = tiles, chunks (workgroups), direct bins,
  * staged pixels per frame, entries, LDS bytes per workgroup, bands,
- * pixels per staging load. */
+ * bytes per batch element (2 or 4), and 64-pixel blocks holding pixels of
+ * the map (the blocks the transposes rewrite). */
 int mc_map_layout_info(mc_map *m, int32_t mode, int32_t dtype, int64_t *out);
 /* Times candidate tunings on synthetic frames of the map's shape and keeps
  * the fastest; writes the winner's figures. */

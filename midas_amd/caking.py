@@ -241,11 +241,11 @@ class CSRGeometry:
 
     def layout_info(self, mode: str = "bilinear", dtype=np.uint16) -> dict:
         """Tile layout of the batched kernel under the current tuning."""
-        out = (C.c_int64 * 8)()
+        out = (C.c_int64 * 9)()
         check(lib().mc_map_layout_info(self.handle, _lib.MODES[mode],
                                        NP_TO_CODE[np.dtype(dtype)], out))
         keys = ("tiles", "chunks", "direct_bins", "staged_px", "entries", "lds_bytes", "bands",
-                "vec")
+                "vec", "px_blocks")
         return dict(zip(keys, (int(v) for v in out)))
 
     def close(self):

@@ -501,6 +501,26 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
     return rc;
   if (!tpix.empty())
     MC_HIP(hipMemcpy(t->tpix.p, tpix.data(), tpix.size() * 4, hipMemcpyHostToDevice));
+  // Pixel blocks the transposes rewrite: those holding a pixel of the map.
+  {
+    const int64_t npx = m->npx();
+    std::vector<char> used((size_t)((npx + 63) / 64), 0);
+    for (int32_t p : c.h_pix) used[(size_t)p / 64] = 1;
+    std::vector<int32_t> b64, b128;
+    for (size_t i = 0; i < used.size(); i++)
+      if (used[i]) b64.push_back((int32_t)i);
+    for (size_t i = 0; i < used.size(); i += 2)
+      if (used[i] || (i + 1 < used.size() && used[i + 1])) b128.push_back((int32_t)(i / 2));
+    t->n_blocks64 = (int)b64.size();
+    t->n_blocks128 = (int)b128.size();
+    if ((rc = t->blocks64.alloc(std::max<size_t>(b64.size(), 1) * 4)) ||
+        (rc = t->blocks128.alloc(std::max<size_t>(b128.size(), 1) * 4)))
+      return rc;
+    if (!b64.empty())
+      MC_HIP(hipMemcpy(t->blocks64.p, b64.data(), b64.size() * 4, hipMemcpyHostToDevice));
+    if (!b128.empty())
+      MC_HIP(hipMemcpy(t->blocks128.p, b128.data(), b128.size() * 4, hipMemcpyHostToDevice));
+  }
   MC_HIP(hipMemcpy(t->chunks.p, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->tiles.p, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->tbins.p, tbins.data(), tbins.size() * 16, hipMemcpyHostToDevice));
@@ -854,7 +874,8 @@ int mc_map_info_get(mc_map *m, mc_map_info *info) {
     hb += c.row.bytes + c.pix.bytes + c.w.bytes + c.dterm.bytes;
     for (auto &t : m->tiles[q])
       hb += t->chunks.bytes + t->tiles.bytes + t->tbins.bytes + t->ents.bytes +
-            t->band_chunk.bytes + t->part.bytes + t->recs.bytes + t->tpix.bytes;
+            t->band_chunk.bytes + t->part.bytes + t->recs.bytes + t->tpix.bytes +
+            t->blocks64.bytes + t->blocks128.bytes;
   }
   info->hbm_bytes = hb;
   for (int q = 0; q < 3; q++) info->skipped[q] = m->skipped[q];
@@ -1164,9 +1185,9 @@ int mc_map_layout_info(mc_map *m, int32_t mode, int32_t dtype, int64_t *out) {
   Tiles *t = nullptr;
   int rc = ensure_tiles(m, mode, batch_elem(dtype), &t);
   if (rc) return rc;
-  const int64_t v[8] = {t->n_tiles, t->n_chunks, t->n_direct, t->staged_px,
-                        t->n_ents,  (int64_t)t->lds_needed, t->n_bands, t->in_size};
-  for (int i = 0; i < 8; i++) out[i] = v[i];
+  const int64_t v[9] = {t->n_tiles, t->n_chunks,  t->n_direct, t->staged_px, t->n_ents,
+                        (int64_t)t->lds_needed, t->n_bands, t->in_size, t->n_blocks64};
+  for (int i = 0; i < 9; i++) out[i] = v[i];
   return MC_OK;
 }
 

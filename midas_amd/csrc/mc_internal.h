@@ -112,6 +112,8 @@ struct Tiles {
   DBuf ents;         // uint32 x2: byte offset of the staged pixel row (or global pixel), weight bits
   DBuf band_chunk;   // int32 [n_bands+1]: chunk range of each band
   DBuf tpix;         // int32: staged pixels of every tile (tile t at px_beg .. + npx)
+  DBuf blocks64, blocks128;  // int32: 64- / 128-pixel blocks holding pixels of the map
+  int n_blocks64 = 0, n_blocks128 = 0;
   DBuf xcd;          // int32 [9]: chunk range of each XCD (equal cost)
   int grid = 0;      // workgroups of the launch: 8 x the largest XCD range
   DBuf part;         // float [n_chunks][rows][4][64]: lineout partials

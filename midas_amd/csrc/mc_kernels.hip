@@ -286,11 +286,13 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 template <typename TIn, typename TL>
 __global__ __launch_bounds__(256) void k_transpose(const TIn *__restrict__ frames,
                                                    int64_t frame_stride, int n_frames,
-                                                   int64_t npx, int vec_ok, TL *__restrict__ xt) {
+                                                   int64_t npx, int vec_ok,
+                                                   const int32_t *__restrict__ blocks,
+                                                   TL *__restrict__ xt) {
   constexpr int PX = 64;                       // pixels per block
   constexpr int EV = 16 / (int)sizeof(TL);     // elements per 16-byte store
   __shared__ TL tile[kXtFrames][PX + 4 / (int)sizeof(TL)];
-  const int64_t p0 = (int64_t)blockIdx.x * PX;
+  const int64_t p0 = (int64_t)(blocks ? blocks[blockIdx.x] : (int32_t)blockIdx.x) * PX;
   const int t = threadIdx.x;
   if (vec_ok && sizeof(TIn) == 2 && sizeof(TL) == 2) {
     // 16-byte loads: thread -> (frame t/4, 16 pixels as two vectors)
@@ -346,9 +348,10 @@ __global__ __launch_bounds__(256) void k_transpose(const TIn *__restrict__ frame
 // 8 pixels per wave) fall on 32 different banks.
 __global__ __launch_bounds__(256) void k_transpose16(const uint16_t *__restrict__ frames,
                                                      int64_t frame_stride, int n_frames,
-                                                     int64_t npx, uint16_t *__restrict__ xt) {
+                                                     int64_t npx, const int32_t *__restrict__ blocks,
+                                                     uint16_t *__restrict__ xt) {
   __shared__ __attribute__((aligned(16))) uint16_t tile[kXtFrames * 128];
-  const int64_t p0 = (int64_t)blockIdx.x * 128;
+  const int64_t p0 = (int64_t)(blocks ? blocks[blockIdx.x] : (int32_t)blockIdx.x) * 128;
   const int t = threadIdx.x;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
@@ -932,12 +935,17 @@ static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mas
   // 1. pixel-major batch
   const int64_t npx = m.npx();
   const int vec_ok = (npx % 64 == 0) && (((uintptr_t)frames & 15) == 0) ? 1 : 0;
-  if (sizeof(TIn) == 2 && sizeof(TL) == 2 && vec_ok && npx % 128 == 0)
-    hipLaunchKernelGGL(k_transpose16, dim3((unsigned)(npx / 128)), dim3(256), 0, s,
-                       (const uint16_t *)frames, npx, n_frames, npx, (uint16_t *)xt);
-  else
-    hipLaunchKernelGGL((k_transpose<TIn, TL>), dim3((unsigned)((npx + 63) / 64)), dim3(256), 0,
-                       s, (const TIn *)frames, npx, n_frames, npx, vec_ok, (TL *)xt);
+  // Only the blocks holding pixels the map reads are rewritten.
+  if (sizeof(TIn) == 2 && sizeof(TL) == 2 && vec_ok && npx % 128 == 0) {
+    if (t.n_blocks128)
+      hipLaunchKernelGGL(k_transpose16, dim3((unsigned)t.n_blocks128), dim3(256), 0, s,
+                         (const uint16_t *)frames, npx, n_frames, npx,
+                         t.blocks128.as<int32_t>(), (uint16_t *)xt);
+  } else if (t.n_blocks64) {
+    hipLaunchKernelGGL((k_transpose<TIn, TL>), dim3((unsigned)t.n_blocks64), dim3(256), 0, s,
+                       (const TIn *)frames, npx, n_frames, npx, vec_ok, t.blocks64.as<int32_t>(),
+                       (TL *)xt);
+  }
   MC_HIP(hipGetLastError());
   // 2. tiles
   auto kern = k_batched<TL, G>;

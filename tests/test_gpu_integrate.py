@@ -221,3 +221,24 @@ def test_passes_match_single_passes():
     c2, l2 = ma.integrate_batch(frames[::-1].copy(), g, mode="bilinear", mask_nan=True)
     np.testing.assert_array_equal(c2[::-1], cb)
     np.testing.assert_array_equal(l2[::-1], lb)
+
+
+@pytest.mark.parametrize("ny,nz", [(256, 192), (250, 192)])
+def test_unused_pixel_blocks_are_skipped(ny, nz):
+    """Rings up to 20 px on a 256 x 192 (250 x 192) detector: the
+    transposes rewrite only the pixel blocks the map reads (most blocks
+    hold none), with the 128-pixel kernel and with the 64-pixel one; the
+    batched cakes still equal the one-frame kernel's, which reads the
+    frames directly."""
+    P = IntegrationParams(NrPixelsY=ny, NrPixelsZ=nz, pxY=200.0, pxZ=200.0, Lsd=1e6,
+                          BC_y=60.3, BC_z=130.6, RMin=2.0, RMax=20.0, RBinSize=0.5,
+                          EtaBinSize=10.0)
+    rng = np.random.default_rng(13)
+    frames = rng.poisson(300.0, (7, nz, ny)).astype(np.uint16)
+    print(f"[integrate] sparse blocks {ny}x{nz}", flush=True)
+    g = ma.build_map(P)
+    cb, lb = ma.integrate_batch(frames, g, mode="bilinear", mask_nan=True)
+    for i in range(len(frames)):
+        c1, l1 = ma.integrate_batch(frames[i], g, mode="bilinear", mask_nan=True)
+        np.testing.assert_allclose(cb[i], c1[0], rtol=3e-6, atol=1e-4, equal_nan=True)
+        np.testing.assert_allclose(lb[i], l1[0], rtol=3e-6, atol=1e-4)
