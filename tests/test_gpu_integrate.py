@@ -237,6 +237,7 @@ def test_unused_pixel_blocks_are_skipped(ny, nz):
     frames = rng.poisson(300.0, (7, nz, ny)).astype(np.uint16)
     print(f"[integrate] sparse blocks {ny}x{nz}", flush=True)
     g = ma.build_map(P)
+    assert 0 < g.layout_info()["px_blocks"] < ny * nz // 64 // 2
     cb, lb = ma.integrate_batch(frames, g, mode="bilinear", mask_nan=True)
     for i in range(len(frames)):
         c1, l1 = ma.integrate_batch(frames[i], g, mode="bilinear", mask_nan=True)
