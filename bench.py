@@ -210,6 +210,13 @@ def main():
     fps_b = args.frames * args.steps / wall
     log(f"batched: {fps_b:,.0f} frames/s wall, kernel {ms_launch_b*1e3:.1f} us/launch")
 
+    # ---- the same steps replayed as one HIP graph each ----
+    log("batched passes replayed as a HIP graph")
+    _, ms_iter_g = ma.time_integrate(g, frames, kind="graph", mode=args.mode, iters=args.steps,
+                                     warmup=args.warmup, cakes_dev=cakes, lineouts_dev=lo)
+    fps_g = args.frames / (ms_iter_g / 1e3)
+    log(f"graph: {fps_g:,.0f} frames/s ({ms_iter_g:.3f} ms per step)")
+
     # ---- one-frame kernel: latency distribution ----
     log("one-frame kernel")
     ms_launch_p, ms_iter_p = ma.time_integrate(g, frames, kind="plain", mode=args.mode,
@@ -301,6 +308,8 @@ def main():
                                f"solid angle + polarisation, {args.mode}, map nnz {info.nnz}",
                    "batch": B, "rows_per_group": H, "lds_bytes": L,
                    "frames_per_step": args.frames, "layout": layout},
+        "graph": {"frames_per_s": fps_g, "ms_per_step": ms_iter_g,
+                  "what": "the step's batched passes replayed as one HIP graph (mc_graph_launch)"},
         "plain_fps": fps_p,
         "batched_over_plain": fps_b / fps_p,
         "latency_ms": {"p50": p50, "p99": p99, "what": "one frame through mc_integrate "

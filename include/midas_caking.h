@@ -230,6 +230,29 @@ int mc_integrate(mc_map *m, const void *frames, int32_t dtype,
                  uint32_t flags, float *cakes, int32_t cakes_on_device,
                  float *lineouts, int32_t lineouts_on_device);
 
+/* ---- the batched passes captured in a HIP graph ----
+ * For a caller that integrates the same device buffers over and over (the
+ * per-frame loop of IntegratorFitPeaksGPUStream.cu:2567-2883 refills one
+ * frame buffer per batch): mc_graph_create runs mc_integrate's batched
+ * passes once on device frames[n_frames] (n_frames >= 1, batched kernel
+ * also for one frame), captures the same passes into a graph and
+ * instantiates it; mc_graph_launch replays it on the context's stream
+ * (asynchronous: mc_ctx_sync waits), reading whatever frames_dev holds at
+ * that time and writing cakes_dev / lineouts_dev (device pointers, NULL
+ * lineouts for none).  A new dark (mc_map_set_dark) is folded into the
+ * graph's tables before each replay; a new tuning leaves the graph on the
+ * layout it captured; a new beam centre in MC_MODE_GRADIENT, or a frame
+ * type needing a larger batch buffer, makes mc_graph_launch fail with
+ * MC_ERR_STATE (create a new graph).  The graph must be destroyed before
+ * its map.  The reference has no counterpart:
+ * the CUDA path launches kernel by kernel. */
+typedef struct mc_graph mc_graph;
+int mc_graph_create(mc_map *m, const void *frames_dev, int32_t dtype, int32_t n_frames,
+                    int32_t mode, uint32_t flags, float *cakes_dev, float *lineouts_dev,
+                    mc_graph **out);
+int mc_graph_launch(mc_graph *g);
+int mc_graph_destroy(mc_graph *g);
+
 /* Cakes with propagated variance (integrate_with_variance,
  * packages/midas_integrate/midas_integrate/kernels.py:476-561): per bin,
  * besides the value mc_integrate returns for (mode, flags), the variance
@@ -302,9 +325,10 @@ int mc_median_eta(mc_ctx *ctx, const float *cakes, int32_t n_cakes, int32_t n_r,
 
 /* Launch timing of the kernels on device-resident frames, with HIP events
  * on the context's stream.  kind 0 = one-frame kernel (n_frames launches),
- * 1 = batched kernel (n_frames/batch launches per iteration).
- * ms_per_launch = mean kernel time, ms_per_iter = wall time per pass over
- * all n_frames. */
+ * 1 = batched kernel (n_frames/batch launches per iteration), 2 = the
+ * batched passes of one iteration replayed as one HIP graph (mc_graph_*).
+ * ms_per_launch = mean kernel time (kind 2: ms_per_iter / passes),
+ * ms_per_iter = wall time per pass over all n_frames. */
 int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype,
                       int32_t n_frames, int32_t mode, uint32_t flags,
                       int32_t kind, int32_t iters, int32_t warmup,

@@ -13,7 +13,8 @@ from .bin_io import (MAP_HEADER_MAGIC, MAP_HEADER_SIZE, PXLIST_DTYPE, MapHeader,
 from .params import (IntegrationParams, bin_edges, parse_integrator_params,
                      parse_mapper_params, parse_params)
 from .caking import (AREA_THRESHOLD, CSRGeometry, Context, build_csr, build_map,
-                     default_context, integrate, integrate_batch, integrate_with_variance,
+                     IntegrationGraph, default_context, integrate, integrate_batch,
+                     integrate_with_variance,
                      load_map_gpu, profile_1d, time_integrate)
 from .outputs import ResultFiles
 from .postprocess import gauss_smooth_eta, median_filter_eta
@@ -24,7 +25,8 @@ __all__ = [
     "PixelMap", "canonical_string", "header_for", "load_map", "write_map",
     "IntegrationParams", "bin_edges", "parse_integrator_params", "parse_mapper_params",
     "parse_params", "AREA_THRESHOLD", "CSRGeometry", "Context", "build_csr", "build_map",
-    "default_context", "integrate", "integrate_batch", "integrate_with_variance",
+    "IntegrationGraph", "default_context", "integrate", "integrate_batch",
+    "integrate_with_variance",
     "load_map_gpu", "profile_1d", "time_integrate", "ResultFiles", "gauss_smooth_eta",
     "median_filter_eta", "build_fused_geometry",
 ]

@@ -135,6 +135,9 @@ SIGNATURES = {
                                    _I32]),
     "mc_time_integrate": (_I, [_V, _V, _I32, _I32, _I32, _U32, _I32, _I32, _I32, _V, _V,
                                _PD, _PD]),
+    "mc_graph_create": (_I, [_V, _V, _I32, _I32, _I32, _U32, _V, _V, _P(_V)]),
+    "mc_graph_launch": (_I, [_V]),
+    "mc_graph_destroy": (_I, [_V]),
     "mc_fit_windows": (_I, [_V, _I32, _I32, _I32, _V, _V, _V, _D, _I32, _V, _I32]),
     "mc_map_fuse": (_I, [_V, _I32, _V, _I32, _P(_V)]),
     "mc_blosc_decompress": (_I, [_V, _I64, _V, _I64, _P(_I64)]),

@@ -451,6 +451,62 @@ def profile_1d(int2d, geom: CSRGeometry, *, mode: str = "area_weighted"):
     return out[0] if squeeze else out
 
 
+class IntegrationGraph:
+    """The batched passes over one device frame buffer, captured once in a
+    HIP graph and replayed by :meth:`launch`.  The caller refills
+    ``frames`` (a CUDA tensor [F, NZ, NY]) between launches; each launch
+    writes ``cakes`` [F, n_r, n_eta] and, with ``lineouts``, ``lineouts``
+    [F, 3, n_r] in place.  Results equal :func:`integrate_batch` on the same
+    frames (the same kernels run).  A new dark is picked up by the next
+    launch; a new tuning leaves the graph on its captured layout; a new beam
+    centre in gradient mode invalidates it: :meth:`launch` then raises."""
+
+    def __init__(self, geom: CSRGeometry, frames, *, mode: str = "bilinear",
+                 normalize: bool = True, mask_nan: bool = True, lineouts: bool = True):
+        if mode not in _lib.MODES:
+            raise ValueError(f"unknown integration mode {mode!r}")
+        if not _is_cuda(frames):
+            raise ValueError("IntegrationGraph reads device frames (a CUDA tensor)")
+        t = _torch()
+        npx = geom.n_pixels_y * geom.n_pixels_z
+        if not frames.is_contiguous() or frames.numel() % npx or frames.numel() == 0:
+            raise ValueError(f"frames must be a contiguous [F, {geom.n_pixels_z}, "
+                             f"{geom.n_pixels_y}] tensor")
+        F = frames.numel() // npx
+        self.geom, self.frames = geom, frames
+        self.cakes = t.empty((F, geom.n_r, geom.n_eta), dtype=t.float32, device=frames.device)
+        self.lineouts = (t.empty((F, 3, geom.n_r), dtype=t.float32, device=frames.device)
+                         if lineouts else None)
+        self.handle = C.c_void_p()
+        t.cuda.current_stream(frames.device).synchronize()
+        check(lib().mc_graph_create(
+            geom.handle, C.c_void_p(frames.data_ptr()), _dtype_code(frames), F,
+            _lib.MODES[mode], _flags(normalize, mask_nan), C.c_void_p(self.cakes.data_ptr()),
+            C.c_void_p(self.lineouts.data_ptr()) if lineouts else None,
+            C.byref(self.handle)))
+        geom.ctx.sync()
+
+    def launch(self, sync: bool = True):
+        """Replays the graph on the map's stream after the work already
+        queued on torch's current stream (the writes into ``frames``)."""
+        _torch().cuda.current_stream(self.frames.device).synchronize()
+        check(lib().mc_graph_launch(self.handle))
+        if sync:
+            self.geom.ctx.sync()
+        return self.cakes, self.lineouts
+
+    def close(self):
+        if self.handle:
+            lib().mc_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def time_integrate(geom: CSRGeometry, frames_dev, *, kind: str, mode="bilinear",
                    normalize=True, mask_nan=True, iters=10, warmup=2, cakes_dev=None,
                    lineouts_dev=None):
@@ -466,7 +522,7 @@ def time_integrate(geom: CSRGeometry, frames_dev, *, kind: str, mode="bilinear",
     check(lib().mc_time_integrate(
         geom.handle, C.c_void_p(frames_dev.data_ptr()), _dtype_code(frames_dev), F,
         _lib.MODES[mode], _flags(normalize, mask_nan),
-        {"plain": 0, "batched": 1}[kind], iters,
+        {"plain": 0, "batched": 1, "graph": 2}[kind], iters,
         warmup, C.c_void_p(cakes_dev.data_ptr()),
         C.c_void_p(lineouts_dev.data_ptr()) if lineouts_dev is not None else None,
         C.byref(a), C.byref(b)))

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -529,6 +530,8 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
                    hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->xcd.p, xcd.data(), 9 * 4, hipMemcpyHostToDevice));
   t->built = true;
+  static std::atomic<uint64_t> next_id{1};
+  t->id = next_id++;
   *out = t.get();
   m->tiles[mode].push_back(std::move(t));
   return MC_OK;
@@ -1318,12 +1321,129 @@ int mc_profile(mc_map *m, const float *cakes, int32_t n_frames, int32_t cakes_on
   return MC_OK;
 }
 
+// ---- the batched passes of one call, captured in a HIP graph ----
+struct mc_graph {
+  mc_map *m = nullptr;
+  int mode = 0, dtype = 0;
+  uint32_t flags = 0;
+  uint64_t tiles_id = 0;      // the tile layout the graph reads
+  Tiles *t = nullptr;
+  void *xt = nullptr;         // and the pixel-major batch
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int mc_graph_create(mc_map *m, const void *frames_dev, int32_t dtype, int32_t n_frames,
+                    int32_t mode, uint32_t flags, float *cakes_dev, float *lineouts_dev,
+                    mc_graph **out) {
+  MC_CHECK(m && frames_dev && cakes_dev && out && n_frames >= 1, MC_ERR_ARG, "bad arguments");
+  MC_CHECK(mode >= 0 && mode <= 2, MC_ERR_ARG, "mode %d", mode);
+  MC_CHECK(dtype_size(dtype) > 0 && dtype != MC_DTYPE_HYBRID, MC_ERR_ARG,
+           "unsupported frame dtype %d", dtype);
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  std::lock_guard<std::mutex> lk2(m->mu);
+  MC_HIP(hipSetDevice(m->ctx->device));
+  hipStream_t s = m->ctx->stream;
+  int rc;
+  // One eager call builds every table and buffer the passes read (nothing
+  // is allocated or copied while capturing) and sets the kernels' LDS
+  // attributes.
+  if ((rc = run_integrate(m, s, frames_dev, dtype, n_frames, mode, flags, cakes_dev,
+                          lineouts_dev, false, true)))
+    return rc;
+  MC_HIP(hipStreamSynchronize(s));
+  Tiles *t = nullptr;
+  if ((rc = ensure_tiles(m, mode, batch_elem(dtype), &t))) return rc;
+  auto *g = new mc_graph();
+  std::unique_ptr<mc_graph> guard(g);
+  g->m = m;
+  g->mode = mode;
+  g->dtype = dtype;
+  g->flags = flags;
+  g->t = t;
+  g->tiles_id = t->id;
+  g->xt = m->xt.p;
+  MC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  rc = run_integrate(m, s, frames_dev, dtype, n_frames, mode, flags, cakes_dev, lineouts_dev,
+                     false, true);
+  const hipError_t e = hipStreamEndCapture(s, &g->graph);
+  if (rc) return rc;
+  MC_CHECK(e == hipSuccess && g->graph, MC_ERR_HIP, "graph capture failed: %s",
+           hipGetErrorString(e));
+  MC_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  *out = guard.release();
+  return MC_OK;
+}
+
+int mc_graph_launch(mc_graph *g) {
+  MC_CHECK(g && g->exec, MC_ERR_ARG, "null graph");
+  mc_map *m = g->m;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  std::lock_guard<std::mutex> lk2(m->mu);
+  MC_HIP(hipSetDevice(m->ctx->device));
+  // The graph reads the map's buffers in place: the tile layout and the
+  // pixel-major batch must still be the ones captured (a new beam centre in
+  // gradient mode rebuilds the layout, a larger frame type the batch); a
+  // new dark is folded into the same record buffer before the launch.
+  const float2 *binfo = nullptr;
+  int rc;
+  if ((rc = ensure_binfo(m, g->mode, g->flags, &binfo))) return rc;
+  bool alive = false;
+  for (auto &tp : m->tiles[g->mode]) alive |= tp.get() == g->t && tp->id == g->tiles_id;
+  MC_CHECK(alive && m->xt.p == g->xt, MC_ERR_STATE,
+           "the map's batched layout changed since the graph was captured");
+  if ((rc = ensure_recs(m, m->ctx->stream, g->t, g->mode, g->flags, binfo))) return rc;
+  MC_HIP(hipGraphLaunch(g->exec, m->ctx->stream));
+  return MC_OK;
+}
+
+int mc_graph_destroy(mc_graph *g) {
+  if (!g) return MC_OK;
+  {
+    std::lock_guard<std::recursive_mutex> lk(g->m->ctx->mu);
+    (void)hipStreamSynchronize(g->m->ctx->stream);
+  }
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return MC_OK;
+}
+
 int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype, int32_t n_frames,
                       int32_t mode, uint32_t flags, int32_t kind, int32_t iters, int32_t warmup,
                       float *cakes_dev, float *lineouts_dev, double *ms_per_launch,
                       double *ms_per_iter) {
   MC_CHECK(m && frames_dev && cakes_dev && n_frames >= 1 && iters >= 1, MC_ERR_ARG,
            "bad arguments");
+  MC_CHECK(kind >= 0 && kind <= 2, MC_ERR_ARG, "timing kind %d", kind);
+  if (kind == 2) {  // the batched passes replayed as one graph per iteration
+    mc_graph *g = nullptr;
+    int rc = mc_graph_create(m, frames_dev, dtype, n_frames, mode, flags, cakes_dev,
+                             lineouts_dev, &g);
+    if (rc) return rc;
+    std::unique_ptr<mc_graph, int (*)(mc_graph *)> keep(g, mc_graph_destroy);
+    for (int w = 0; w < warmup; w++)
+      if ((rc = mc_graph_launch(g))) return rc;
+    hipStream_t s = m->ctx->stream;
+    MC_HIP(hipStreamSynchronize(s));
+    hipEvent_t e0, e1;
+    MC_HIP(hipEventCreate(&e0));
+    MC_HIP(hipEventCreate(&e1));
+    MC_HIP(hipEventRecord(e0, s));
+    for (int it = 0; it < iters && !rc; it++) rc = mc_graph_launch(g);
+    MC_HIP(hipEventRecord(e1, s));
+    MC_HIP(hipEventSynchronize(e1));
+    float tot = 0;
+    MC_HIP(hipEventElapsedTime(&tot, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    const int step = pass_frames(*g->t);
+    const int launches = (n_frames + step - 1) / step;
+    if (ms_per_launch) *ms_per_launch = tot / iters / launches;
+    if (ms_per_iter) *ms_per_iter = tot / iters;
+    return MC_OK;
+  }
   std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
   std::lock_guard<std::mutex> lk2(m->mu);
   MC_HIP(hipSetDevice(m->ctx->device));

@@ -98,6 +98,7 @@ struct Csr {
 // LDS budget).
 struct Tiles {
   bool built = false;
+  uint64_t id = 0;   // unique per layout built (captured graphs check it)
   int batch = 0, rows = 0, lds_bytes = 0;  // frames per pass, band rows (16 * G), LDS budget
   int in_size = 0;   // bytes per element of the pixel-major batch (2 or 4)
   int n_chunks = 0, n_bands = 0;
