@@ -40,7 +40,9 @@
  *                       (PeakFitIO.c:52-267, IntegratorZarrOMP.c:2176-2195),
  *                       and the GPU stream's lineout.bin,
  *                       lineout_simple_mean.bin, Int2D.bin and its writer
- *                       thread (IntegratorFitPeaksGPUStream.cu:506-565)
+ *                       thread (IntegratorFitPeaksGPUStream.cu:506-565);
+ *                       the caked file's Zarr zip copy of
+ *                       FF_HEDM/workflows/integrator.py:379-426
  *   mc_blosc_*          blosc1_decompress of the Zarr-zip chunks,
  *                       IntegratorZarrOMP.c:1416-1454, 1646-1669 and
  *                       ZarrReader.c:7-62
@@ -373,6 +375,12 @@ typedef struct mc_out mc_out;
 #define MC_OUT_STREAM 0x20u     /* <dir>/lineout.bin and lineout_simple_mean.bin */
 #define MC_OUT_INT2D 0x40u      /* <dir>/Int2D.bin */
 #define MC_OUT_PEAKS 0x80u      /* <prefix>_fit.bin, _fit_per_eta.csv, _caked_peaks.h5 */
+/* With any of the .caked.hdf parts: also <dir>/<name>.caked.hdf.zarr.zip, the
+ * same groups, datasets and attributes as a Zarr (format 2) zip store, as the
+ * reference's conversion writes it (FF_HEDM/workflows/integrator.py:379-426,
+ * utils/converters/midas2zip.py:31-93): one chunk per array, Blosc entry
+ * (zstd, level 3, bit shuffle), chunks stored uncompressed. */
+#define MC_OUT_CAKED_ZARR 0x100u
 
 /* An output set.  <name> is the data file's name: the .caked.hdf file is
  * <dir>/<name>.caked.hdf and <prefix> is <name> without its extension.

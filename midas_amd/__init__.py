@@ -19,6 +19,7 @@ from .caking import (AREA_THRESHOLD, CSRGeometry, Context, build_csr, build_map,
 from .outputs import ResultFiles
 from .postprocess import gauss_smooth_eta, median_filter_eta
 from .fused_csr import build_fused_geometry
+from .zarr_out import enrich_zarr
 
 __all__ = [
     "__version__", "MAP_HEADER_MAGIC", "MAP_HEADER_SIZE", "PXLIST_DTYPE", "MapHeader",
@@ -28,5 +29,5 @@ __all__ = [
     "IntegrationGraph", "default_context", "integrate", "integrate_batch",
     "integrate_with_variance",
     "load_map_gpu", "profile_1d", "time_integrate", "ResultFiles", "gauss_smooth_eta",
-    "median_filter_eta", "build_fused_geometry",
+    "median_filter_eta", "build_fused_geometry", "enrich_zarr",
 ]

@@ -92,6 +92,7 @@ class PeakRow(C.Structure):
 
 MC_OUT_CAKED_HDF, MC_OUT_PER_FRAME, MC_OUT_SUM_FRAMES, MC_OUT_OMEGAS = 0x01, 0x02, 0x04, 0x08
 MC_OUT_LINEOUT, MC_OUT_STREAM, MC_OUT_INT2D, MC_OUT_PEAKS = 0x10, 0x20, 0x40, 0x80
+MC_OUT_CAKED_ZARR = 0x100
 
 
 RESULT_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.c_int32,

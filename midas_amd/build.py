@@ -33,7 +33,7 @@ PER_FILE = {
     "mc_mapper.hip": ["-ffp-contract=off"],
 }
 SOURCES = ["mc_core.cpp", "mc_kernels.hip", "mc_mapper.hip", "mc_fit.hip", "mc_server.cpp",
-           "mc_h5.cpp", "mc_out.cpp", "mc_post.hip", "mc_blosc.cpp"]
+           "mc_h5.cpp", "mc_out.cpp", "mc_post.hip", "mc_blosc.cpp", "mc_zarr.cpp"]
 
 
 def hipcc() -> str:

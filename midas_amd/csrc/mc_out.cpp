@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "mc_h5.h"
+#include "mc_zarr.h"
 #include "mc_internal.h"
 
 namespace {
@@ -35,13 +36,59 @@ struct PeakRow {  // PeakH5Row, PeakFitIO.h:27-40
   double eta_deg, center_2theta, area, sig, gam, fwhm, eta_mix, d_spacing, chi_sq;
 };
 
+// The caked results go to the HDF5 file and, when asked for, to its Zarr
+// copy (<name>.caked.hdf.zarr.zip): the same groups, datasets and
+// attributes through both writers.
+struct CakedSink {
+  mc::h5::Writer h5;
+  mc::zarr::Writer zr;
+  bool zarr = false;
+  std::string err;
+  int check(int a, const std::string &e) {
+    if (a && err.empty()) err = e;
+    return a;
+  }
+  int open(const std::string &h5_path, bool with_zarr) {
+    zarr = with_zarr;
+    if (check(h5.open(h5_path), h5.error())) return -1;
+    return zarr ? check(zr.open(h5_path + ".zarr.zip"), zr.error()) : 0;
+  }
+  int group(const std::string &p) {
+    return check(h5.group(p), h5.error()) || (zarr && check(zr.group(p), zr.error()));
+  }
+  int dataset(const std::string &p, mc::h5::Kind k, const std::vector<uint64_t> &dims,
+              const void *data) {
+    return check(h5.dataset(p, k, dims, data), h5.error()) ||
+           (zarr && check(zr.dataset(p, k, dims, data), zr.error()));
+  }
+  int attr_f64(const std::string &p, const std::string &n, const double *v, int c) {
+    return check(h5.attr_f64(p, n, v, c), h5.error()) ||
+           (zarr && check(zr.attr_f64(p, n, v, c), zr.error()));
+  }
+  int attr_i32(const std::string &p, const std::string &n, const int32_t *v, int c) {
+    return check(h5.attr_i32(p, n, v, c), h5.error()) ||
+           (zarr && check(zr.attr_i32(p, n, v, c), zr.error()));
+  }
+  int attr_str(const std::string &p, const std::string &n, const std::string &v) {
+    return check(h5.attr_str(p, n, v), h5.error()) ||
+           (zarr && check(zr.attr_str(p, n, v), zr.error()));
+  }
+  int close() {
+    const int a = check(h5.close(), h5.error());
+    const int b = zarr ? check(zr.close(), zr.error()) : 0;
+    return a || b;
+  }
+  bool is_open() const { return h5.is_open(); }
+  const std::string &error() const { return err; }
+};
+
 }  // namespace
 
 struct mc_out {
   mc_out_config cfg{};
   std::vector<double> reta, rc, eta_lo, eta_hi, instrument;
   std::string source, dir, name, prefix;
-  mc::h5::Writer caked;
+  CakedSink caked;
   FILE *f_lineout = nullptr, *f_xy = nullptr, *f_stream = nullptr, *f_stream_mean = nullptr,
        *f_int2d = nullptr, *f_fit = nullptr, *f_csv = nullptr;
   // OmegaSumFrame accumulation (IntegratorZarrOMP.c:1633-1642, 2093-2132)
@@ -364,7 +411,7 @@ int mc_out_open(const char *dir, const char *name, const mc_out_config *cfg, mc_
   if (cfg->files & MC_OUT_SUM_FRAMES) o->sum.assign(nb, 0.0);
   const std::string d = o->dir + "/";
   if (o->hdf()) {
-    if (o->caked.open(d + o->name + ".caked.hdf")) {
+    if (o->caked.open(d + o->name + ".caked.hdf", (cfg->files & MC_OUT_CAKED_ZARR) != 0)) {
       mc::set_error("%s", o->caked.error().c_str());
       return MC_ERR_IO;
     }

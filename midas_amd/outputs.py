@@ -15,7 +15,11 @@ files the reference integrators write:
   (IntegratorFitPeaksGPUStream.cu:506-565),
 * ``<prefix>_fit.bin``, ``<prefix>_fit_per_eta.csv`` and
   ``<prefix>_caked_peaks.h5`` (IntegratorZarrOMP.c:1530-1549, 2038-2058,
-  PeakFitIO.c:52-267).
+  PeakFitIO.c:52-267),
+* with ``"caked_zarr"``: ``<name>.caked.hdf.zarr.zip``, the caked file's
+  groups, datasets and attributes as a Zarr zip store (the conversion of
+  FF_HEDM/workflows/integrator.py:379-426); :func:`enrich_zarr` then adds
+  the input zip's metadata.
 
 ``<prefix>`` is the data file name without its extension.  The HDF5 files
 are written by the library's own HDF5 writer (no libhdf5 dependency).
@@ -37,6 +41,7 @@ FILES = {
     "sum_frames": _lib.MC_OUT_SUM_FRAMES, "omegas": _lib.MC_OUT_OMEGAS,
     "lineout": _lib.MC_OUT_LINEOUT, "stream": _lib.MC_OUT_STREAM,
     "int2d": _lib.MC_OUT_INT2D, "peaks": _lib.MC_OUT_PEAKS,
+    "caked_zarr": _lib.MC_OUT_CAKED_ZARR,
 }
 
 
