@@ -95,6 +95,150 @@ def fit_peaks(x, intensity, peak_locations, x_bin_size: float, fit_roi_padding: 
     return out
 
 
+# ---- automatic peak detection (PeakFit.c:148-246, 549-868) ----
+
+_SG = {5: ([-3, 12, 17, 12, -3], 35.0), 7: ([-2, 3, 6, 7, 6, 3, -2], 21.0),
+       9: ([-21, 14, 39, 54, 59, 54, 39, 14, -21], 231.0)}
+
+
+def smooth_data(y, window: int = 7) -> np.ndarray:
+    """Savitzky-Golay smoothing of pf_smoothData (PeakFit.c:148-213):
+    windows 5, 7 or 9 (others copy); the W//2 samples at each end are
+    copied."""
+    y = np.asarray(y, dtype=np.float64)
+    out = y.copy()
+    if window not in _SG or y.size < window:
+        return out
+    c, norm = _SG[window]
+    c = [v / norm for v in c]
+    H = window // 2
+    n = y.size
+    acc = np.zeros(n - 2 * H)
+    for j in range(window):  # the reference's summation order
+        acc = acc + c[j] * y[j:n - 2 * H + j]
+    out[H:n - H] = acc
+    return out
+
+
+def estimate_initial_params(y, peak_idx: int):
+    """pf_estimate_initial_params (PeakFit.c:218-246): (FWHM in samples,
+    background, amplitude) from the window's ends and the half-maximum
+    crossing."""
+    y = np.asarray(y, dtype=np.float64)
+    n = y.size
+    bw = max(int(min(5, n // 4)), 1)
+    s = 0.0
+    for i in range(bw):  # sum order of the reference
+        s += y[i]
+        s += y[n - 1 - i]
+    bg = s / (2.0 * bw)
+    amp = y[peak_idx] - bg
+    if amp <= 0:
+        amp = y[peak_idx]
+    half = bg + amp / 2.0
+    lo = peak_idx
+    while lo > 0 and y[lo] > half:
+        lo -= 1
+    hi = peak_idx
+    while hi < n - 1 and y[hi] > half:
+        hi += 1
+    fwhm = float(hi - lo)
+    return (fwhm if fwhm > 1.0 else 2.0), bg, amp
+
+
+def snip_background(y, n_iter: int = 50) -> np.ndarray:
+    """SNIP background with the LLS transform (pf_snip_background,
+    PeakFit.c:549-592): clipping windows n_iter down to 1."""
+    y = np.asarray(y, dtype=np.float64)
+    n = y.size
+    if n == 0 or n_iter <= 0:
+        return np.zeros(n)
+    v = np.log(np.log(np.sqrt(np.maximum(y, 0.0) + 1.0) + 1.0) + 1.0)
+    for p in range(n_iter, 0, -1):
+        if n - 2 * p <= 0:
+            continue
+        avg = (v[:n - 2 * p] + v[2 * p:]) / 2.0
+        v[p:n - p] = np.minimum(v[p:n - p], avg)
+    w = np.exp(np.exp(v) - 1.0) - 1.0
+    return np.maximum(w * w - 1.0, 0.0)
+
+
+def detect_peaks(x, corrected, max_peaks: int, min_separation: float) -> np.ndarray:
+    """pf_detect_peaks (PeakFit.c:620-712): local maxima of the negative
+    second difference of the SG-7 smoothed signal, ranked by prominence,
+    picked greedily at least max(3, min_separation / bin) bins apart, in x
+    order."""
+    x = np.asarray(x, dtype=np.float64)
+    c = np.asarray(corrected, dtype=np.float64)
+    n = c.size
+    if n < 10 or max_peaks <= 0:
+        return np.zeros(0)
+    sm = smooth_data(c, 7)
+    nd2 = np.zeros(n)
+    nd2[1:n - 1] = -(sm[2:] - 2.0 * sm[1:n - 1] + sm[:n - 2])
+    bw = abs(x[1] - x[0]) if n > 1 else 1.0
+    msep = int(max(3.0, min_separation / bw))
+    i = np.arange(2, n - 2)
+    ok = (nd2[i] > 0) & (nd2[i] >= nd2[i - 1]) & (nd2[i] >= nd2[i + 1]) & (c[i] > 0)
+    cands = []
+    for k in i[ok][: n // 2]:
+        lm = nd2[max(0, k - msep):k + 1].min()
+        rm = nd2[k:min(n, k + msep + 1)].min()
+        prom = nd2[k] - max(lm, rm)
+        cands.append((int(k), prom if prom >= 0 else nd2[k]))
+    cands.sort(key=lambda t: -t[1])  # stable, as glibc's merge-sort qsort
+    sel: list[int] = []
+    for k, _ in cands:
+        if len(sel) >= max_peaks:
+            break
+        if all(abs(k - s) >= msep for s in sel):
+            sel.append(k)
+    return x[np.array(sorted(sel), dtype=np.int64)] if sel else np.zeros(0)
+
+
+def select_auto_fits(rows, max_peaks: int, x_bin_size: float) -> list:
+    """Quality filter of fitPeaksAutoDetect (PeakFit.c:781-860): indices of
+    the fits kept, in detection order."""
+    ent = [(i, abs(r[0]), r[4]) for i, r in enumerate(rows) if abs(r[0]) > 0 and r[4] > 0]
+    if len(ent) > 2:
+        f = sorted(e[2] for e in ent)
+        mx = max(5.0 * f[len(ent) // 2], x_bin_size * 2.0)
+        ent = [e for e in ent if e[2] <= mx]
+    if len(ent) > max_peaks:
+        for a in range(len(ent) - 1):  # the reference's exchange sort, by area
+            for b in range(a + 1, len(ent)):
+                if ent[b][1] > ent[a][1]:
+                    ent[a], ent[b] = ent[b], ent[a]
+        ent = ent[:max_peaks]
+    return sorted(e[0] for e in ent)
+
+
+def fit_peaks_auto(x, intensity, max_peaks: int, x_bin_size: float,
+                   fit_roi_padding: int = 20, snip_iterations: int = 50, *,
+                   max_iter: int = 200, ctx: Context | None = None):
+    """fitPeaksAutoDetect (PeakFit.c:718-868): SNIP background, detection of
+    up to 3 x max_peaks candidates 3 bins apart, TCH fits of the
+    background-subtracted profile on the GPU (:func:`fit_peaks`), then the
+    quality filter.  Returns ([max_peaks, 7] zero-padded, count)."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(intensity, dtype=np.float64)
+    out = np.zeros((max(max_peaks, 0), PF_PARAMS_PER_PEAK))
+    n = y.size
+    if n == 0 or max_peaks <= 0:
+        return out, 0
+    corrected = np.maximum(y - snip_background(y, snip_iterations), 0.0)
+    over = min(max_peaks * 3, n // 2)
+    locs = detect_peaks(x, corrected, over, x_bin_size * 3.0)
+    if locs.size == 0:
+        return out, 0
+    rows = fit_peaks(x, corrected, locs, x_bin_size, fit_roi_padding, max_iter=max_iter,
+                     ctx=ctx)
+    keep = select_auto_fits(rows, max_peaks, x_bin_size)
+    for k, i in enumerate(keep):
+        out[k] = rows[i]
+    return out, len(keep)
+
+
 def pseudo_voigt(x, center, mu, fwhm, imax, bg):
     """Height-normalised pV of pf_pv_singlet_obj (PeakFit.c:882-920)."""
     x = np.asarray(x, dtype=np.float64)

@@ -96,3 +96,57 @@ def test_fit_peaks_on_a_lineout():
     out = peakfit.fit_peaks(x, y, locs, 0.25, 20)
     assert out.shape == (3, 7)
     np.testing.assert_allclose(out[:, 1], locs, atol=0.02)
+
+
+def _lineout(x, locs, slope=0.05, seed=None):
+    y = 20.0 + slope * (x - x[0])
+    for c in locs:
+        y = y + peakfit.pseudo_voigt(x, c, 0.4, 1.2, 500.0, 0.0)
+    if seed is not None:
+        y = y + np.random.default_rng(seed).normal(0, 0.3, x.size)
+    return y
+
+
+def test_fit_peaks_auto_finds_the_peaks():
+    """fitPeaksAutoDetect: SNIP removes the sloped background, detection
+    finds the three peaks, the fits put the centres on them."""
+    print("[fit] auto-detect", flush=True)
+    x = 10.0 + 0.25 * np.arange(400)
+    locs = [30.0, 55.5, 80.25]
+    y = _lineout(x, locs, seed=4)
+    out, n = peakfit.fit_peaks_auto(x, y, 3, 0.25, 20, 50)
+    assert out.shape == (3, 7) and n == 3  # the largest areas: the three peaks
+    np.testing.assert_allclose(out[:, 1], locs, atol=0.05)
+    # over-selection also fits noise bumps; the peaks are among the five kept
+    out5, n5 = peakfit.fit_peaks_auto(x, y, 5, 0.25, 20, 50)
+    assert 3 <= n5 <= 5 and np.all(out5[n5:] == 0)
+    assert np.all(np.diff(out5[:n5, 1]) > 0)
+    for c in locs:
+        assert np.min(np.abs(out5[:n5, 1] - c)) < 0.05
+    out1, n1 = peakfit.fit_peaks_auto(x, _lineout(x, locs, seed=4), 2, 0.25, 20, 50)
+    assert n1 == 2  # the two largest areas (all equal here: any two), in x order
+    assert out1[0, 1] < out1[1, 1]
+
+
+def test_stream_fitter_batches_frames(tmp_path):
+    """Specified peaks over many frames fitted in shared launches give the
+    per-frame fit_peaks rows; fit.bin holds them in frame order; a live
+    update switches the peak list."""
+    from midas_amd.stream_fits import StreamPeakFitter
+    print("[fit] stream fitter", flush=True)
+    x = 10.0 + 0.25 * np.arange(400)
+    locs = [30.0, 55.5, 80.25]
+    frames = [_lineout(x, [c + 0.01 * k for c in locs], seed=k) for k in range(12)]
+    with StreamPeakFitter(x, 0.25, locs, out_dir=tmp_path, max_batch=8) as f:
+        for k, y in enumerate(frames):
+            f.submit(k, y)
+    for k, y in enumerate(frames):
+        want = peakfit.fit_peaks(x, y, [x[np.argmin(np.abs(x - c))] for c in locs], 0.25, 20)
+        np.testing.assert_allclose(f.fitted[k], want, rtol=1e-6, atol=1e-9)
+    raw = np.fromfile(tmp_path / "fit.bin", "<f8").reshape(12, 3, 7)
+    np.testing.assert_array_equal(raw, np.stack([f.fitted[k] for k in range(12)]))
+    (tmp_path / "peak_update.txt").write_text("replace\n55.5\n")
+    with StreamPeakFitter(x, 0.25, locs, out_dir=tmp_path, fit_roi_auto=True) as g:
+        g.submit(0, frames[0])
+    assert g.locations == [55.5] and g.fitted[0].shape == (1, 7)
+    assert abs(g.fitted[0][0, 1] - 55.5) < 0.05

@@ -1,0 +1,87 @@
+"""Automatic peak detection (PeakFit.c:148-246, 549-868): the oracle's
+plain-loop restatement is pinned by properties the mathematics fixes, and
+the product's vectorised functions are compared with it.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import peak_detect as od
+from midas_amd import peakfit as pf
+
+
+def _profile(n=400, seed=0, peaks=((80.3, 1.2, 500.0), (200.0, 2.0, 300.0),
+                                   (320.6, 1.5, 800.0))):
+    rng = np.random.default_rng(seed)
+    x = 10.0 + 0.25 * np.arange(n)
+    i = np.arange(n, dtype=np.float64)
+    y = 50.0 + 0.1 * i
+    for c, s, a in peaks:
+        y = y + a * np.exp(-0.5 * ((i - c) / s) ** 2)
+    return x, y + rng.normal(0, 0.5, n)
+
+
+def test_savitzky_golay_reproduces_cubics():
+    """SG filters of order 2/3 leave cubic polynomials unchanged: pins the
+    window-5/7/9 coefficient tables."""
+    t = np.arange(40, dtype=np.float64)
+    y = 0.002 * t ** 3 - 0.3 * t ** 2 + 2.0 * t + 7.0
+    for W in (5, 7, 9):
+        s = od.smooth_data(list(y), W)
+        np.testing.assert_allclose(s, y, rtol=1e-12, atol=1e-10)
+    noisy = list(np.random.default_rng(1).normal(size=40))
+    assert od.smooth_data(noisy, 7)[:3] == noisy[:3]        # ends copied
+    assert od.smooth_data(noisy, 6) == noisy                 # unsupported window
+
+
+def test_snip_constant_and_narrow_peak():
+    """The LLS transform round-trips a constant background, and clipping
+    windows wider than a narrow peak bring it down to the background."""
+    c = [37.5] * 200
+    np.testing.assert_allclose(od.snip_background(c, 20), c, rtol=1e-12)
+    y = [37.5 + (400.0 * math.exp(-0.5 * ((i - 100) / 2.0) ** 2)) for i in range(200)]
+    bg = od.snip_background(y, 30)
+    assert max(abs(b - 37.5) for b in bg) < 0.05
+    assert od.snip_background(y, 0) == [0.0] * 200
+
+
+def test_detection_finds_generating_peaks():
+    x, y = _profile(peaks=((80.0, 1.5, 500.0), (200.0, 2.0, 300.0), (320.0, 1.5, 800.0)))
+    corr = [max(a - b, 0.0) for a, b in zip(y, od.snip_background(list(y), 50))]
+    locs = od.detect_peaks(list(x), corr, 3, 0.75)
+    assert locs == [x[80], x[200], x[320]]
+    # separation: two peaks 2 bins apart count once at min separation 3 bins
+    assert len(od.detect_peaks(list(x), corr, 9, 0.75)) >= 3
+
+
+def test_estimate_initial_params_on_a_box():
+    d = [1.0] * 5 + [11.0] * 6 + [1.0] * 5
+    fwhm, bg, amp = od.estimate_initial_params(d, 8)
+    assert (bg, amp) == (1.0, 10.0) and fwhm == 7.0   # crossings at 4 and 11
+
+
+def test_select_fits_rules():
+    rows = [[10, 0, 0, 0, 1.0, 0, 0], [0, 0, 0, 0, 1.0, 0, 0], [30, 0, 0, 0, 1.2, 0, 0],
+            [20, 0, 0, 0, 9.0, 0, 0], [5, 0, 0, 0, 0.9, 0, 0], [-40, 0, 0, 0, 1.1, 0, 0]]
+    # no area: out; FWHM 9 > 5 x median(1.1): out; top 3 areas: 40, 30, 10
+    assert od.select_fits(rows, 3, 0.25) == [0, 2, 5]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_product_matches_oracle(seed):
+    x, y = _profile(seed=seed)
+    for W in (5, 7, 9):
+        np.testing.assert_array_equal(pf.smooth_data(y, W), od.smooth_data(list(y), W))
+    np.testing.assert_allclose(pf.snip_background(y, 50), od.snip_background(list(y), 50),
+                               rtol=1e-13, atol=1e-12)
+    corr = np.maximum(y - pf.snip_background(y, 50), 0.0)
+    for mp, sep in ((3, 0.75), (9, 0.75), (20, 2.0)):
+        np.testing.assert_array_equal(pf.detect_peaks(x, corr, mp, sep),
+                                      od.detect_peaks(list(x), list(corr), mp, sep))
+    for k in (40, 200, 320):
+        assert pf.estimate_initial_params(y[k - 30:k + 30], 30) == \
+            od.estimate_initial_params(list(y[k - 30:k + 30]), 30)
+    rng = np.random.default_rng(seed)
+    rows = rng.normal(size=(12, 7))
+    rows[:, 4] = np.abs(rows[:, 4])
+    assert pf.select_auto_fits(rows, 4, 0.25) == od.select_fits(rows.tolist(), 4, 0.25)
