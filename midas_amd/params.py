@@ -79,6 +79,9 @@ class IntegrationParams:
     SaveIndividualFrames: int = 1
     DoPeakFit: int = 0
     FitROIPadding: int = 20
+    FitROIAuto: int = 0          # GPU stream: ROI from the widest FWHM estimate
+    DoSmoothing: int = 0         # GPU stream flags, read and kept
+    MultiplePeaks: int = 0
     AutoDetectPeaks: int = 0
     SNIPIterations: int = 50
     DoubletSeparation: float = 0.0
@@ -317,6 +320,9 @@ def parse_integrator_params(path: str | Path, params: IntegrationParams | None =
             elif key in ("U", "V", "W", "X", "Y", "Z"): setattr(P, key, fl())
             elif key == "DoPeakFit": P.DoPeakFit = it()
             elif key == "FitROIPadding": P.FitROIPadding = it()
+            elif key == "FitROIAuto": P.FitROIAuto = it()
+            elif key == "DoSmoothing": P.DoSmoothing = it()
+            elif key == "MultiplePeaks": P.MultiplePeaks = it()
             elif key == "AutoDetectPeaks":
                 P.AutoDetectPeaks = it()
                 P.DoPeakFit = 1

@@ -87,6 +87,15 @@ class StreamPeakFitter:
         self._th = threading.Thread(target=self._loop, daemon=True)
         self._th.start()
 
+    @classmethod
+    def from_params(cls, P, r_centers, **kw):
+        """The stream's fit settings of a parameter file (.cu:1930-1966):
+        PeakLocation, AutoDetectPeaks, SNIPIterations, FitROIPadding,
+        FitROIAuto; bins of RBinSize."""
+        return cls(r_centers, P.RBinSize, P.PeakLocations, auto_detect=P.AutoDetectPeaks,
+                   snip_iterations=P.SNIPIterations, fit_roi_padding=P.FitROIPadding,
+                   fit_roi_auto=bool(P.FitROIAuto), **kw)
+
     # ---- live updates (check_peak_update, .cu:1712-1776) ----
     def _write_active(self):
         with open(self.dir / "active_peaks.txt", "w") as f:

@@ -39,3 +39,19 @@ def test_peak_update_file(tmp_path):
         assert not f.poll_update() and f.locations == [8.0]
     finally:
         f.close()
+
+
+def test_fit_settings_from_a_parameter_file(tmp_path):
+    from midas_amd.params import parse_integrator_params
+    pf_ = tmp_path / "ps.txt"
+    pf_.write_text("RBinSize 0.5\nPeakLocation 12.5\nPeakLocation 30.25\nFitROIAuto 1\n"
+                   "FitROIPadding 12\nSNIPIterations 30\nDoSmoothing 1\nMultiplePeaks 1\n")
+    P = parse_integrator_params(pf_)
+    assert (P.FitROIAuto, P.DoSmoothing, P.MultiplePeaks, P.DoPeakFit) == (1, 1, 1, 1)
+    f = StreamPeakFitter.from_params(P, np.arange(100.0), out_dir=tmp_path)
+    try:
+        assert f.locations == [12.5, 30.25] and f.roi_auto and f.padding == 12
+        assert f.snip == 30 and f.r_bin_size == 0.5 and f.enabled
+        assert (tmp_path / "active_peaks.txt").read_text() == "12.5000\n30.2500\n"
+    finally:
+        f.close()
