@@ -41,11 +41,15 @@ template <typename T> __device__ __forceinline__ float to_f(T v) {
 // Sum over groups of 16 lanes with DPP row operations: quad xor 1, quad
 // xor 2, half-row mirror, row mirror.  Every lane of the group ends with the
 // group's total.
-__device__ __forceinline__ float dpp_sum16(float v) {
+// Sum over aligned groups of N lanes (N = 4, 8 or 16), every lane of a
+// group receiving the total: quad swaps, then the half-row and row mirrors.
+template <int N> __device__ __forceinline__ float dpp_sum(float v) {
   v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xf, 0xf, false));
   v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xf, 0xf, false));
-  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xf, 0xf, false));
-  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xf, 0xf, false));
+  if (N >= 8)
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xf, 0xf, false));
+  if (N >= 16)
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xf, 0xf, false));
   return v;
 }
 
@@ -98,17 +102,19 @@ __device__ __forceinline__ void block_sum4(float4 &v, float4 *red) {
 }
 
 // ---------------------------------------------------------------------
-// One-frame kernel: workgroup = one R row of one frame, 16 lanes per bin.
-// grid = (n_r, n_frames).
+// One-frame kernel: workgroup = one R row of one frame, LPB lanes per bin
+// (8: a bin has about 15 entries; 16 lanes leave half of them idle, 4
+// lengthen each lane's chain of dependent loads).  grid = (n_r, n_frames).
 // ---------------------------------------------------------------------
-template <typename T>
+constexpr int kPlainLanes = 8;
+template <typename T, int LPB>
 __global__ __launch_bounds__(256) void k_plain(
     const int32_t *__restrict__ row, const int32_t *__restrict__ pix,
     const float *__restrict__ w, const float2 *__restrict__ binfo,
     const float *__restrict__ a_inb, const float4 *__restrict__ row_static,
     const T *__restrict__ frames, int64_t frame_stride, int n_eta, int n_r,
     int64_t nb, int mask_nan, float *__restrict__ cake, float *__restrict__ lineouts) {
-  constexpr int LPB = 16, SLOTS = 256 / LPB;
+  constexpr int SLOTS = 256 / LPB;
   const int r = blockIdx.x, f = blockIdx.y;
   const T *__restrict__ img = frames + (int64_t)f * frame_stride;
   float *__restrict__ out = cake + (int64_t)f * nb;
@@ -119,7 +125,7 @@ __global__ __launch_bounds__(256) void k_plain(
     const int beg = row[b], end = row[b + 1];
     Kahan acc;
     for (int k = beg + sub; k < end; k += LPB) acc.add(to_f(img[pix[k]]) * w[k]);
-    float v = dpp_sum16(acc.value());
+    float v = dpp_sum<LPB>(acc.value());
     if (sub == 0) {
       const float2 bi = binfo[b];
       const float val = (v - bi.y) * fabsf(bi.x);
@@ -160,7 +166,7 @@ __global__ __launch_bounds__(256) void k_plain_var(
     const float *__restrict__ d2term, const T *__restrict__ frames, const float *__restrict__ vimg,
     int64_t frame_stride, int n_eta, int64_t nb, float *__restrict__ cake,
     float *__restrict__ var) {
-  constexpr int LPB = 16, SLOTS = 256 / LPB;
+  constexpr int LPB = kPlainLanes, SLOTS = 256 / LPB;  // the one-frame kernel's order
   const int r = blockIdx.x, f = blockIdx.y;
   const T *__restrict__ img = frames + (int64_t)f * frame_stride;
   const float *__restrict__ vi = POISSON ? nullptr : vimg + (int64_t)f * frame_stride;
@@ -175,8 +181,8 @@ __global__ __launch_bounds__(256) void k_plain_var(
       acc.add(x * ww);
       acc2.add((POISSON ? x : vi[p]) * ww * ww);
     }
-    const float v = dpp_sum16(acc.value());
-    const float v2 = dpp_sum16(acc2.value());
+    const float v = dpp_sum<LPB>(acc.value());
+    const float v2 = dpp_sum<LPB>(acc2.value());
     if (sub == 0) {
       const float2 bi = binfo[b];
       const float sc = fabsf(bi.x);
@@ -866,7 +872,7 @@ int launch_plain(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo,
                  int64_t frame_stride) {
   return dispatch_dtype(dtype, [&](auto tag) -> int {
     using T = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
-    hipLaunchKernelGGL(k_plain<T>, dim3(m.n_r, n_frames), dim3(256), 0, s,
+    hipLaunchKernelGGL((k_plain<T, kPlainLanes>), dim3(m.n_r, n_frames), dim3(256), 0, s,
                        c.row.as<int32_t>(), c.pix.as<int32_t>(), c.w.as<float>(), binfo,
                        m.d_area_inb.as<float>(), m.d_row_static.as<float4>(),
                        (const T *)frames, frame_stride, m.n_eta, m.n_r, m.nb(), mask_nan,
