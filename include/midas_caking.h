@@ -195,6 +195,13 @@ int mc_map_fuse(mc_map *const *maps, int32_t n_maps, const double *r_axes,
 int mc_map_reta(mc_map *m, const double *r_lo, const double *r_hi,
                 const double *eta_lo, const double *eta_hi, double px,
                 double lsd, double lambda, double *out5);
+/* The same rows as the GPU stream computes them for RTthEtaAreaMap.bin
+ * (initialize_PerFrameArr_Area_kernel, IntegratorFitPeaksGPUStream.cu
+ * :817-899, written at :2443-2462): the area row sums areaWeight over the
+ * entries whose pixel lies on the detector only. */
+int mc_map_reta_stream(mc_map *m, const double *r_lo, const double *r_hi,
+                       const double *eta_lo, const double *eta_hi, double px,
+                       double lsd, double lambda, double *out5);
 int mc_map_destroy(mc_map *m);
 
 /* Dark: mean of n_frames frames (FP64 on the GPU) times scale (the

@@ -123,6 +123,7 @@ SIGNATURES = {
     "mc_map_info_get": (_I, [_V, _P(MapInfo)]),
     "mc_map_download": (_I, [_V, _V, _V, _V]),
     "mc_map_reta": (_I, [_V, _PD, _PD, _PD, _PD, _D, _D, _D, _PD]),
+    "mc_map_reta_stream": (_I, [_V, _PD, _PD, _PD, _PD, _D, _D, _D, _PD]),
     "mc_map_destroy": (_I, [_V]),
     "mc_map_set_dark": (_I, [_V, _V, _I32, _I32, _D, _I32]),
     "mc_map_set_beam_center": (_I, [_V, _D, _D]),

@@ -190,12 +190,16 @@ class CSRGeometry:
         Path(directory).mkdir(parents=True, exist_ok=True)
         check(lib().mc_map_write(self.handle, str(directory).encode(), hdr))
 
-    def reta_map(self, params: Optional[IntegrationParams] = None, lam: float | None = None):
-        """REtaMap [5, n_r, n_eta]: R, 2theta, eta, sum areaWeight, Q."""
+    def reta_map(self, params: Optional[IntegrationParams] = None, lam: float | None = None,
+                 in_bounds: bool = False):
+        """REtaMap [5, n_r, n_eta]: R, 2theta, eta, sum areaWeight, Q.
+        in_bounds=True: the GPU stream's RTthEtaAreaMap rows (the area sums
+        only entries on the detector)."""
         P = params or self.params
         RLo, RHi, ELo, EHi = bin_edges(P)
         out = np.zeros((5, self.n_r, self.n_eta))
-        check(lib().mc_map_reta(self.handle, _dp(RLo), _dp(RHi), _dp(ELo), _dp(EHi), P.pxY,
+        fn = lib().mc_map_reta_stream if in_bounds else lib().mc_map_reta
+        check(fn(self.handle, _dp(RLo), _dp(RHi), _dp(ELo), _dp(EHi), P.pxY,
                                 P.Lsd, P.Lam if lam is None else lam,
                                 out.ctypes.data_as(C.POINTER(C.c_double))))
         return out

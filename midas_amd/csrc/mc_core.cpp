@@ -894,9 +894,11 @@ int mc_map_download(mc_map *m, void *records, int32_t *pairs, int32_t *flags) {
   return MC_OK;
 }
 
-int mc_map_reta(mc_map *m, const double *r_lo, const double *r_hi, const double *eta_lo,
-                const double *eta_hi, double px, double lsd, double lambda, double *out5) {
+static int reta_rows(mc_map *m, const double *r_lo, const double *r_hi, const double *eta_lo,
+                     const double *eta_hi, double px, double lsd, double lambda,
+                     const std::vector<double> &area, double *out5) {
   MC_CHECK(m && r_lo && r_hi && eta_lo && eta_hi && out5, MC_ERR_ARG, "null argument");
+  MC_CHECK((int64_t)area.size() == m->nb(), MC_ERR_STATE, "map has no per-bin areas");
   const int64_t nb = m->nb();
   for (int r = 0; r < m->n_r; r++) {
     const double Rm = (r_lo[r] + r_hi[r]) / 2;
@@ -906,11 +908,24 @@ int mc_map_reta(mc_map *m, const double *r_lo, const double *r_hi, const double 
       out5[b] = Rm;
       out5[nb + b] = (180.0 / M_PI) * tth;
       out5[2 * nb + b] = (eta_lo[e] + eta_hi[e]) / 2;
-      out5[3 * nb + b] = m->h_area_all[b];
+      out5[3 * nb + b] = area[b];
       out5[4 * nb + b] = lambda > 0 ? (4.0 * M_PI / lambda) * sin(tth / 2.0) : 0.0;
     }
   }
   return MC_OK;
+}
+
+int mc_map_reta(mc_map *m, const double *r_lo, const double *r_hi, const double *eta_lo,
+                const double *eta_hi, double px, double lsd, double lambda, double *out5) {
+  MC_CHECK(m, MC_ERR_ARG, "null map");
+  return reta_rows(m, r_lo, r_hi, eta_lo, eta_hi, px, lsd, lambda, m->h_area_all, out5);
+}
+
+int mc_map_reta_stream(mc_map *m, const double *r_lo, const double *r_hi,
+                       const double *eta_lo, const double *eta_hi, double px, double lsd,
+                       double lambda, double *out5) {
+  MC_CHECK(m, MC_ERR_ARG, "null map");
+  return reta_rows(m, r_lo, r_hi, eta_lo, eta_hi, px, lsd, lambda, m->h_area_inb, out5);
 }
 
 // Linear interpolation of the source R axis at r (fused_csr.py:64-99):

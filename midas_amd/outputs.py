@@ -69,7 +69,7 @@ class ResultFiles:
 
     def __init__(self, directory, data_name: str, params: IntegrationParams, *,
                  files=None, omega_sum_frames: Optional[int] = None, reta=None,
-                 source: Optional[str] = None):
+                 source: Optional[str] = None, reta_stream=None):
         P = params
         self.params = P
         RLo, RHi, ELo, EHi = bin_edges(P)
@@ -101,6 +101,13 @@ class ResultFiles:
             r_centers=dp(self.r_centers), eta_lo=dp(self._eta_lo), eta_hi=dp(self._eta_hi),
             instrument=dp(self._inst), source=self._src)
         Path(directory).mkdir(parents=True, exist_ok=True)
+        if reta_stream is not None and "stream" in names:
+            # RTthEtaAreaMap.bin of the GPU stream (.cu:2443-2462): the five
+            # [n_r * n_eta] float64 rows of reta_map(in_bounds=True)
+            rs = np.ascontiguousarray(reta_stream, dtype="<f8")
+            if rs.shape != (5, self.n_r, self.n_eta):
+                raise ValueError(f"reta_stream must be [5, {self.n_r}, {self.n_eta}]")
+            rs.tofile(Path(directory) / "RTthEtaAreaMap.bin")
         self.handle = C.c_void_p()
         check(lib().mc_out_open(str(directory).encode(), str(data_name).encode(), C.byref(cfg),
                                 C.byref(self.handle)))

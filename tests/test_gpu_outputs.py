@@ -40,3 +40,25 @@ def test_integrate_then_write(tmp_path):
         np.testing.assert_allclose(lb[i], ref_out.lineout_cpu_mean(cakes[i]), rtol=1e-14)
         # the GPU's FP32 CPU-mean row agrees with the FP64 one written
         np.testing.assert_allclose(lo[i, 0], lb[i], rtol=2e-5, atol=1e-3)
+
+
+def test_rtth_eta_area_map_of_the_stream(tmp_path, oracle_lib):
+    """RTthEtaAreaMap.bin (IntegratorFitPeaksGPUStream.cu:817-899, 2443-2462):
+    the REtaMap rows with the area summed over on-detector entries only
+    (the oracle's in-bounds area), five float64 rows of n_r * n_eta."""
+    from helpers import panel_shifts_for, to_oracle
+    from midas_amd.outputs import ResultFiles
+    P = small_cases()["panels"]
+    print("[outputs] RTthEtaAreaMap", flush=True)
+    g = ma.build_map(P, panel_shifts=panel_shifts_for("panels"))
+    om = oracle_lib.build_map(to_oracle(P), panel_shifts=panel_shifts_for("panels"))
+    full = g.reta_map(P)
+    inb = g.reta_map(P, in_bounds=True)
+    np.testing.assert_array_equal(inb[[0, 1, 2, 4]], full[[0, 1, 2, 4]])
+    want = oracle_lib.area_inbounds(om, P.NrPixelsY, P.NrPixelsZ)
+    np.testing.assert_allclose(inb[3], want, rtol=1e-6, atol=1e-9)
+    assert np.all(inb[3] <= full[3] + 1e-9)
+    with ResultFiles(tmp_path, "s.zip", P, files={"stream"}, reta_stream=inb):
+        pass
+    raw = np.fromfile(tmp_path / "RTthEtaAreaMap.bin", "<f8")
+    np.testing.assert_array_equal(raw, inb.reshape(-1))
