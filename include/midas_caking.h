@@ -348,6 +348,10 @@ int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype,
 #define MC_FIT_PV 0  /* height-normalised pV, 5 params [R, mu, G, Imax, BG] */
 #define MC_FIT_TCH 1 /* area-normalised TCH pV, 6 free params
                         [area, center, sig, gam, bg0, bg1] */
+#define MC_FIT_PV2 2 /* pV doublet, 8 params [R1, R2, mu, G1, I1, G2, I2, BG]
+                        (pf_fit_doublet_pv, PeakFit.c:984-1101); windows
+                        only; peak_guess holds 2 values per window (R1, R2);
+                        out: R1, R2, SNR1, SNR2, Gamma1, Gamma2, mu, status */
 #define MC_FIT_OUT 8 /* per window: 7 PF_PARAMS_PER_PEAK values + status */
 
 /* Fits n_windows windows of n_points samples each.  x, y are

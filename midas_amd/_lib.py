@@ -18,7 +18,7 @@ MC_DTYPE_U8, MC_DTYPE_U16, MC_DTYPE_U32, MC_DTYPE_I64 = 0, 1, 2, 3
 MC_DTYPE_F32, MC_DTYPE_F64, MC_DTYPE_HYBRID, MC_DTYPE_I32 = 4, 5, 6, 7
 MC_MODE_FLOOR, MC_MODE_BILINEAR, MC_MODE_GRADIENT = 0, 1, 2
 MC_F_NORMALIZE, MC_F_MASK_NAN = 1, 2
-MC_FIT_PV, MC_FIT_TCH, MC_FIT_OUT = 0, 1, 8
+MC_FIT_PV, MC_FIT_TCH, MC_FIT_PV2, MC_FIT_OUT = 0, 1, 2, 8
 
 MODES = {"floor": MC_MODE_FLOOR, "bilinear": MC_MODE_BILINEAR, "gradient": MC_MODE_GRADIENT}
 

@@ -11,6 +11,10 @@
 //        and mixing eta(sig, gam) by Thompson-Cox-Hastings; params
 //        [A, center, sig, gam, bg0, bg1], start and bounds of fitPeaks for
 //        one peak whose ROI is the window (:356-469).
+//   pV doublet (MC_FIT_PV2, :984-1101): two pV peaks sharing mu and BG,
+//        params [R1, R2, mu, Gamma1, Imax1, Gamma2, Imax2, BG], start and
+//        bounds of pf_fit_doublet_pv (:1037-1078): R1 in [x0, Rmid], R2 in
+//        [Rmid, x_last], Rmid the guesses' midpoint.
 // The objective is the reference's: the sum of squared residuals.  The
 // reference minimises it with NLopt (L-BFGS, Nelder-Mead fallback); this
 // kernel uses Levenberg-Marquardt with Marquardt damping and box bounds by
@@ -27,7 +31,9 @@
 namespace mc {
 namespace {
 
-constexpr int kMaxP = 6;
+template <int MODEL> constexpr int n_params() {
+  return MODEL == MC_FIT_PV ? 5 : MODEL == MC_FIT_TCH ? 6 : 8;
+}
 constexpr float kLn2 = 0.69314718055994530942f;
 
 struct Win {
@@ -59,6 +65,32 @@ __device__ __forceinline__ float pv_eval(const double *p, float xr, float cr, fl
     J[4] = 1.0f;
   }
   return bg + imax * (mu * L + (1.0f - mu) * G);
+}
+
+// pV doublet value and Jacobian (pf_pv_doublet_obj, PeakFit.c:984-1035).
+__device__ __forceinline__ float pv2_eval(const double *p, float xr, float c1, float c2,
+                                          float *J) {
+  const float mu = (float)p[2], g1 = (float)p[3], i1 = (float)p[4], g2 = (float)p[5],
+              i2 = (float)p[6], bg = (float)p[7];
+  const float C0 = 4.0f * kLn2;
+  const float d1 = xr - c1, d2 = xr - c2, d1s = d1 * d1, d2s = d2 * d2;
+  const float ig1 = 1.0f / (g1 * g1), ig2 = 1.0f / (g2 * g2);
+  const float den1 = 1.0f + 4.0f * d1s * ig1, den2 = 1.0f + 4.0f * d2s * ig2;
+  const float L1 = 1.0f / den1, L2 = 1.0f / den2;
+  const float G1 = __expf(-C0 * d1s * ig1), G2 = __expf(-C0 * d2s * ig2);
+  if (J) {
+    J[0] = i1 * (mu * 8.0f * d1 * ig1 / (den1 * den1) + (1.0f - mu) * G1 * 2.0f * C0 * d1 * ig1);
+    J[1] = i2 * (mu * 8.0f * d2 * ig2 / (den2 * den2) + (1.0f - mu) * G2 * 2.0f * C0 * d2 * ig2);
+    J[2] = i1 * (L1 - G1) + i2 * (L2 - G2);
+    J[3] = i1 * (mu * 8.0f * d1s * ig1 / (g1 * den1 * den1) +
+                 (1.0f - mu) * G1 * 2.0f * C0 * d1s * ig1 / g1);
+    J[4] = mu * L1 + (1.0f - mu) * G1;
+    J[5] = i2 * (mu * 8.0f * d2s * ig2 / (g2 * den2 * den2) +
+                 (1.0f - mu) * G2 * 2.0f * C0 * d2s * ig2 / g2);
+    J[6] = mu * L2 + (1.0f - mu) * G2;
+    J[7] = 1.0f;
+  }
+  return bg + i1 * (mu * L1 + (1.0f - mu) * G1) + i2 * (mu * L2 + (1.0f - mu) * G2);
 }
 
 // TCH width/mixing with derivatives (PeakFit.c:17-42).
@@ -131,8 +163,9 @@ __device__ __forceinline__ float tch_eval(const double *p, const Tch &t, float x
 // Sum of squared residuals and (optionally) the FP64 normal equations.
 template <int MODEL>
 __device__ double chi2_normal(const Win &w, const double *p, double *JtJ, double *Jtr) {
-  constexpr int NP = MODEL == MC_FIT_PV ? 5 : 6;
-  const float cr = (float)(p[MODEL == MC_FIT_PV ? 0 : 1] - w.x0);
+  constexpr int NP = n_params<MODEL>();
+  const float cr = (float)(p[MODEL == MC_FIT_TCH ? 1 : 0] - w.x0);
+  const float cr2 = MODEL == MC_FIT_PV2 ? (float)(p[1] - w.x0) : 0.0f;
   const float xrange = fmaxf(w.xr[w.n - 1] - w.xr[0], 1e-12f);
   Tch t;
   if (MODEL == MC_FIT_TCH) t = tch(p[2], p[3]);
@@ -145,9 +178,10 @@ __device__ double chi2_normal(const Win &w, const double *p, double *JtJ, double
     const float yi = w.y[i];
     if (isnan(yi)) continue;  // masked cake bins carry no information
     const float xr = w.xr[i];
-    float J[kMaxP];
+    float J[NP];
     float f;
     if (MODEL == MC_FIT_PV) f = pv_eval(p, xr, cr, JtJ ? J : nullptr);
+    else if (MODEL == MC_FIT_PV2) f = pv2_eval(p, xr, cr, cr2, JtJ ? J : nullptr);
     else f = tch_eval(p, t, xr, cr, 2.0f * (xr - w.xr[0]) / xrange - 1.0f, JtJ ? J : nullptr);
     const float r = yi - f;
     chi += (double)r * r;
@@ -191,9 +225,27 @@ template <int NP> __device__ bool cholesky_solve(double *A, double *b) {
 
 // Start values and bounds (see the file comment).
 template <int MODEL>
-__device__ void init_params(const Win &w, double rstep, double rmean, double *p, double *lo,
-                            double *hi) {
+__device__ void init_params(const Win &w, double rstep, double rmean, double rmean2, double *p,
+                            double *lo, double *hi) {
   const int n = w.n;
+  if (MODEL == MC_FIT_PV2) {  // pf_fit_doublet_pv, PeakFit.c:1046-1066
+    double bg = ((double)w.y[0] + w.y[n - 1]) / 2;
+    if (bg < 0) bg = 0;
+    double mx = -1e20;
+    for (int i = 0; i < n; i++) mx = fmax(mx, (double)w.y[i]);
+    double ig = mx - bg;
+    if (ig < 1e-6) ig = 1e-6;
+    const double mid = (rmean + rmean2) / 2;
+    p[0] = rmean; lo[0] = w.xa(0); hi[0] = mid;
+    p[1] = rmean2; lo[1] = mid; hi[1] = w.xa(n - 1);
+    p[2] = 0.5; lo[2] = 0; hi[2] = 1;
+    p[3] = rstep * 3; lo[3] = rstep / 2; hi[3] = rstep * n / 2;
+    p[4] = ig / 2; lo[4] = ig / 100; hi[4] = mx * 1.5;
+    p[5] = rstep * 3; lo[5] = rstep / 2; hi[5] = rstep * n / 2;
+    p[6] = ig / 2; lo[6] = ig / 100; hi[6] = mx * 1.5;
+    p[7] = bg; lo[7] = 0; hi[7] = (bg > 0) ? bg * 1.5 : mx * 0.5;
+    return;
+  }
   if (MODEL == MC_FIT_PV) {
     double bg = ((double)w.y[0] + w.y[n - 1]) / 2;
     if (bg < 0) bg = 0;
@@ -253,9 +305,10 @@ __device__ void init_params(const Win &w, double rstep, double rmean, double *p,
 }
 
 template <int MODEL>
-__device__ void fit_one(const Win &w, double rstep, double rmean, int max_iter, double *out) {
-  constexpr int NP = MODEL == MC_FIT_PV ? 5 : 6;
-  double p[kMaxP], lo[kMaxP], hi[kMaxP];
+__device__ void fit_one(const Win &w, double rstep, double rmean, double rmean2, int max_iter,
+                        double *out) {
+  constexpr int NP = n_params<MODEL>();
+  double p[NP], lo[NP], hi[NP];
   for (int k = 0; k < 8; k++) out[k] = 0;
   int valid = 0;
   for (int i = 0; i < w.n; i++) valid += !isnan(w.y[i]);
@@ -263,8 +316,8 @@ __device__ void fit_one(const Win &w, double rstep, double rmean, int max_iter, 
     out[7] = -1;
     return;
   }
-  init_params<MODEL>(w, rstep, rmean, p, lo, hi);
-  double JtJ[kMaxP * kMaxP], Jtr[kMaxP], M[kMaxP * kMaxP], d[kMaxP], q[kMaxP];
+  init_params<MODEL>(w, rstep, rmean, rmean2, p, lo, hi);
+  double JtJ[NP * NP], Jtr[NP], M[NP * NP], d[NP], q[NP];
   double chi = chi2_normal<MODEL>(w, p, JtJ, Jtr);
   double lam = 1e-3;
   int it = 0, stall = 0;
@@ -310,7 +363,16 @@ __device__ void fit_one(const Win &w, double rstep, double rmean, int max_iter, 
     return;
   }
   const int n_used = valid;
-  if (MODEL == MC_FIT_PV) {
+  if (MODEL == MC_FIT_PV2) {  // pf_fit_doublet_pv outputs, PeakFit.c:1093-1100
+    const double rms = sqrt(chi / n_used);
+    out[0] = p[0];
+    out[1] = p[1];
+    out[2] = rms > 0 ? p[4] / rms : 1.0;
+    out[3] = rms > 0 ? p[6] / rms : 1.0;
+    out[4] = p[3];
+    out[5] = p[5];
+    out[6] = p[2];
+  } else if (MODEL == MC_FIT_PV) {
     const double G = p[2], mu = p[1], imax = p[3];
     out[0] = imax * G * (mu * M_PI / 2 + (1 - mu) * sqrt(M_PI / (4 * log(2.0))));
     out[1] = p[0];
@@ -348,7 +410,11 @@ __global__ __launch_bounds__(64) void k_fit_windows(int nw, int npts, const floa
   w.y = y + (int64_t)i * npts;
   w.n = npts;
   w.x0 = x0;
-  fit_one<MODEL>(w, rstep, guess[i], max_iter, out + (int64_t)i * MC_FIT_OUT);
+  if (MODEL == MC_FIT_PV2)
+    fit_one<MODEL>(w, rstep, guess[2 * i], guess[2 * i + 1], max_iter,
+                   out + (int64_t)i * MC_FIT_OUT);
+  else
+    fit_one<MODEL>(w, rstep, guess[i], 0.0, max_iter, out + (int64_t)i * MC_FIT_OUT);
 }
 
 // Windows cut from cakes: one thread per (frame, eta bin, ring).  The
@@ -384,7 +450,7 @@ __global__ __launch_bounds__(64) void k_fit_rings(
   w.y = ys;
   w.n = n;
   w.x0 = x0s[k];
-  fit_one<MODEL>(w, rstep, ring[k], max_iter, o);
+  fit_one<MODEL>(w, rstep, ring[k], 0.0, max_iter, o);
 }
 
 __global__ void k_nonzero(const float *__restrict__ cakes, int n_frames, int n_r, int n_eta,
@@ -407,7 +473,8 @@ extern "C" int mc_fit_windows(mc_ctx *ctx, int32_t model, int32_t n_windows, int
                               const float *x, const float *y, const float *peak_guess,
                               double rstep, int32_t max_iter, double *out, int32_t on_device) {
   MC_CHECK(ctx && x && y && peak_guess && out, MC_ERR_ARG, "null argument");
-  MC_CHECK(model == MC_FIT_PV || model == MC_FIT_TCH, MC_ERR_ARG, "model %d", model);
+  MC_CHECK(model == MC_FIT_PV || model == MC_FIT_TCH || model == MC_FIT_PV2, MC_ERR_ARG,
+           "model %d", model);
   MC_CHECK(n_windows >= 0 && n_points >= 2 && n_points <= 4096 && max_iter >= 1, MC_ERR_ARG,
            "bad sizes n_windows=%d n_points=%d", n_windows, n_points);
   if (n_windows == 0) return MC_OK;
@@ -415,17 +482,18 @@ extern "C" int mc_fit_windows(mc_ctx *ctx, int32_t model, int32_t n_windows, int
   MC_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   const size_t wb = (size_t)n_windows * n_points * 4, ob = (size_t)n_windows * MC_FIT_OUT * 8;
+  const size_t gb = (size_t)n_windows * (model == MC_FIT_PV2 ? 2 : 1) * 4;  // guesses
   DBuf dx, dy, dg, dout, dxr;
   const float *px = x, *py = y, *pg = peak_guess;
   double *po = out;
   int rc;
   if (!on_device) {
-    if ((rc = dx.alloc(wb)) || (rc = dy.alloc(wb)) || (rc = dg.alloc(n_windows * 4)) ||
+    if ((rc = dx.alloc(wb)) || (rc = dy.alloc(wb)) || (rc = dg.alloc(gb)) ||
         (rc = dout.alloc(ob)))
       return rc;
     MC_HIP(hipMemcpyAsync(dx.p, x, wb, hipMemcpyHostToDevice, s));
     MC_HIP(hipMemcpyAsync(dy.p, y, wb, hipMemcpyHostToDevice, s));
-    MC_HIP(hipMemcpyAsync(dg.p, peak_guess, n_windows * 4, hipMemcpyHostToDevice, s));
+    MC_HIP(hipMemcpyAsync(dg.p, peak_guess, gb, hipMemcpyHostToDevice, s));
     px = dx.as<float>();
     py = dy.as<float>();
     pg = dg.as<float>();
@@ -435,6 +503,9 @@ extern "C" int mc_fit_windows(mc_ctx *ctx, int32_t model, int32_t n_windows, int
   const unsigned blocks = (unsigned)((n_windows + 63) / 64);
   if (model == MC_FIT_PV)
     hipLaunchKernelGGL(k_fit_windows<MC_FIT_PV>, dim3(blocks), dim3(64), 0, s, n_windows,
+                       n_points, px, py, pg, rstep, max_iter, dxr.as<float>(), po);
+  else if (model == MC_FIT_PV2)
+    hipLaunchKernelGGL(k_fit_windows<MC_FIT_PV2>, dim3(blocks), dim3(64), 0, s, n_windows,
                        n_points, px, py, pg, rstep, max_iter, dxr.as<float>(), po);
   else
     hipLaunchKernelGGL(k_fit_windows<MC_FIT_TCH>, dim3(blocks), dim3(64), 0, s, n_windows,

@@ -15,20 +15,24 @@ from ._lib import check, lib
 from .caking import CSRGeometry, Context, _is_cuda, _torch, default_context
 
 PF_PARAMS_PER_PEAK = 7
-MODELS = {"pv": _lib.MC_FIT_PV, "tch": _lib.MC_FIT_TCH}
+MODELS = {"pv": _lib.MC_FIT_PV, "tch": _lib.MC_FIT_TCH, "pv2": _lib.MC_FIT_PV2}
 
 
 def fit_windows(x, y, peak_guess, rstep: float, *, model: str = "pv", max_iter: int = 100,
                 ctx: Context | None = None) -> np.ndarray:
     """Fits W windows of N samples each ([W, N] arrays) in one launch.
     Returns [W, 8]: area, center, sig, gam, FWHM, eta, chi2 (SNR for pV),
-    status (iterations; negative when the window could not be fitted)."""
+    status (iterations; negative when the window could not be fitted).
+    model "pv2" (pV doublet, PeakFit.c:984-1101) takes peak_guess [W, 2]
+    (the two centres) and returns R1, R2, SNR1, SNR2, Gamma1, Gamma2, mu,
+    status."""
     ctx = ctx or default_context()
     x = np.ascontiguousarray(x, dtype=np.float32)
     y = np.ascontiguousarray(y, dtype=np.float32)
     g = np.ascontiguousarray(peak_guess, dtype=np.float32).reshape(-1)
-    if x.shape != y.shape or x.ndim != 2 or g.size != x.shape[0]:
-        raise ValueError("x, y must be [W, N] and peak_guess [W]")
+    per = 2 if model == "pv2" else 1
+    if x.shape != y.shape or x.ndim != 2 or g.size != per * x.shape[0]:
+        raise ValueError(f"x, y must be [W, N] and peak_guess [W{', 2' if per == 2 else ''}]")
     out = np.zeros((x.shape[0], _lib.MC_FIT_OUT), dtype=np.float64)
     check(lib().mc_fit_windows(ctx.handle, MODELS[model], x.shape[0], x.shape[1],
                                x.ctypes.data, y.ctypes.data, g.ctypes.data, rstep, max_iter,
@@ -237,6 +241,24 @@ def fit_peaks_auto(x, intensity, max_peaks: int, x_bin_size: float,
     for k, i in enumerate(keep):
         out[k] = rows[i]
     return out, len(keep)
+
+
+def doublet_pairs(peak_locations, separation: float) -> list:
+    """Pairs of adjacent peak locations closer than `separation`, each peak
+    in at most one pair, scanned left to right (IntegratorZarrOMP.c
+    :1912-1923); fit each pair with ``fit_windows(model="pv2")``."""
+    locs = [float(v) for v in peak_locations]
+    taken = [False] * len(locs)
+    pairs = []
+    if separation <= 0.0:
+        return pairs
+    for i in range(len(locs) - 1):
+        if taken[i]:
+            continue
+        if abs(locs[i + 1] - locs[i]) < separation:
+            taken[i] = taken[i + 1] = True
+            pairs.append((i, i + 1))
+    return pairs
 
 
 def pseudo_voigt(x, center, mu, fwhm, imax, bg):

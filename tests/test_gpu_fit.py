@@ -150,3 +150,31 @@ def test_stream_fitter_batches_frames(tmp_path):
         g.submit(0, frames[0])
     assert g.locations == [55.5] and g.fitted[0].shape == (1, 7)
     assert abs(g.fitted[0][0, 1] - 55.5) < 0.05
+
+
+def test_pv_doublet_recovers_both_centres():
+    """pf_fit_doublet_pv (PeakFit.c:984-1101): two pV peaks 1.2-2 px apart
+    sharing mu and background, noise-free, recovered from starting centres
+    0.3 px off; SNR and widths come back with them."""
+    print("[fit] pV doublet", flush=True)
+    rng = np.random.default_rng(9)
+    W, N, step = 64, 41, 0.25
+    x = 100.0 + step * np.arange(N)[None, :] + rng.uniform(0, 50, (W, 1)).round(2)
+    c1 = x[:, 15] + rng.uniform(-0.1, 0.1, W)
+    c2 = c1 + rng.uniform(1.2, 2.0, W)
+    mu, g1, g2 = 0.4, 0.9, 1.1
+    i1 = rng.uniform(200, 400, W)
+    i2 = rng.uniform(200, 400, W)
+    y = (20.0 + peakfit.pseudo_voigt(x, c1[:, None], mu, g1, i1[:, None], 0.0)
+         + peakfit.pseudo_voigt(x, c2[:, None], mu, g2, i2[:, None], 0.0))
+    guess = np.stack([c1 - 0.3, c2 + 0.3], axis=1)
+    out = peakfit.fit_windows(x, y, guess, step, model="pv2", max_iter=200)
+    assert np.all(out[:, 7] > 0)
+    np.testing.assert_allclose(out[:, 0], c1, atol=2e-3)
+    np.testing.assert_allclose(out[:, 1], c2, atol=2e-3)
+    np.testing.assert_allclose(out[:, 4], g1, rtol=2e-2)
+    np.testing.assert_allclose(out[:, 5], g2, rtol=2e-2)
+    np.testing.assert_allclose(out[:, 6], mu, atol=2e-2)
+    assert np.all(out[:, 2] > 100) and np.all(out[:, 3] > 100)  # noise-free: large SNR
+    with pytest.raises(ValueError):
+        peakfit.fit_windows(x, y, c1, step, model="pv2")

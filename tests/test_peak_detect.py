@@ -85,3 +85,11 @@ def test_product_matches_oracle(seed):
     rows = rng.normal(size=(12, 7))
     rows[:, 4] = np.abs(rows[:, 4])
     assert pf.select_auto_fits(rows, 4, 0.25) == od.select_fits(rows.tolist(), 4, 0.25)
+
+
+def test_doublet_pairs_scan():
+    """IntegratorZarrOMP.c:1912-1923: adjacent locations under the
+    separation pair up left to right, a peak joins at most one pair."""
+    assert pf.doublet_pairs([10.0, 11.0, 11.5, 20.0, 20.5], 1.2) == [(0, 1), (3, 4)]
+    assert pf.doublet_pairs([10.0, 11.0], 0.0) == []
+    assert pf.doublet_pairs([10.0, 12.0], 2.0) == []          # strict <
