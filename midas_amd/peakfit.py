@@ -41,10 +41,14 @@ def fit_windows(x, y, peak_guess, rstep: float, *, model: str = "pv", max_iter: 
 
 
 def fit_cake_rings(cakes, geom: CSRGeometry, r_centers, rings, *, pad: int = 20,
-                   model: str = "pv", max_iter: int = 100):
+                   model: str = "pv", max_iter: int = 100, doublet_separation: float = 0.0):
     """Per-eta fits of every ring in every cake, in one launch
     (IntegratorZarrOMP.c:1903-2070).  ``cakes`` is a CUDA tensor
-    [F, n_r, n_eta]; returns [F, n_eta, n_rings, 8] float64 (host)."""
+    [F, n_r, n_eta]; returns [F, n_eta, n_rings, 8] float64 (host).
+    With doublet_separation > 0 (pV only), adjacent rings closer than it
+    are fitted jointly as pV doublets on one ROI spanning both
+    (:1912-1923, 1960-2016): their rows then hold the centre [1], SNR [6]
+    and status [7] of the doublet fit, the other fields 0."""
     if not _is_cuda(cakes):
         raise TypeError("cakes must be a device tensor (the output of integrate_batch)")
     t = _torch()
@@ -58,7 +62,47 @@ def fit_cake_rings(cakes, geom: CSRGeometry, r_centers, rings, *, pad: int = 20,
                                   rc.ctypes.data_as(C.POINTER(C.c_double)),
                                   rg.ctypes.data_as(C.POINTER(C.c_double)), rg.size, pad,
                                   MODELS[model], max_iter, out.ctypes.data, 0))
+    pairs = doublet_pairs(rg, doublet_separation) if model == "pv" else []
+    if pairs:
+        _fit_ring_doublets(c, rc, rg, pairs, pad, max_iter, out, geom)
     return out
+
+
+def _fit_ring_doublets(c, rc, rg, pairs, pad, max_iter, out, geom):
+    """The doublet pass of the per-eta fits (IntegratorZarrOMP.c:1960-2016):
+    ROI [min - pad bins, max + pad bins] over both rings, skipped when under
+    6 bins or when the eta profile was skipped (status -3)."""
+    F, n_eta = out.shape[0], out.shape[1]
+    step = float(rc[1] - rc[0]) if rc.size > 1 else 1.0
+    host = None
+    for i, j in pairs:
+        lo_r, hi_r = (i, j) if rg[i] < rg[j] else (j, i)
+        plo, phi = rg[lo_r], rg[hi_r]
+        roi_lo, roi_hi = plo - pad * step, phi + pad * step
+        ge = np.nonzero(rc >= roi_lo)[0]
+        le = np.nonzero(rc <= roi_hi)[0]
+        out[:, :, i, :7] = 0.0
+        out[:, :, j, :7] = 0.0
+        out[:, :, i, 7] = out[:, :, j, 7] = -1.0
+        if ge.size == 0 or le.size == 0 or le[-1] - ge[0] < 5:
+            continue
+        lo, hi = int(ge[0]), int(le[-1])
+        if host is None:
+            host = c.float().cpu().numpy()
+        ok = out[:, :, lo_r, 7] != -3  # eta profiles the singlet pass kept
+        ok &= out[:, :, hi_r, 7] != -3
+        fi, ei = np.nonzero(ok)
+        if fi.size == 0:
+            continue
+        Y = host[fi, lo:hi + 1, ei]                       # [W, roiN]
+        X = np.broadcast_to(rc[lo:hi + 1], Y.shape)
+        G = np.tile([plo, phi], (fi.size, 1))
+        res = fit_windows(X, Y, G, step, model="pv2", max_iter=max_iter, ctx=geom.ctx)
+        good = res[:, 7] > 0
+        for ring, col_r, col_snr in ((lo_r, 0, 2), (hi_r, 1, 3)):
+            out[fi, ei, ring, 1] = np.where(good, res[:, col_r], 0.0)
+            out[fi, ei, ring, 6] = np.where(good, res[:, col_snr], 0.0)
+            out[fi, ei, ring, 7] = res[:, 7]
 
 
 def fit_peaks(x, intensity, peak_locations, x_bin_size: float, fit_roi_padding: int = 20,

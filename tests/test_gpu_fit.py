@@ -178,3 +178,34 @@ def test_pv_doublet_recovers_both_centres():
     assert np.all(out[:, 2] > 100) and np.all(out[:, 3] > 100)  # noise-free: large SNR
     with pytest.raises(ValueError):
         peakfit.fit_windows(x, y, c1, step, model="pv2")
+
+
+def test_fit_cake_rings_with_a_doublet():
+    """Per-eta doublet pass (IntegratorZarrOMP.c:1912-2016): two rings
+    1.5 px apart in synthetic cakes, fitted jointly, come back on their own
+    centres; a ring far from both stays a singlet fit."""
+    torch = pytest.importorskip("torch")
+    print("[fit] cake rings with a doublet", flush=True)
+    P = synth.plain_1024()
+    g = ma.build_map(P)
+    RLo, RHi, _, _ = ma.bin_edges(P)
+    rc = (RLo + RHi) / 2
+    nr, ne = P.n_r_bins, P.n_eta_bins
+    step = float(rc[1] - rc[0])
+    r1, r2, r3 = rc[nr // 3] + 0.1, rc[nr // 3] + 0.1 + 6 * step, rc[2 * nr // 3]
+    prof = 30.0 + peakfit.pseudo_voigt(rc, r1, 0.3, 3 * step, 400.0, 0.0) \
+        + peakfit.pseudo_voigt(rc, r2, 0.3, 3 * step, 300.0, 0.0) \
+        + peakfit.pseudo_voigt(rc, r3, 0.3, 3 * step, 500.0, 0.0)
+    cakes = np.repeat(prof[None, :, None], ne, axis=2).repeat(2, axis=0).astype(np.float32)
+    dev = torch.from_numpy(cakes).cuda()
+    rings = [r1, r2, r3]
+    single = peakfit.fit_cake_rings(dev, g, rc, rings, pad=20, model="pv")
+    both = peakfit.fit_cake_rings(dev, g, rc, rings, pad=20, model="pv",
+                                  doublet_separation=8 * step)
+    np.testing.assert_array_equal(both[..., 2, :], single[..., 2, :])
+    assert np.all(both[..., :2, 7] > 0)
+    np.testing.assert_allclose(both[..., 0, 1], r1, atol=2e-3)
+    np.testing.assert_allclose(both[..., 1, 1], r2, atol=2e-3)
+    assert np.all(both[..., :2, 6] > 1.0)
+    # the singlet fits of the close pair are pulled by the neighbour
+    assert np.abs(single[..., 0, 1] - r1).max() > np.abs(both[..., 0, 1] - r1).max()
