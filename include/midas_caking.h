@@ -252,8 +252,8 @@ int mc_integrate(mc_map *m, const void *frames, int32_t dtype,
  * graph's tables before each replay; a new tuning leaves the graph on the
  * layout it captured; a new beam centre in MC_MODE_GRADIENT, or a frame
  * type needing a larger batch buffer, makes mc_graph_launch fail with
- * MC_ERR_STATE (create a new graph).  The graph must be destroyed before
- * its map.  The reference has no counterpart:
+ * MC_ERR_STATE (create a new graph).  Launch the graph only while its map
+ * lives; mc_graph_destroy needs only the map's context.  The reference has no counterpart:
  * the CUDA path launches kernel by kernel. */
 typedef struct mc_graph mc_graph;
 int mc_graph_create(mc_map *m, const void *frames_dev, int32_t dtype, int32_t n_frames,

@@ -1339,6 +1339,7 @@ int mc_profile(mc_map *m, const float *cakes, int32_t n_frames, int32_t cakes_on
 // ---- the batched passes of one call, captured in a HIP graph ----
 struct mc_graph {
   mc_map *m = nullptr;
+  Ctx *ctx = nullptr;         // outlives the map (destroy needs only the stream)
   int mode = 0, dtype = 0;
   uint32_t flags = 0;
   uint64_t tiles_id = 0;      // the tile layout the graph reads
@@ -1372,6 +1373,7 @@ int mc_graph_create(mc_map *m, const void *frames_dev, int32_t dtype, int32_t n_
   auto *g = new mc_graph();
   std::unique_ptr<mc_graph> guard(g);
   g->m = m;
+  g->ctx = m->ctx;
   g->mode = mode;
   g->dtype = dtype;
   g->flags = flags;
@@ -1415,8 +1417,8 @@ int mc_graph_launch(mc_graph *g) {
 int mc_graph_destroy(mc_graph *g) {
   if (!g) return MC_OK;
   {
-    std::lock_guard<std::recursive_mutex> lk(g->m->ctx->mu);
-    (void)hipStreamSynchronize(g->m->ctx->stream);
+    std::lock_guard<std::recursive_mutex> lk(g->ctx->mu);
+    (void)hipStreamSynchronize(g->ctx->stream);
   }
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
