@@ -482,6 +482,13 @@ int mc_server_submit_buffer(mc_server *s, int32_t geometry, void *buf,
                             int64_t request_id, int32_t timeout_ms);
 int mc_server_release_buffer(mc_server *s, int32_t geometry, void *buf);
 int mc_server_drain(mc_server *s, int32_t timeout_ms);
+/* Batching window (0 = none, the default): a free slot waits up to
+ * window_us after the oldest queued request arrived for max_batch frames of
+ * its geometry before launching what is queued; mc_server_drain closes the
+ * window.  No counterpart in the reference, which launches one frame per
+ * request (IntegratorFitPeaksGPUStream.cu:2567-2883): it trades latency for
+ * larger batches under load. */
+int mc_server_set_batch_window(mc_server *s, int64_t window_us);
 int mc_server_stats_get(mc_server *s, mc_server_stats *st);
 int mc_server_destroy(mc_server *s);
 

@@ -154,6 +154,7 @@ SIGNATURES = {
     "mc_server_submit_buffer": (_I, [_V, _I32, _V, _I64, _I32]),
     "mc_server_release_buffer": (_I, [_V, _I32, _V]),
     "mc_server_drain": (_I, [_V, _I32]),
+    "mc_server_set_batch_window": (_I, [_V, _I64]),
     "mc_server_stats_get": (_I, [_V, _P(ServerStats)]),
     "mc_server_destroy": (_I, [_V]),
     "mc_out_open": (_I, [C.c_char_p, C.c_char_p, _P(OutConfig), _P(_V)]),

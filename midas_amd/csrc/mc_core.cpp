@@ -259,7 +259,10 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   // Pixels a tile can stage: what the LDS budget holds after the records,
   // the entries, the zero row and the two pixel lists (4 bytes a pixel,
   // rounded up to 16 bytes).
-  const int64_t cap_px = (m->lds_bytes - fixed - RB - 24) / (RB + 8);
+  int64_t cap_px = (m->lds_bytes - fixed - RB - 24) / (RB + 8);
+  // uint16 entries carry the row's byte offset in 16 bits (the high half
+  // holds the pixel's dark): the zero row, at npx * RB, must stay below 2^16.
+  if (in_size == 2) cap_px = std::min<int64_t>(cap_px, 65535 / RB - 1);
   MC_CHECK(cap_px >= 16, MC_ERR_ARG, "LDS budget %d too small", m->lds_bytes);
   // Per-bin footprints: the sorted distinct pixels of its entries; a bin
   // whose footprint or entries do not fit a tile alone is direct.
@@ -497,7 +500,6 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   if ((rc = t->chunks.alloc(chunks.size() * 4)) || (rc = t->tiles.alloc(tiles.size() * 4)) ||
       (rc = t->tbins.alloc(tbins.size() * 16)) || (rc = t->ents.alloc(ents.size() * 8)) ||
       (rc = t->band_chunk.alloc(band_chunk.size() * 4)) || (rc = t->xcd.alloc(9 * 4)) ||
-      (rc = t->part.alloc((size_t)t->n_chunks * H * 4 * kXtFrames * 4)) ||
       (rc = t->tpix.alloc(std::max<size_t>(tpix.size(), 1) * 4)))
     return rc;
   if (!tpix.empty())
@@ -546,7 +548,8 @@ int ensure_binfo(Map *m, int mode, uint32_t flags, const float2 **out) {
   const int key = mode * 4 + (int)flags;
   if (!c.dterm_valid) {
     if ((rc = launch_dark_term(s, c.row.as<int32_t>(), c.pix.as<int32_t>(), c.w.as<float>(),
-                               m->has_dark ? m->d_dark.as<double>() : nullptr, m->nb(),
+                               m->has_dark ? m->d_dark.as<double>() : nullptr,
+                               m->has_dark ? m->d_dint.as<uint16_t>() : nullptr, m->nb(),
                                c.dterm.as<float>())))
       return rc;
     c.dterm_valid = true;
@@ -568,15 +571,22 @@ int ensure_binfo(Map *m, int mode, uint32_t flags, const float2 **out) {
 }
 
 // Tile records of t for the binfo table of (mode, flags); rebuilt when the
-// table was (a new dark, other flags).  Synchronous like the table itself:
-// the records are read by kernels on other streams (the frame server).
+// table was (a new dark, other flags).  The uint16 layout's staged entries
+// carry the pixel darks of the current dark.  Synchronous like the table
+// itself: records and entries are read by kernels on other streams (the
+// frame server).
 static int ensure_recs(Map *m, hipStream_t s, Tiles *t, int mode, uint32_t flags,
                        const float2 *binfo) {
   const int key = mode * 4 + (int)flags;
+  int rc;
+  if (t->in_size == 2 && t->ents_dark != m->dark_version) {
+    if ((rc = launch_ents_dark(s, *m, *t))) return rc;
+    MC_HIP(hipStreamSynchronize(s));
+    t->ents_dark = m->dark_version;
+  }
   if (t->recs.p && t->recs_binfo == binfo && t->recs_version == m->binfo_version[key])
     return MC_OK;
-  int rc = launch_tile_recs(s, *m, *t, binfo);
-  if (rc) return rc;
+  if ((rc = launch_tile_recs(s, *m, *t, binfo))) return rc;
   MC_HIP(hipStreamSynchronize(s));
   t->recs_binfo = binfo;
   t->recs_version = m->binfo_version[key];
@@ -594,6 +604,36 @@ static int ensure_xt(DBuf &xt, const Map &m, int dtype) {
   return xt.bytes >= need ? MC_OK : xt.alloc(need);
 }
 
+// Scratch of the one-frame kernels with a dark: up to 8 frames of FP32
+// dark-subtracted pixels (a larger buffer, e.g. a batched pass's, holds
+// more).  Frames per chunk in *frames_out.
+static int ensure_sub(DBuf &xt, const Map &m, int n_frames, int64_t *frames_out) {
+  const size_t per = (size_t)m.npx() * sizeof(float);
+  *frames_out = 0;
+  if (!m.has_dark) return MC_OK;
+  const size_t want = per * (size_t)std::min(n_frames, 8);
+  if (xt.bytes < want) {
+    int rc = xt.alloc(want);
+    if (rc) return rc;
+  }
+  *frames_out = (int64_t)(xt.bytes / per);
+  return MC_OK;
+}
+
+static int ensure_part(DBuf &part, const Tiles &t) {
+  const size_t need = t.part_bytes();
+  return part.bytes >= need ? MC_OK : part.alloc(need);
+}
+
+int ensure_work(Map *m, int mode, int dtype, Work *work) {
+  Tiles *t = nullptr;
+  int rc;
+  if ((rc = ensure_tiles(m, mode, batch_elem(dtype), &t)) ||
+      (rc = ensure_xt(work->xt, *m, dtype)) || (rc = ensure_part(work->part, *t)))
+    return rc;
+  return MC_OK;
+}
+
 int prepare_batched(Map *m, int mode, uint32_t flags, int dtype) {
   const float2 *binfo = nullptr;
   Tiles *t = nullptr;
@@ -606,29 +646,32 @@ int prepare_batched(Map *m, int mode, uint32_t flags, int dtype) {
 
 int run_integrate(Map *m, hipStream_t s, const void *frames_dev, int dtype, int n_frames,
                   int mode, uint32_t flags, float *cakes_dev, float *lineouts_dev,
-                  bool force_plain, bool force_batched, DBuf *xt) {
+                  bool force_plain, bool force_batched, Work *work) {
   int rc;
   const float2 *binfo = nullptr;
   if ((rc = ensure_binfo(m, mode, flags, &binfo))) return rc;
   const int mask_nan = (flags & MC_F_MASK_NAN) ? 1 : 0;
   const size_t es = dtype_size(dtype);
   MC_CHECK(es > 0, MC_ERR_ARG, "unsupported dtype %d", dtype);
+  Work &ws = work ? *work : m->work;
   if ((n_frames == 1 && !force_batched) || force_plain) {
+    int64_t sub_frames = 0;
+    if ((rc = ensure_sub(ws.xt, *m, n_frames, &sub_frames))) return rc;
     return launch_plain(s, *m, m->csr[mode], binfo, mask_nan, frames_dev, dtype, cakes_dev,
-                        lineouts_dev, n_frames, m->npx());
+                        lineouts_dev, n_frames, m->npx(), ws.xt.as<float>(), sub_frames);
   }
   Tiles *t = nullptr;
   if ((rc = ensure_tiles(m, mode, batch_elem(dtype), &t))) return rc;
   if ((rc = ensure_recs(m, s, t, mode, flags, binfo))) return rc;
-  DBuf &ws = xt ? *xt : m->xt;
-  if ((rc = ensure_xt(ws, *m, dtype))) return rc;
+  if ((rc = ensure_xt(ws.xt, *m, dtype)) || (rc = ensure_part(ws.part, *t))) return rc;
   const int step = pass_frames(*t);
   for (int f0 = 0; f0 < n_frames; f0 += step) {
     const int nf = std::min(step, n_frames - f0);
     const char *fb = (const char *)frames_dev + (size_t)f0 * m->npx() * es;
     float *cb = cakes_dev + (size_t)f0 * m->nb();
     float *lb = lineouts_dev ? lineouts_dev + (size_t)f0 * 3 * m->n_r : nullptr;
-    if ((rc = launch_batched(s, *m, *t, binfo, mask_nan, fb, dtype, nf, cb, lb, ws.p)))
+    if ((rc = launch_batched(s, *m, *t, binfo, mask_nan, fb, dtype, nf, cb, lb, ws.xt.p,
+                             ws.part.as<float>())))
       return rc;
   }
   return MC_OK;
@@ -870,15 +913,16 @@ int mc_map_info_get(mc_map *m, mc_map_info *info) {
   info->n_flagged = nf;
   info->q_mode = m->q_mode;
   int64_t hb = m->d_rec.bytes + m->d_pairs.bytes + m->d_flags.bytes + m->d_area_inb.bytes +
-               m->d_row_static.bytes + m->inv_area.bytes + m->d_dark.bytes + m->xt.bytes;
+               m->d_row_static.bytes + m->inv_area.bytes + m->d_dark.bytes +
+               m->d_dint.bytes + m->work.xt.bytes + m->work.part.bytes;
   for (const DBuf &b : m->d_binfo) hb += b.bytes;
   for (int q = 0; q < 3; q++) {
     const Csr &c = m->csr[q];
-    hb += c.row.bytes + c.pix.bytes + c.w.bytes + c.dterm.bytes;
+    hb += c.row.bytes + c.pix.bytes + c.w.bytes + c.dterm.bytes + c.d2term.bytes;
     for (auto &t : m->tiles[q])
       hb += t->chunks.bytes + t->tiles.bytes + t->tbins.bytes + t->ents.bytes +
-            t->band_chunk.bytes + t->part.bytes + t->recs.bytes + t->tpix.bytes +
-            t->blocks64.bytes + t->blocks128.bytes;
+            t->band_chunk.bytes + t->recs.bytes + t->tpix.bytes + t->blocks64.bytes +
+            t->blocks128.bytes;
   }
   info->hbm_bytes = hb;
   for (int q = 0; q < 3; q++) info->skipped[q] = m->skipped[q];
@@ -1130,6 +1174,7 @@ int mc_map_set_dark(mc_map *m, const void *dark, int32_t dtype, int32_t n_frames
   MC_HIP(hipSetDevice(m->ctx->device));
   for (auto &c : m->csr) c.dterm_valid = c.d2_valid = false;
   for (bool &ok : m->binfo_ok) ok = false;
+  m->dark_version++;
   if (!dark) {
     m->has_dark = false;
     return MC_OK;
@@ -1138,7 +1183,8 @@ int mc_map_set_dark(mc_map *m, const void *dark, int32_t dtype, int32_t n_frames
   const size_t es = dtype_size(dtype);
   MC_CHECK(es > 0, MC_ERR_ARG, "unsupported dark dtype %d", dtype);
   int rc;
-  if ((rc = m->d_dark.alloc(m->npx() * 8))) return rc;
+  m->has_dark = false;
+  if ((rc = m->d_dark.alloc(m->npx() * 8)) || (rc = m->d_dint.alloc(m->npx() * 2))) return rc;
   const void *src = dark;
   DBuf tmp;
   const size_t bytes = (size_t)n_frames * m->npx() * es;
@@ -1148,7 +1194,9 @@ int mc_map_set_dark(mc_map *m, const void *dark, int32_t dtype, int32_t n_frames
     src = tmp.p;
   }
   if ((rc = launch_dark_mean(m->ctx->stream, src, dtype, n_frames, m->npx(), scale,
-                             m->d_dark.as<double>())))
+                             m->d_dark.as<double>())) ||
+      (rc = launch_dark_int(m->ctx->stream, m->d_dark.as<double>(), m->npx(),
+                            m->d_dint.as<uint16_t>())))
     return rc;
   MC_HIP(hipStreamSynchronize(m->ctx->stream));
   m->has_dark = true;
@@ -1271,7 +1319,8 @@ int mc_integrate_variance(mc_map *m, const void *frames, int32_t dtype, int32_t 
     if (!c.d2_valid) {
       if (!c.d2term.p && (rc = c.d2term.alloc(m->nb() * 4))) return rc;
       if ((rc = launch_dark_term2(s, c.row.as<int32_t>(), c.pix.as<int32_t>(), c.w.as<float>(),
-                                  m->d_dark.as<double>(), m->nb(), c.d2term.as<float>())))
+                                  m->d_dark.as<double>(), m->d_dint.as<uint16_t>(), m->nb(),
+                                  c.d2term.as<float>())))
         return rc;
       c.d2_valid = true;
     }
@@ -1299,7 +1348,11 @@ int mc_integrate_variance(mc_map *m, const void *frames, int32_t dtype, int32_t 
     cd = dc.as<float>();
     vard = dvar.as<float>();
   }
-  if ((rc = launch_plain_var(s, *m, c, binfo, d2, fd, dtype, vd, cd, vard, n_frames))) return rc;
+  int64_t sub_frames = 0;
+  if ((rc = ensure_sub(m->work.xt, *m, n_frames, &sub_frames))) return rc;
+  if ((rc = launch_plain_var(s, *m, c, binfo, d2, fd, dtype, vd, cd, vard, n_frames,
+                             m->work.xt.as<float>(), sub_frames)))
+    return rc;
   if (!out_on_device) {
     MC_HIP(hipMemcpyAsync(cakes, cd, cbytes, hipMemcpyDeviceToHost, s));
     MC_HIP(hipMemcpyAsync(variances, vard, cbytes, hipMemcpyDeviceToHost, s));
@@ -1344,7 +1397,7 @@ struct mc_graph {
   uint32_t flags = 0;
   uint64_t tiles_id = 0;      // the tile layout the graph reads
   Tiles *t = nullptr;
-  void *xt = nullptr;         // and the pixel-major batch
+  Work work;                  // the graph's own scratch (its passes write it)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -1361,17 +1414,17 @@ int mc_graph_create(mc_map *m, const void *frames_dev, int32_t dtype, int32_t n_
   MC_HIP(hipSetDevice(m->ctx->device));
   hipStream_t s = m->ctx->stream;
   int rc;
+  auto *g = new mc_graph();
+  std::unique_ptr<mc_graph> guard(g);
   // One eager call builds every table and buffer the passes read (nothing
   // is allocated or copied while capturing) and sets the kernels' LDS
   // attributes.
   if ((rc = run_integrate(m, s, frames_dev, dtype, n_frames, mode, flags, cakes_dev,
-                          lineouts_dev, false, true)))
+                          lineouts_dev, false, true, &g->work)))
     return rc;
   MC_HIP(hipStreamSynchronize(s));
   Tiles *t = nullptr;
   if ((rc = ensure_tiles(m, mode, batch_elem(dtype), &t))) return rc;
-  auto *g = new mc_graph();
-  std::unique_ptr<mc_graph> guard(g);
   g->m = m;
   g->ctx = m->ctx;
   g->mode = mode;
@@ -1379,10 +1432,9 @@ int mc_graph_create(mc_map *m, const void *frames_dev, int32_t dtype, int32_t n_
   g->flags = flags;
   g->t = t;
   g->tiles_id = t->id;
-  g->xt = m->xt.p;
   MC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   rc = run_integrate(m, s, frames_dev, dtype, n_frames, mode, flags, cakes_dev, lineouts_dev,
-                     false, true);
+                     false, true, &g->work);
   const hipError_t e = hipStreamEndCapture(s, &g->graph);
   if (rc) return rc;
   MC_CHECK(e == hipSuccess && g->graph, MC_ERR_HIP, "graph capture failed: %s",
@@ -1398,17 +1450,16 @@ int mc_graph_launch(mc_graph *g) {
   std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
   std::lock_guard<std::mutex> lk2(m->mu);
   MC_HIP(hipSetDevice(m->ctx->device));
-  // The graph reads the map's buffers in place: the tile layout and the
-  // pixel-major batch must still be the ones captured (a new beam centre in
-  // gradient mode rebuilds the layout, a larger frame type the batch); a
-  // new dark is folded into the same record buffer before the launch.
+  // The graph reads the map's buffers in place: the tile layout must still
+  // be the one captured (a new beam centre in gradient mode rebuilds it); a
+  // new dark is folded into the same record and entry buffers before the
+  // launch.  The scratch buffers are the graph's own.
   const float2 *binfo = nullptr;
   int rc;
   if ((rc = ensure_binfo(m, g->mode, g->flags, &binfo))) return rc;
   bool alive = false;
   for (auto &tp : m->tiles[g->mode]) alive |= tp.get() == g->t && tp->id == g->tiles_id;
-  MC_CHECK(alive && m->xt.p == g->xt, MC_ERR_STATE,
-           "the map's batched layout changed since the graph was captured");
+  MC_CHECK(alive, MC_ERR_STATE, "the map's batched layout changed since the graph was captured");
   if ((rc = ensure_recs(m, m->ctx->stream, g->t, g->mode, g->flags, binfo))) return rc;
   MC_HIP(hipGraphLaunch(g->exec, m->ctx->stream));
   return MC_OK;
@@ -1472,8 +1523,11 @@ int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype, int32_t 
   Tiles *t = nullptr;
   if (kind == 1 && ((rc = ensure_tiles(m, mode, batch_elem(dtype), &t)) ||
                     (rc = ensure_recs(m, s, t, mode, flags, binfo)) ||
-                    (rc = ensure_xt(m->xt, *m, dtype))))
+                    (rc = ensure_xt(m->work.xt, *m, dtype)) ||
+                    (rc = ensure_part(m->work.part, *t))))
     return rc;
+  int64_t sub_frames = 0;
+  if (kind == 0 && (rc = ensure_sub(m->work.xt, *m, 1, &sub_frames))) return rc;
   const size_t es = dtype_size(dtype);
   auto one_iter = [&](std::vector<hipEvent_t> *ev) -> int {
     const int step = kind == 1 ? pass_frames(*t) : 1;
@@ -1484,9 +1538,9 @@ int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype, int32_t 
       float *lb = lineouts_dev ? lineouts_dev + (size_t)f0 * 3 * m->n_r : nullptr;
       if (ev) MC_HIP(hipEventRecord((*ev)[2 * (f0 / step)], s));
       int r2 = kind == 1 ? launch_batched(s, *m, *t, binfo, mask_nan, fb, dtype, nf, cb, lb,
-                                          m->xt.p)
+                                          m->work.xt.p, m->work.part.as<float>())
                          : launch_plain(s, *m, m->csr[mode], binfo, mask_nan, fb, dtype, cb, lb,
-                                        1, m->npx());
+                                        1, m->npx(), m->work.xt.as<float>(), sub_frames);
       if (r2) return r2;
       if (ev) MC_HIP(hipEventRecord((*ev)[2 * (f0 / step) + 1], s));
     }

@@ -88,9 +88,9 @@ struct Csr {
   DBuf w;     // float [nnz]
   std::vector<int32_t> h_row, h_pix;  // host copies for the tile builder
   std::vector<float> h_w;
-  DBuf dterm;                         // float [nb], dark term of this CSR
+  DBuf dterm;                         // float [nb], dark remainder sum w (dark - dint)
   bool dterm_valid = false;
-  DBuf d2term;                        // float [nb], sum w^2 * dark (variance)
+  DBuf d2term;                        // float [nb], sum w^2 (dark - dint) (variance)
   bool d2_valid = false;
 };
 
@@ -117,8 +117,20 @@ struct Tiles {
   int n_blocks64 = 0, n_blocks128 = 0;
   DBuf xcd;          // int32 [9]: chunk range of each XCD (equal cost)
   int grid = 0;      // workgroups of the launch: 8 x the largest XCD range
-  DBuf part;         // float [n_chunks][rows][4][64]: lineout partials
+  uint64_t ents_dark = 0;  // dark version whose pixel darks the staged entries carry
   int64_t n_tiles = 0, n_tbins = 0, n_ents = 0, n_direct = 0, staged_px = 0;
+  // Bytes of lineout partials a pass over this layout writes:
+  // float [n_chunks][rows][4][64].
+  size_t part_bytes() const { return (size_t)n_chunks * rows * 4 * 64 * sizeof(float); }
+};
+
+// Scratch of the batched passes of one caller: the pixel-major batch and
+// the lineout partials.  Each stream that runs passes owns one (the map's
+// context, each server slot, each captured graph): two passes on different
+// streams never share a buffer that a pass writes.
+struct Work {
+  DBuf xt;
+  DBuf part;
 };
 
 struct Map {
@@ -145,7 +157,11 @@ struct Map {
   double bc_y = 0, bc_z = 0;
   // dark
   bool has_dark = false;
-  DBuf d_dark;                      // double [npx]
+  uint64_t dark_version = 1;        // bumped by every mc_map_set_dark
+  DBuf d_dark;                      // double [npx]: mean dark x scale
+  DBuf d_dint;                      // uint16 [npx]: the dark rounded to an integer in
+                                    // [0, 65535], subtracted from each pixel exactly;
+                                    // the remainder is folded per bin (dark term)
   // per mode layouts
   Csr csr[3];
   std::vector<std::unique_ptr<Tiles>> tiles[3];
@@ -156,7 +172,7 @@ struct Map {
   uint64_t binfo_version[12] = {};  // bumped whenever a table is rebuilt
   // tuning
   int batch = 64, rows_per_group = 16, lds_bytes = 81920;
-  DBuf xt;  // pixel-major batch of the calls made through the map's context
+  Work work;  // scratch of the calls made through the map's context stream
   int64_t nb() const { return (int64_t)n_r * n_eta; }
   int64_t npx() const { return (int64_t)ny * nz; }
 };
@@ -167,28 +183,40 @@ size_t dtype_size(int dtype);
 int launch_area_fill(hipStream_t s, float *p, int64_t n, float v);
 int launch_dark_mean(hipStream_t s, const void *frames, int dtype,
                      int n_frames, int64_t npx, double scale, double *out);
+// dint[p] = the mean dark rounded to an integer and clamped to [0, 65535].
+int launch_dark_int(hipStream_t s, const double *dark, int64_t npx, uint16_t *dint);
+// dterm[b] = sum_k w_k * (dark - dint)[pix_k] in FP64 (0 without a dark).
 int launch_dark_term(hipStream_t s, const int32_t *row, const int32_t *pix,
-                     const float *w, const double *dark, int64_t nb,
+                     const float *w, const double *dark, const uint16_t *dint, int64_t nb,
                      float *dterm);
 int launch_binfo(hipStream_t s, const float *inv_area, const int32_t *flags,
                  const float *dterm, int64_t nb, int normalize, int mask_nan,
                  float2 *binfo);
+// d2term[b] = sum_k w_k^2 * (dark - dint)[pix_k] (Poisson variance).
 int launch_dark_term2(hipStream_t s, const int32_t *row, const int32_t *pix, const float *w,
-                      const double *dark, int64_t nb, float *d2term);
+                      const double *dark, const uint16_t *dint, int64_t nb, float *d2term);
 // One-frame kernel with variance propagation; vimg NULL = Poisson (the
 // frame itself, minus the dark's d2term when that is given).
 int launch_plain_var(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo,
                      const float *d2term, const void *frames, int dtype, const float *vimg,
-                     float *cake, float *var, int n_frames);
+                     float *cake, float *var, int n_frames, float *sub, int64_t sub_frames);
+// One-frame kernel.  With a dark, each frame is first written to sub as
+// FP32 minus the pixel's integer dark (sub holds sub_frames frames; the
+// frames go through in chunks of that many).
 int launch_plain(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo,
                  int mask_nan, const void *frame, int dtype, float *cake,
-                 float *lineouts, int n_frames, int64_t frame_stride);
+                 float *lineouts, int n_frames, int64_t frame_stride, float *sub,
+                 int64_t sub_frames);
 // Tile records of t for one binfo table (scale/dark/area folded in).
 int launch_tile_recs(hipStream_t s, const Map &m, Tiles &t, const float2 *binfo);
-// One batched pass of up to 64 frames; xt is scratch of batch_xt_bytes().
+// Writes each staged entry's pixel dark (dint, 0 without a dark) into the
+// high half of its first word (2-byte batch elements).
+int launch_ents_dark(hipStream_t s, const Map &m, Tiles &t);
+// One batched pass of up to 64 frames; xt is scratch of batch_xt_bytes(),
+// part of t.part_bytes() (needed when lineouts is not NULL).
 int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
                    int mask_nan, const void *frames, int dtype, int n_frames, float *cakes,
-                   float *lineouts, void *xt);
+                   float *lineouts, void *xt, float *part);
 size_t batch_xt_bytes(const Map &m, int dtype);
 int launch_profile(hipStream_t s, const Map &m, const float *cakes, int n_frames,
                    int mask_nan, float *lineouts);
@@ -204,10 +232,13 @@ int map_finish_host(Map *m);  // derived arrays + upload
 int ensure_csr(Map *m, int mode);
 int ensure_tiles(Map *m, int mode, int esize, Tiles **out);
 // Launches the integration of n_frames device frames on stream s (no sync).
-// xt: scratch of the batched passes (NULL: the map's own).
+// work: the caller's scratch for batched passes on s (NULL: the map's own,
+// for the map's context stream only).
 int run_integrate(Map *m, hipStream_t s, const void *frames_dev, int dtype, int n_frames,
                   int mode, uint32_t flags, float *cakes_dev, float *lineouts_dev,
-                  bool force_plain, bool force_batched, DBuf *xt = nullptr);
+                  bool force_plain, bool force_batched, Work *work = nullptr);
+// Sizes work for batched passes of (mode, dtype) over m's current layout.
+int ensure_work(Map *m, int mode, int dtype, Work *work);
 // Everything the batched kernel reads for (mode, flags, dtype), built ahead.
 int prepare_batched(Map *m, int mode, uint32_t flags, int dtype);
 // The per-bin table of (mode, flags); builds it (and the dark term) once.

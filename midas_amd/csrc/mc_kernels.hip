@@ -1,16 +1,24 @@
 // mc_kernels.hip -- the gather-reduce kernels of the caking path (gfx950).
 //
 // Per (R, eta) bin b and frame f the cake value is
-//     I[f][b] = ( sum_k frame_f[pix_k] * w_k  -  D_b ) * s_b
+//     I[f][b] = ( sum_k (frame_f[pix_k] - dhi[pix_k]) * w_k  -  D_b ) * s_b
 // where (pix_k, w_k) are the bin's CSR entries (w = Map.bin frac, or frac
-// times a bilinear corner weight), D_b = sum_k dark[pix_k] * w_k is the dark
-// term of the bin (computed once per dark in FP64), and s_b = 1 / sum
-// areaWeight (Normalize=1) or 1.  This is the per-bin loop of
-// IntegratorZarrOMP.c:1710-1778 with "Image = raw - dark" (:1646-1669)
-// distributed over the sum: subtracting the dark per bin instead of per
-// pixel is exact in real arithmetic and removes the dark from the
-// per-frame traffic.  Bins flagged in maskMap.bin carry a negative s_b and
-// come out NaN when MC_F_MASK_NAN is set (IntegratorZarrOMP.c:1766-1767).
+// times a bilinear corner weight), dhi is the dark rounded to an integer
+// (clamped to [0, 65535]), D_b = sum_k (dark - dhi)[pix_k] * w_k is the
+// bin's dark remainder (computed once per dark in FP64), and s_b =
+// 1 / sum areaWeight (Normalize=1) or 1.  This is the per-bin loop of
+// IntegratorZarrOMP.c:1710-1778 with "Image = raw - dark" (:1646-1669):
+// the integer part of the dark is subtracted from each pixel before the
+// product, exactly (integer counts and integer darks are exact in FP32), so
+// the FP32 sums see dark-subtracted values even under a dark pedestal of
+// thousands of counts; only the fractional remainder, below 0.5 a pixel,
+// is distributed over the sum.  Since |dark - dhi| <= |frame - dark| on a
+// pixel whose count equals dhi, and |frame - dark| >= 1/2 on any other,
+// the remainder is at most twice sum_k w_k |frame - dark|[pix_k]: the FP32
+// rounding stays relative to the dark-subtracted signal of the bin.  Bins
+// flagged in maskMap.bin carry a NaN
+// s_b when MC_F_MASK_NAN is set and come out NaN (IntegratorZarrOMP.c
+// :1766-1767).
 //
 // Accumulation is FP32: uint16/int32 counts convert exactly (int32 up to
 // 2^24), weights are FP32, sums use Kahan compensation per lane and a
@@ -74,9 +82,6 @@ __device__ __forceinline__ void write_lineouts(float4 lo, float4 rs, int mask_na
   L[2 * n_r + r] = na > 0.f ? lo.w / na : 0.f;
 }
 
-__device__ __forceinline__ bool flagged(float scale) {
-  return __float_as_int(scale) < 0;
-}
 
 // Block-wide sum of 4 floats (256 threads), result valid in thread 0.
 __device__ __forceinline__ void block_sum4(float4 &v, float4 *red) {
@@ -107,13 +112,17 @@ __device__ __forceinline__ void block_sum4(float4 &v, float4 *red) {
 // lengthen each lane's chain of dependent loads).  grid = (n_r, n_frames).
 // ---------------------------------------------------------------------
 constexpr int kPlainLanes = 8;
+//
+// With a dark the kernel reads frames already dark-subtracted by
+// k_frame_dark (FP32, count minus the pixel's integer dark, exact): a second
+// gather per entry for the dark costs more than that elementwise pass.
 template <typename T, int LPB>
 __global__ __launch_bounds__(256) void k_plain(
     const int32_t *__restrict__ row, const int32_t *__restrict__ pix,
     const float *__restrict__ w, const float2 *__restrict__ binfo,
     const float *__restrict__ a_inb, const float4 *__restrict__ row_static,
-    const T *__restrict__ frames, int64_t frame_stride, int n_eta, int n_r,
-    int64_t nb, int mask_nan, float *__restrict__ cake, float *__restrict__ lineouts) {
+    const T *__restrict__ frames, int64_t frame_stride, int n_eta, int n_r, int64_t nb,
+    int mask_nan, float *__restrict__ cake, float *__restrict__ lineouts) {
   constexpr int SLOTS = 256 / LPB;
   const int r = blockIdx.x, f = blockIdx.y;
   const T *__restrict__ img = frames + (int64_t)f * frame_stride;
@@ -128,8 +137,7 @@ __global__ __launch_bounds__(256) void k_plain(
     float v = dpp_sum<LPB>(acc.value());
     if (sub == 0) {
       const float2 bi = binfo[b];
-      const float val = (v - bi.y) * fabsf(bi.x);
-      const float o = flagged(bi.x) ? NAN : val;
+      const float o = (v - bi.y) * bi.x;  // NaN scale: a masked bin
       out[b] = o;
       if (lineouts) {
         if (o != 0.f && !isnan(o)) {
@@ -155,9 +163,10 @@ __global__ __launch_bounds__(256) void k_plain(
 // One-frame kernel with variance propagation (integrate_with_variance,
 // kernels.py:476-561): beside each cake value, the sum of w^2 * var over
 // the same merged (bin, pixel) weights, scaled by the square of the value's
-// own scale.  var is the frame itself (Poisson: Var(I) = I, with the dark's
-// sum w^2 * dark subtracted as the dark is from the value) or a variance
-// image.  grid = (n_r, n_frames).
+// own scale.  var is the dark-subtracted frame (Poisson: Var(I) = I; the
+// integer dark per pixel, the remainder's sum w^2 (dark - dint) per bin)
+// or a variance image.  With a dark the frames are k_frame_dark's output.
+// grid = (n_r, n_frames).
 // ---------------------------------------------------------------------
 template <typename T, bool POISSON>
 __global__ __launch_bounds__(256) void k_plain_var(
@@ -177,7 +186,8 @@ __global__ __launch_bounds__(256) void k_plain_var(
     Kahan acc, acc2;
     for (int k = beg + sub; k < end; k += LPB) {
       const int p = pix[k];
-      const float x = to_f(img[p]), ww = w[k];
+      const float x = to_f(img[p]);
+      const float ww = w[k];
       acc.add(x * ww);
       acc2.add((POISSON ? x : vi[p]) * ww * ww);
     }
@@ -185,13 +195,30 @@ __global__ __launch_bounds__(256) void k_plain_var(
     const float v2 = dpp_sum<LPB>(acc2.value());
     if (sub == 0) {
       const float2 bi = binfo[b];
-      const float sc = fabsf(bi.x);
-      const bool fl = flagged(bi.x);
-      cake[(int64_t)f * nb + b] = fl ? NAN : (v - bi.y) * sc;
+      const float sc = bi.x;  // NaN: a masked bin
+      cake[(int64_t)f * nb + b] = (v - bi.y) * sc;
       const float raw = POISSON && d2term ? v2 - d2term[b] : v2;
-      var[(int64_t)f * nb + b] = fl ? NAN : raw * sc * sc;
+      var[(int64_t)f * nb + b] = raw * sc * sc;
     }
   }
+}
+
+// ---------------------------------------------------------------------
+// Frames minus the integer dark per pixel, as FP32 (exact for integer
+// counts below 2^24): the one-frame kernels' input when a dark is set.
+// Four pixels per thread; grid = (pixels / 1024, frames).
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_frame_dark(const T *__restrict__ frames, int64_t npx,
+                                                    const uint16_t *__restrict__ dint,
+                                                    float *__restrict__ out) {
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i >= npx) return;
+  const T *__restrict__ src = frames + (int64_t)blockIdx.y * npx;
+  float *__restrict__ dst = out + (int64_t)blockIdx.y * npx;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (i + q < npx) dst[i + q] = to_f(src[i + q]) - (float)dint[i + q];
 }
 
 // ---------------------------------------------------------------------
@@ -289,11 +316,16 @@ typedef unsigned u4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 // ---- pixel-major batch ----
+// FP32 batches (32/64-bit frames) carry the dark-subtracted value: the
+// integer dark dhi of the pixel is subtracted here, exactly for integer
+// counts below 2^24 (uint16 batches keep the raw count and subtract the
+// dark in k_batched, from the staged entries).
 template <typename TIn, typename TL>
 __global__ __launch_bounds__(256) void k_transpose(const TIn *__restrict__ frames,
                                                    int64_t frame_stride, int n_frames,
                                                    int64_t npx, int vec_ok,
                                                    const int32_t *__restrict__ blocks,
+                                                   const uint16_t *__restrict__ dint,
                                                    TL *__restrict__ xt) {
   constexpr int PX = 64;                       // pixels per block
   constexpr int EV = 16 / (int)sizeof(TL);     // elements per 16-byte store
@@ -321,7 +353,15 @@ __global__ __launch_bounds__(256) void k_transpose(const TIn *__restrict__ frame
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       TL v = static_cast<TL>(0);
-      if (f < n_frames && p0 + q0 + q < npx) v = static_cast<TL>(src[q]);
+      if (f < n_frames && p0 + q0 + q < npx) {
+        if constexpr (sizeof(TL) == 4) {
+          float x = static_cast<float>(src[q]);
+          if (dint) x -= (float)dint[p0 + q0 + q];
+          v = x;
+        } else {
+          v = static_cast<TL>(src[q]);
+        }
+      }
       tile[f][q0 + q] = v;
     }
   }
@@ -543,16 +583,26 @@ __device__ __forceinline__ void kahan_add(Kahan &k, float x) {
 
 // Two workgroups share a CU (8 waves per SIMD, at most 64 VGPRs and half
 // the LDS each): while one waits for its tile's copies, the other reduces.
-template <typename TL, int G>
+//
+// DARK (uint16 batches only): the first word of a staged entry holds the
+// byte offset of its pixel row in the low half and the pixel's integer dark
+// in the high half (k_ents_dark); each value is converted and the dark
+// subtracted before the product, so the sums are of dark-subtracted counts.
+// Direct bins read the dark from dint.
+//
+// A bin's record fields that steer control flow (cake bin, entry range,
+// direct flag) are read once into scalar registers: the granule bookkeeping
+// and the entry loop bounds are then scalar work, and the granule slot is a
+// uniform index.
+template <typename TL, int G, bool DARK>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batched(
     const int32_t *__restrict__ chunks, const int32_t *__restrict__ xcd,
     const int32_t *__restrict__ tiles, const BinRec *__restrict__ recs,
     const uint2 *__restrict__ ents, const int32_t *__restrict__ tpix, const TL *__restrict__ xt,
-    int n_frames, int tp_off, int tp_len, int64_t nb, float *__restrict__ cakes,
-    float *__restrict__ part) {
+    const uint16_t *__restrict__ dint, int n_frames, int tp_off, int tp_len, int64_t nb,
+    float *__restrict__ cakes, float *__restrict__ part) {
   extern __shared__ __attribute__((aligned(16))) char lds_raw[];
   typedef typename Quad<TL>::type QT;
-  constexpr int RB = kXtFrames * (int)sizeof(TL);
   constexpr int QB = 4 * (int)sizeof(TL);  // bytes of one lane's 4 frames
   constexpr int RBAND = kWaves * G;
   constexpr int LG = 8;  // cake bins per granule: 32-byte stores
@@ -583,6 +633,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
     for (int k = 0; k < 4; ++k) lo[g][k] = 0.f;
   }
   const bool vec_ok = ((nb & 3) == 0) && (((uintptr_t)cakes & 15) == 0);
+  const bool live = frame < n_frames;
   const char *lpl = plane + l16 * QB;  // this lane's 4 frames in a pixel row
 
   // Pixel lists of tiles t (buffer (t - t_beg) & 1) and t + 1 (the other):
@@ -617,10 +668,14 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
       for (int e = 0; e < h.nc; ++e) {
         const BinRec r = rn;
         if (e + 1 < h.nc) rn = rrow[e + 1];  // next record early: hides its LDS latency
+        const int rbin = __builtin_amdgcn_readfirstlane(r.tb.x);
+        const int rbeg = __builtin_amdgcn_readfirstlane(r.tb.y);
+        const int rend = __builtin_amdgcn_readfirstlane(r.tb.z);
+        const int rdir = __builtin_amdgcn_readfirstlane(r.tb.w);
         float tot[4];
-        if (!r.tb.w) {
-          int k = r.tb.y - h.e_beg;
-          const int nblk = (r.tb.z - r.tb.y) >> 3;
+        if (!rdir) {
+          int k = rbeg - h.e_beg;
+          const int nblk = (rend - rbeg) >> 3;
           if (!primed) {
             tw = *reinterpret_cast<const u4v *>(lent + k + 2 * grp);
             primed = true;
@@ -629,14 +684,24 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
           Kahan acc[4];  // set at the fourth group of 8, used only past it
           for (int blk = 0; blk < nblk; ++blk, k += 8) {
             const u4v two = tw;
-            const QT da = *reinterpret_cast<const QT *>(lpl + two.x);
-            const QT db = *reinterpret_cast<const QT *>(lpl + two.z);
+            const uint32_t oa = DARK ? (two.x & 0xffffu) : two.x;
+            const uint32_t obb = DARK ? (two.z & 0xffffu) : two.z;
+            const QT da = *reinterpret_cast<const QT *>(lpl + oa);
+            const QT db = *reinterpret_cast<const QT *>(lpl + obb);
             // The run's next group (past the run's end this reads unused
             // LDS words inside the allocation).
             tw = *reinterpret_cast<const u4v *>(lent + k + 8 + 2 * grp);
             float fa[4], fb[4];
             Quad<TL>::unpack(da, fa);
             Quad<TL>::unpack(db, fb);
+            if constexpr (DARK) {
+              const float dka = (float)(two.x >> 16), dkb = (float)(two.z >> 16);
+#pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                fa[j] -= dka;
+                fb[j] -= dkb;
+              }
+            }
             const float wa = __uint_as_float(two.y), wb = __uint_as_float(two.w);
 #pragma unroll
             for (int j = 0; j < 4; ++j) gs[j] = fmaf(fb[j], wb, fmaf(fa[j], wa, gs[j]));
@@ -666,12 +731,17 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
         } else {
           Kahan acc[4];
           // Entries stay in global memory: group g takes entries g, g+4, ...
-          for (int k = r.tb.y + grp; k < r.tb.z; k += 4) {
+          for (int k = rbeg + grp; k < rend; k += 4) {
             const uint2 en = ents[k];
             const QT d = *reinterpret_cast<const QT *>(
                 reinterpret_cast<const char *>(xt + (int64_t)en.x * kXtFrames) + l16 * QB);
             float fv[4];
             Quad<TL>::unpack(d, fv);
+            if constexpr (DARK) {
+              const float dk = (float)dint[en.x];
+#pragma unroll
+              for (int j = 0; j < 4; ++j) fv[j] -= dk;
+            }
 #pragma unroll
             for (int j = 0; j < 4; ++j) kahan_add(acc[j], fv[j] * __uint_as_float(en.y));
           }
@@ -680,16 +750,14 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
         }
         // The four groups hold partial sums of the same frames.
         const float mine = group_scatter(tot);
-        const int gr = r.tb.x / LG, gk = r.tb.x - gr * LG;
+        const int gr = rbin / LG, gk = rbin - gr * LG;
         if (gr != og[g]) {
-          if (om[g] && frame < n_frames) flush_granule<LG>(ob[g], og[g], om[g], frame, cakes, nb,
-                                                            vec_ok);
+          if (om[g] && live) flush_granule<LG>(ob[g], og[g], om[g], frame, cakes, nb, vec_ok);
           og[g] = gr;
           om[g] = 0u;
         }
         om[g] |= 1u << gk;
-        const float val = (mine - r.info.y) * fabsf(r.info.x);
-        const float o = flagged(r.info.x) ? NAN : val;
+        const float o = (mine - r.info.y) * r.info.x;  // NaN scale: a masked bin
         ob[g][gk] = o;
         if (o != 0.f && !isnan(o)) {
           lo[g][0] += o;
@@ -704,14 +772,33 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
   }
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    if (om[g] && frame < n_frames)
-      flush_granule<LG>(ob[g], og[g], om[g], frame, cakes, nb, vec_ok);
+    if (om[g] && live) flush_granule<LG>(ob[g], og[g], om[g], frame, cakes, nb, vec_ok);
     if (want_lo) {
       // part[c][row][k][64] for the band row this slot holds.
       float *Pp = part + ((int64_t)c * RBAND + kWaves * g + wave) * 4 * kXtFrames;
 #pragma unroll
       for (int k = 0; k < 4; ++k) Pp[k * kXtFrames + frame] = lo[g][k];
     }
+  }
+}
+
+// Staged entries of the uint16 layout: the low half of the first word is
+// the byte offset of the pixel row in the tile's staged footprint; the high
+// half receives the pixel's integer dark (dhi, 0 without a dark).  One
+// workgroup per tile; the padding entries point at the zero row (offset
+// npx * 128) and get 0.
+__global__ __launch_bounds__(256) void k_ents_dark(const int32_t *__restrict__ tiles,
+                                                   const int32_t *__restrict__ tpix,
+                                                   const uint16_t *__restrict__ dint,
+                                                   uint2 *__restrict__ ents) {
+  const int32_t *T = tiles + kTileInts * blockIdx.x;
+  const int px_beg = T[0], npx = T[1], e_beg = T[6], e_end = T[7];
+  for (int i = e_beg + (int)threadIdx.x; i < e_end; i += 256) {
+    const uint32_t off = ents[i].x & 0xffffu;
+    const int slot = (int)(off / (kXtFrames * 2));
+    uint32_t d = 0u;
+    if (dint && slot < npx) d = dint[tpix[px_beg + slot]];
+    ents[i].x = off | (d << 16);
   }
 }
 
@@ -771,33 +858,53 @@ __global__ void k_dark_mean(const T *__restrict__ frames, int n_frames,
   out[i] = s / n_frames * scale;
 }
 
+// The integer part of the dark that the kernels subtract per pixel: the
+// nearest integer, clamped to the uint16 range the batched entries carry.
+__global__ void k_dark_int(const double *__restrict__ dark, int64_t npx,
+                           uint16_t *__restrict__ dint) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npx) return;
+  dint[i] = (uint16_t)fmin(fmax(rint(dark[i]), 0.0), 65535.0);
+}
+
+// The dark remainder of each bin, sum w * (dark - dhi), in FP64.
 __global__ void k_dark_term(const int32_t *__restrict__ row, const int32_t *__restrict__ pix,
                             const float *__restrict__ w, const double *__restrict__ dark,
-                            int64_t nb, float *__restrict__ dterm) {
+                            const uint16_t *__restrict__ dint, int64_t nb,
+                            float *__restrict__ dterm) {
   int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   double s = 0;
   if (dark)
-    for (int k = row[b]; k < row[b + 1]; ++k) s += dark[pix[k]] * (double)w[k];
+    for (int k = row[b]; k < row[b + 1]; ++k) {
+      const int p = pix[k];
+      s += (dark[p] - (double)dint[p]) * (double)w[k];
+    }
   dterm[b] = (float)s;
 }
 
 __global__ void k_dark_term2(const int32_t *__restrict__ row, const int32_t *__restrict__ pix,
                              const float *__restrict__ w, const double *__restrict__ dark,
-                             int64_t nb, float *__restrict__ d2term) {
+                             const uint16_t *__restrict__ dint, int64_t nb,
+                             float *__restrict__ d2term) {
   int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   double s = 0;
-  for (int k = row[b]; k < row[b + 1]; ++k) s += dark[pix[k]] * (double)w[k] * (double)w[k];
+  for (int k = row[b]; k < row[b + 1]; ++k) {
+    const int p = pix[k];
+    s += (dark[p] - (double)dint[p]) * (double)w[k] * (double)w[k];
+  }
   d2term[b] = (float)s;
 }
+
+// Per-bin {scale, dark remainder}; the scale of a bin masked to NaN is NaN.
 __global__ void k_binfo(const float *__restrict__ inv_area, const int32_t *__restrict__ flags,
                         const float *__restrict__ dterm, int64_t nb, int normalize,
                         int mask_nan, float2 *__restrict__ binfo) {
   int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   float s = normalize ? inv_area[b] : 1.f;
-  if (mask_nan && flags && flags[b]) s = -s;
+  if (mask_nan && flags && flags[b]) s = NAN;
   binfo[b] = make_float2(s, dterm ? dterm[b] : 0.f);
 }
 
@@ -850,10 +957,18 @@ int launch_dark_mean(hipStream_t s, const void *frames, int dtype, int n_frames,
   });
 }
 
+int launch_dark_int(hipStream_t s, const double *dark, int64_t npx, uint16_t *dint) {
+  hipLaunchKernelGGL(k_dark_int, dim3(blocks_for(npx, 256)), dim3(256), 0, s, dark, npx,
+                     dint);
+  MC_HIP(hipGetLastError());
+  return MC_OK;
+}
+
 int launch_dark_term(hipStream_t s, const int32_t *row, const int32_t *pix,
-                     const float *w, const double *dark, int64_t nb, float *dterm) {
+                     const float *w, const double *dark, const uint16_t *dint, int64_t nb,
+                     float *dterm) {
   hipLaunchKernelGGL(k_dark_term, dim3(blocks_for(nb, 256)), dim3(256), 0, s, row, pix,
-                     w, dark, nb, dterm);
+                     w, dark, dint, nb, dterm);
   MC_HIP(hipGetLastError());
   return MC_OK;
 }
@@ -869,44 +984,92 @@ int launch_binfo(hipStream_t s, const float *inv_area, const int32_t *flags,
 
 int launch_plain(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo, int mask_nan,
                  const void *frames, int dtype, float *cake, float *lineouts, int n_frames,
-                 int64_t frame_stride) {
+                 int64_t frame_stride, float *sub, int64_t sub_frames) {
+  const int64_t npx = m.npx();
+  if (m.has_dark) {
+    if (!sub || sub_frames < 1) {
+      set_error("the one-frame kernel needs scratch for dark-subtracted frames");
+      return MC_ERR_STATE;
+    }
+    const size_t es = dtype_size(dtype);
+    for (int f0 = 0; f0 < n_frames; f0 += (int)sub_frames) {
+      const int nf = (int)std::min<int64_t>(sub_frames, n_frames - f0);
+      const char *fb = (const char *)frames + (size_t)f0 * frame_stride * es;
+      int rc = dispatch_dtype(dtype, [&](auto tag) -> int {
+        using T = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
+        hipLaunchKernelGGL(k_frame_dark<T>, dim3(blocks_for((npx + 3) / 4, 256), nf),
+                           dim3(256), 0, s, (const T *)fb, npx, m.d_dint.as<uint16_t>(), sub);
+        MC_HIP(hipGetLastError());
+        return MC_OK;
+      });
+      if (rc) return rc;
+      hipLaunchKernelGGL((k_plain<float, kPlainLanes>), dim3(m.n_r, nf), dim3(256), 0, s,
+                         c.row.as<int32_t>(), c.pix.as<int32_t>(), c.w.as<float>(), binfo,
+                         m.d_area_inb.as<float>(), m.d_row_static.as<float4>(), sub, npx,
+                         m.n_eta, m.n_r, m.nb(), mask_nan, cake + (size_t)f0 * m.nb(),
+                         lineouts ? lineouts + (size_t)f0 * 3 * m.n_r : nullptr);
+      MC_HIP(hipGetLastError());
+    }
+    return MC_OK;
+  }
   return dispatch_dtype(dtype, [&](auto tag) -> int {
     using T = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
     hipLaunchKernelGGL((k_plain<T, kPlainLanes>), dim3(m.n_r, n_frames), dim3(256), 0, s,
                        c.row.as<int32_t>(), c.pix.as<int32_t>(), c.w.as<float>(), binfo,
-                       m.d_area_inb.as<float>(), m.d_row_static.as<float4>(),
-                       (const T *)frames, frame_stride, m.n_eta, m.n_r, m.nb(), mask_nan,
-                       cake, lineouts);
+                       m.d_area_inb.as<float>(), m.d_row_static.as<float4>(), (const T *)frames,
+                       frame_stride, m.n_eta, m.n_r, m.nb(), mask_nan, cake, lineouts);
     MC_HIP(hipGetLastError());
     return MC_OK;
   });
 }
 
 int launch_dark_term2(hipStream_t s, const int32_t *row, const int32_t *pix, const float *w,
-                      const double *dark, int64_t nb, float *d2term) {
+                      const double *dark, const uint16_t *dint, int64_t nb, float *d2term) {
   hipLaunchKernelGGL(k_dark_term2, dim3(blocks_for(nb, 256)), dim3(256), 0, s, row, pix, w,
-                     dark, nb, d2term);
+                     dark, dint, nb, d2term);
   MC_HIP(hipGetLastError());
   return MC_OK;
 }
 
 int launch_plain_var(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo,
                      const float *d2term, const void *frames, int dtype, const float *vimg,
-                     float *cake, float *var, int n_frames) {
-  return dispatch_dtype(dtype, [&](auto tag) -> int {
+                     float *cake, float *var, int n_frames, float *sub, int64_t sub_frames) {
+  const int64_t npx = m.npx(), nb = m.nb();
+  auto run = [&](auto tag, const void *fr, const float *vi, float *ck, float *vr, int nf) -> int {
     using T = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
-    const dim3 grid(m.n_r, n_frames);
-    if (vimg)
-      hipLaunchKernelGGL((k_plain_var<T, false>), grid, dim3(256), 0, s, c.row.as<int32_t>(),
-                         c.pix.as<int32_t>(), c.w.as<float>(), binfo, d2term, (const T *)frames,
-                         vimg, m.npx(), m.n_eta, m.nb(), cake, var);
-    else
-      hipLaunchKernelGGL((k_plain_var<T, true>), grid, dim3(256), 0, s, c.row.as<int32_t>(),
-                         c.pix.as<int32_t>(), c.w.as<float>(), binfo, d2term, (const T *)frames,
-                         vimg, m.npx(), m.n_eta, m.nb(), cake, var);
+    const dim3 grid(m.n_r, nf);
+    auto kern = vi ? k_plain_var<T, false> : k_plain_var<T, true>;
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, c.row.as<int32_t>(), c.pix.as<int32_t>(),
+                       c.w.as<float>(), binfo, d2term, (const T *)fr, vi, npx, m.n_eta, nb, ck,
+                       vr);
     MC_HIP(hipGetLastError());
     return MC_OK;
-  });
+  };
+  if (!m.has_dark)
+    return dispatch_dtype(dtype, [&](auto tag) -> int {
+      return run(tag, frames, vimg, cake, var, n_frames);
+    });
+  if (!sub || sub_frames < 1) {
+    set_error("the variance kernel needs scratch for dark-subtracted frames");
+    return MC_ERR_STATE;
+  }
+  const size_t es = dtype_size(dtype);
+  for (int f0 = 0; f0 < n_frames; f0 += (int)sub_frames) {
+    const int nf = (int)std::min<int64_t>(sub_frames, n_frames - f0);
+    const char *fb = (const char *)frames + (size_t)f0 * npx * es;
+    int rc = dispatch_dtype(dtype, [&](auto tag) -> int {
+      using T = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
+      hipLaunchKernelGGL(k_frame_dark<T>, dim3(blocks_for((npx + 3) / 4, 256), nf),
+                         dim3(256), 0, s, (const T *)fb, npx, m.d_dint.as<uint16_t>(), sub);
+      MC_HIP(hipGetLastError());
+      return MC_OK;
+    });
+    if (rc) return rc;
+    if ((rc = run((const float *)nullptr, sub, vimg ? vimg + (size_t)f0 * npx : nullptr,
+                  cake + (size_t)f0 * nb, var + (size_t)f0 * nb, nf)))
+      return rc;
+  }
+  return MC_OK;
 }
 
 int launch_profile(hipStream_t s, const Map &m, const float *cakes, int n_frames,
@@ -930,6 +1093,15 @@ int launch_tile_recs(hipStream_t s, const Map &m, Tiles &t, const float2 *binfo)
   return MC_OK;
 }
 
+int launch_ents_dark(hipStream_t s, const Map &m, Tiles &t) {
+  if (t.in_size != 2 || t.n_tiles == 0) return MC_OK;
+  hipLaunchKernelGGL(k_ents_dark, dim3((unsigned)t.n_tiles), dim3(256), 0, s,
+                     t.tiles.as<int32_t>(), t.tpix.as<int32_t>(),
+                     m.has_dark ? m.d_dint.as<uint16_t>() : nullptr, t.ents.as<uint2>());
+  MC_HIP(hipGetLastError());
+  return MC_OK;
+}
+
 size_t batch_xt_bytes(const Map &m, int dtype) {
   return (size_t)m.npx() * kXtFrames * (dtype_size(dtype) <= 2 ? 2 : 4);
 }
@@ -937,7 +1109,8 @@ size_t batch_xt_bytes(const Map &m, int dtype) {
 template <typename TIn, typename TL, int G>
 static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mask_nan,
                             const void *frames, int n_frames, float *cakes, float *lineouts,
-                            void *xt) {
+                            void *xt, float *part) {
+  const uint16_t *dhi = m.has_dark ? m.d_dint.as<uint16_t>() : nullptr;
   // 1. pixel-major batch
   const int64_t npx = m.npx();
   const int vec_ok = (npx % 64 == 0) && (((uintptr_t)frames & 15) == 0) ? 1 : 0;
@@ -950,20 +1123,22 @@ static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mas
   } else if (t.n_blocks64) {
     hipLaunchKernelGGL((k_transpose<TIn, TL>), dim3((unsigned)t.n_blocks64), dim3(256), 0, s,
                        (const TIn *)frames, npx, n_frames, npx, vec_ok, t.blocks64.as<int32_t>(),
-                       (TL *)xt);
+                       dhi, (TL *)xt);
   }
   MC_HIP(hipGetLastError());
-  // 2. tiles
-  auto kern = k_batched<TL, G>;
+  // 2. tiles (uint16 batches subtract the dark carried by the entries;
+  // FP32 batches were dark-subtracted by the transpose)
+  constexpr bool kU16 = sizeof(TL) == 2;
+  auto kern = (kU16 && dhi) ? k_batched<TL, G, kU16> : k_batched<TL, G, false>;
   const size_t lds = t.lds_needed;
   if (lds > 65536)
     MC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
-  float *part = lineouts ? t.part.as<float>() : nullptr;
+  if (!lineouts) part = nullptr;
   hipLaunchKernelGGL(kern, dim3((unsigned)t.grid), dim3(64 * kWaves), lds, s,
                      t.chunks.as<int32_t>(), t.xcd.as<int32_t>(), t.tiles.as<int32_t>(),
                      t.recs.as<BinRec>(), t.ents.as<uint2>(), t.tpix.as<int32_t>(),
-                     (const TL *)xt, n_frames, t.tp_off, t.tp_len, m.nb(), cakes, part);
+                     (const TL *)xt, dhi, n_frames, t.tp_off, t.tp_len, m.nb(), cakes, part);
   MC_HIP(hipGetLastError());
   // 3. lineouts
   if (lineouts) {
@@ -978,7 +1153,7 @@ static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mas
 
 int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
                    int mask_nan, const void *frames, int dtype, int n_frames, float *cakes,
-                   float *lineouts, void *xt) {
+                   float *lineouts, void *xt, float *part) {
   (void)binfo;  // folded into the tile records (launch_tile_recs)
   if (n_frames < 1 || n_frames > kXtFrames) {
     set_error("batched pass of %d frames outside 1..%d", n_frames, kXtFrames);
@@ -988,16 +1163,20 @@ int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *bi
     set_error("tile layout built for %d-byte batch elements", t.in_size);
     return MC_ERR_STATE;
   }
+  if (lineouts && !part) {
+    set_error("batched lineouts need a partials buffer");
+    return MC_ERR_ARG;
+  }
   return dispatch_dtype(dtype, [&](auto tag) -> int {
     using T = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
     using TL = typename std::conditional<sizeof(T) <= 2, uint16_t, float>::type;
     switch (t.rows / kWaves) {
       case 1:
         return launch_batched_g<T, TL, 1>(s, m, t, mask_nan, frames, n_frames, cakes, lineouts,
-                                          xt);
+                                          xt, part);
       case 2:
         return launch_batched_g<T, TL, 2>(s, m, t, mask_nan, frames, n_frames, cakes, lineouts,
-                                          xt);
+                                          xt, part);
       default:
         set_error("rows per band %d not in {16, 32}", t.rows);
         return MC_ERR_ARG;

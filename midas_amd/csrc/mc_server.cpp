@@ -51,6 +51,7 @@ struct Geo {
   int dtype = 1, mode = 1;
   uint32_t flags = 0;
   size_t frame_bytes = 0, cake_floats = 0, lo_floats = 0;
+  size_t part_bytes = 0;  // lineout partials of one batched pass
 };
 
 struct Req {
@@ -64,7 +65,7 @@ struct Slot {
   hipStream_t s = nullptr;
   hipEvent_t done = nullptr;
   DBuf frames, cakes, lo;
-  DBuf xt;  // pixel-major batch of the slot's passes
+  Work work;  // scratch of the slot's batched passes (pixel-major batch, lineout partials)
   float *h_cakes = nullptr, *h_lo = nullptr;  // pinned
   size_t cap_frames = 0, cap_cakes = 0, cap_lo = 0;
   std::vector<Req> reqs;
@@ -101,6 +102,11 @@ struct mc_server {
   Ctx *ctx = nullptr;
   int64_t budget = 0;
   int max_batch = 64, qdepth = 256;
+  // Batching window: a free slot waits up to this long after the oldest
+  // queued request's arrival for max_batch frames of its geometry (0: launch
+  // whatever is queued as soon as a slot is free).
+  int64_t window_us = 0;
+  int draining = 0;  // callers in mc_server_drain: close every window
   mc_result_cb cb = nullptr;
   void *user = nullptr;
   std::vector<Geo> geos;
@@ -129,17 +135,19 @@ struct mc_server {
         mc_map_info_get(g.m, &info);
         b += info.hbm_bytes;
       }
-    for (auto &s : slots) b += s.frames.bytes + s.cakes.bytes + s.lo.bytes + s.xt.bytes;
+    for (auto &s : slots)
+      b += s.frames.bytes + s.cakes.bytes + s.lo.bytes + s.work.xt.bytes + s.work.part.bytes;
     return b;
   }
 };
 
 namespace {
 
-int grow_slots(mc_server *sv, size_t fb, size_t cf, size_t lf, size_t xb) {
+int grow_slots(mc_server *sv, size_t fb, size_t cf, size_t lf, size_t xb, size_t pb) {
   for (auto &s : sv->slots) {
     int rc;
-    if (s.xt.bytes < xb && (rc = s.xt.alloc(xb))) return rc;
+    if (s.work.xt.bytes < xb && (rc = s.work.xt.alloc(xb))) return rc;
+    if (s.work.part.bytes < pb && (rc = s.work.part.alloc(pb))) return rc;
     if (s.cap_frames < fb) {
       if ((rc = s.frames.alloc(fb))) return rc;
       s.cap_frames = fb;
@@ -176,6 +184,20 @@ void scheduler(mc_server *sv) {
       return false;
     });
     if (sv->queue.empty()) break;
+    if (sv->window_us > 0 && !sv->stop) {
+      // Wait for a full batch of the oldest request's geometry, at most
+      // until its window closes (a drain or stop ends the wait at once).
+      auto full = [&] {
+        if (sv->stop || sv->draining) return true;
+        const int g0 = sv->queue.front().geo;
+        int n = 0;
+        for (const Req &r : sv->queue) n += r.geo == g0;
+        return n >= sv->max_batch;
+      };
+      const auto until = sv->queue.front().t_submit + std::chrono::microseconds(sv->window_us);
+      sv->cv_work.wait_until(lk, until, full);
+      if (sv->queue.empty()) break;
+    }
     Slot &S = sv->slots[slot];
     // The oldest request picks the geometry; take up to max_batch queued
     // frames of that geometry in arrival order.
@@ -207,7 +229,7 @@ void scheduler(mc_server *sv) {
       // computes all 64 frame lanes): small batches take the one-frame
       // kernel per frame.
       rc = run_integrate(G.m, S.s, S.frames.p, G.dtype, nf, G.mode, G.flags, S.cakes.as<float>(),
-                         S.lo.as<float>(), nf <= kPlainMaxFrames, false, &S.xt);
+                         S.lo.as<float>(), nf <= kPlainMaxFrames, false, &S.work);
     }
     if (!rc && hipMemcpyAsync(S.h_cakes, S.cakes.p, (size_t)nf * G.cake_floats * 4,
                               hipMemcpyDeviceToHost, S.s) != hipSuccess)
@@ -307,10 +329,14 @@ int mc_server_add_geometry(mc_server *sv, mc_map *m, int32_t dtype, int32_t mode
   // Wait for the device to be idle: slot buffers may have to grow.
   sv->cv_idle.wait(lk, [&] { return sv->in_flight == 0 && sv->queue.empty(); });
   int rc;
+  size_t pb = 0;  // lineout partials of this geometry's batched passes
   {
     std::lock_guard<std::recursive_mutex> clk(sv->ctx->mu);
     std::lock_guard<std::mutex> mlk(m->mu);
     if ((rc = prepare_batched(m, mode, flags, dtype))) return rc;
+    Tiles *t = nullptr;
+    if ((rc = ensure_tiles(m, mode, dtype_size(dtype) <= 2 ? 2 : 4, &t))) return rc;
+    pb = t->part_bytes();
   }
   Geo G;
   G.m = m;
@@ -320,12 +346,14 @@ int mc_server_add_geometry(mc_server *sv, mc_map *m, int32_t dtype, int32_t mode
   G.frame_bytes = (size_t)m->npx() * dtype_size(dtype);
   G.cake_floats = (size_t)m->nb();
   G.lo_floats = (size_t)3 * m->n_r;
+  G.part_bytes = pb;
   size_t fb = 0, cf = 0, lf = 0, xb = batch_xt_bytes(*m, dtype);
   for (auto &g : sv->geos) {
     fb = std::max(fb, g.frame_bytes);
     cf = std::max(cf, g.cake_floats);
     lf = std::max(lf, g.lo_floats);
     xb = std::max(xb, batch_xt_bytes(*g.m, g.dtype));
+    pb = std::max(pb, g.part_bytes);
   }
   fb = std::max(fb, G.frame_bytes) * sv->max_batch;
   cf = std::max(cf, G.cake_floats) * sv->max_batch;
@@ -343,7 +371,8 @@ int mc_server_add_geometry(mc_server *sv, mc_map *m, int32_t dtype, int32_t mode
     need += (int64_t)(std::max(fb, s.cap_frames) - s.frames.bytes) +
             (int64_t)(std::max(cf, s.cap_cakes) * 4 - s.cakes.bytes) +
             (int64_t)(std::max(lf, s.cap_lo) * 4 - s.lo.bytes) +
-            (int64_t)(std::max(xb, s.xt.bytes) - s.xt.bytes);
+            (int64_t)(std::max(xb, s.work.xt.bytes) - s.work.xt.bytes) +
+            (int64_t)(std::max(pb, s.work.part.bytes) - s.work.part.bytes);
   if (need > sv->budget) {
     sv->rejected++;
     set_error("geometry needs %lld bytes resident, budget is %lld", (long long)need,
@@ -351,7 +380,7 @@ int mc_server_add_geometry(mc_server *sv, mc_map *m, int32_t dtype, int32_t mode
     return MC_ERR_BUDGET;
   }
   MC_HIP(hipSetDevice(sv->ctx->device));
-  if ((rc = grow_slots(sv, fb, cf, lf, xb))) return rc;
+  if ((rc = grow_slots(sv, fb, cf, lf, xb, pb))) return rc;
   sv->geos.push_back(G);
   sv->free_host.emplace_back();
   // Pinned staging for the first frames of this geometry, so that early
@@ -456,11 +485,25 @@ int mc_server_drain(mc_server *sv, int32_t timeout_ms) {
   MC_CHECK(sv, MC_ERR_ARG, "null server");
   std::unique_lock<std::mutex> lk(sv->mu);
   const auto deadline = Clock::now() + std::chrono::milliseconds(timeout_ms);
-  if (!sv->cv_idle.wait_until(lk, deadline,
-                              [&] { return sv->queue.empty() && sv->in_flight == 0; })) {
+  sv->draining++;
+  sv->cv_work.notify_all();
+  const bool idle = sv->cv_idle.wait_until(
+      lk, deadline, [&] { return sv->queue.empty() && sv->in_flight == 0; });
+  sv->draining--;
+  if (!idle) {
     set_error("drain timed out with %zu queued, %d in flight", sv->queue.size(), sv->in_flight);
     return MC_ERR_TIMEOUT;
   }
+  return MC_OK;
+}
+
+int mc_server_set_batch_window(mc_server *sv, int64_t window_us) {
+  MC_CHECK(sv, MC_ERR_ARG, "null server");
+  MC_CHECK(window_us >= 0 && window_us <= 10000000, MC_ERR_ARG,
+           "batch window %lld us outside 0..10 s", (long long)window_us);
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->window_us = window_us;
+  sv->cv_work.notify_all();
   return MC_OK;
 }
 

@@ -42,7 +42,7 @@ class FrameServer:
 
     def __init__(self, hbm_budget_bytes: int = 64 << 30, max_batch: int = 64,
                  queue_depth: int = 512, keep_results: bool = True,
-                 on_result=None, ctx: Context | None = None):
+                 on_result=None, ctx: Context | None = None, batch_window_ms: float = 0.0):
         self.ctx = ctx or default_context()
         self.keep = keep_results
         self.on_result = on_result
@@ -59,6 +59,13 @@ class FrameServer:
                                      self._cb, None, C.byref(h)))
         self.handle = h
         self._geoms: dict[int, tuple] = {}
+        if batch_window_ms:
+            self.set_batch_window(batch_window_ms)
+
+    def set_batch_window(self, ms: float):
+        """A free slot waits up to ``ms`` after the oldest queued request's
+        arrival for a full batch of its geometry (0: launch at once)."""
+        check(lib().mc_server_set_batch_window(self.handle, int(round(ms * 1000.0))))
 
     def _callback(self, user, rid, geo, cake, lo, status):
         shape = self._shapes[geo]

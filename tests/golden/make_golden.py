@@ -6,7 +6,10 @@ from /root/reference by oracle/Makefile).  Run in the build container:
 
 Fixtures are data only: for tiny geometries the full Map.bin / nMap.bin /
 maskMap.bin content, a seeded frame and the reference cake; for the
-1024x1024 workload, SHA-256 digests of the map arrays and summary values.
+1024x1024 and 2048x2048 workloads, SHA-256 digests of the map arrays and
+summary values.
+
+    python tests/golden/make_golden.py batch_2048   # only the 2048^2 record
 """
 from __future__ import annotations
 
@@ -118,8 +121,45 @@ def make_plain_1024():
     print("plain_1024", rec["nnz"], "records")
 
 
+def make_batch_2048():
+    """The 2048x2048 batching workload: bin membership digests (pairs,
+    flags, the y/z of every record in order), FP64 sums of the areas, and
+    the reference cake and lineout of frame 0 of the seeded generator."""
+    P = synth.batch_2048()
+    frames, dark, mask = synth.frames(P, 1, seed=0, n_rings=12)
+    m = orc.ref_build_map(to_oracle(P), mask=mask)
+    img = frames[0].astype(np.float64) - dark
+    cake, _ = orc.ref_integrate(m, img, P.NrPixelsY, P.NrPixelsZ, bc=(P.BC_y, P.BC_z))
+    lo = orc.lineouts(cake, orc.area_inbounds(m, P.NrPixelsY, P.NrPixelsZ))[0]
+    flat = cake.reshape(-1)
+    rec = {
+        "workload": "batch_2048 (midas_amd.synth.batch_2048, synth.frames seed 0, 12 rings)",
+        "nnz": int(m.entries.size),
+        "sha256_pairs": digest(m.pairs),
+        "sha256_flags": digest(m.flags),
+        "sha256_y": digest(m.entries["y"]),
+        "sha256_z": digest(m.entries["z"]),
+        "sum_frac": float(np.sum(m.entries["frac"], dtype=np.float64)),
+        "sum_area": float(np.sum(m.entries["areaWeight"], dtype=np.float64)),
+        "n_flagged": int(m.flags.sum()),
+        "sha256_mask": digest(mask),
+        "sha256_frame": digest(frames[0]),
+        "sha256_dark": digest(dark),
+        "cake_nansum": float(np.nansum(cake)),
+        "cake_n_nan": int(np.isnan(cake).sum()),
+        "cake_sample": {str(i): float(flat[i]) for i in range(0, flat.size, 49999)},
+        "lineout_cpu_sample": {str(i): float(lo[i]) for i in range(0, lo.size, 131)},
+    }
+    (HERE / "batch_2048.json").write_text(json.dumps(rec, indent=1))
+    print("batch_2048", rec["nnz"], "records")
+
+
 if __name__ == "__main__":
     if orc.ref_lib() is None:
         sys.exit("the reference sources are needed to write the fixtures")
+    if sys.argv[1:] == ["batch_2048"]:
+        make_batch_2048()
+        sys.exit(0)
     make_tiny()
     make_plain_1024()
+    make_batch_2048()

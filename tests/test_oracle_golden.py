@@ -61,3 +61,30 @@ def test_oracle_plain_1024_digests(oracle_lib):
     flat = cake.reshape(-1)
     for k, v in rec["cake_sample"].items():
         assert flat[int(k)] == pytest.approx(v, rel=1e-12, nan_ok=True)
+
+
+@pytest.mark.slow
+def test_oracle_batch_2048_digests(oracle_lib):
+    """The 2048x2048 workload: the oracle's map has the reference's bin
+    membership (pairs, flags, y/z of every record) and the same FP64 area
+    sums; its cake of frame 0 matches the reference cake summary."""
+    from midas_amd import synth
+    rec = json.loads((GOLD / "batch_2048.json").read_text())
+    P = synth.batch_2048()
+    frames, dark, mask = synth.frames(P, 1, seed=0, n_rings=12)
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+    assert sha(frames[0]) == rec["sha256_frame"] and sha(mask) == rec["sha256_mask"]
+    om = oracle_lib.build_map(to_oracle(P), mask=mask)
+    assert om.entries.size == rec["nnz"]
+    assert sha(om.pairs) == rec["sha256_pairs"]
+    assert sha(om.flags) == rec["sha256_flags"]
+    assert sha(om.entries["y"]) == rec["sha256_y"] and sha(om.entries["z"]) == rec["sha256_z"]
+    assert float(np.sum(om.entries["frac"], dtype=np.float64)) == pytest.approx(
+        rec["sum_frac"], rel=1e-13)
+    cake, _ = oracle_lib.integrate(om, frames[0].astype(np.float64) - dark, P.NrPixelsY,
+                                   P.NrPixelsZ, bc=(P.BC_y, P.BC_z))
+    assert int(np.isnan(cake).sum()) == rec["cake_n_nan"]
+    assert float(np.nansum(cake)) == pytest.approx(rec["cake_nansum"], rel=1e-12)
+    flat = cake.reshape(-1)
+    for k, v in rec["cake_sample"].items():
+        assert flat[int(k)] == pytest.approx(v, rel=1e-12, nan_ok=True)
