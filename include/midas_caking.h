@@ -364,6 +364,21 @@ int mc_fit_windows(mc_ctx *ctx, int32_t model, int32_t n_windows,
                    int32_t n_points, const float *x, const float *y,
                    const float *peak_guess, double rstep, int32_t max_iter,
                    double *out, int32_t on_device);
+/* mc_fit_windows without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  x, y,
+ * peak_guess and out are device pointers laid out as for mc_fit_windows.
+ * The kernel's window-relative x values live in a static device array of
+ * 8 Mi floats (32 MiB) per GPU: n_windows * n_points above that is
+ * refused (MC_ERR_ARG).  The fits are queued on the context's stream and
+ * the call returns without waiting (mc_ctx_sync waits).  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (from
+ * contexts with different streams) race; calls on one context's stream
+ * run in order. */
+int mc_fit_windows_noalloc(mc_ctx *ctx, int32_t model, int32_t n_windows,
+                           int32_t n_points, const float *x, const float *y,
+                           const float *peak_guess, double rstep, int32_t max_iter,
+                           double *out);
 /* Cuts windows out of device-resident cakes and fits them in one launch:
  * for every frame, every eta bin and every ring radius ring_r[k] (same
  * units as r_centers, the n_r R-bin centres), the ROI of R bins whose

@@ -141,6 +141,7 @@ SIGNATURES = {
     "mc_graph_launch": (_I, [_V]),
     "mc_graph_destroy": (_I, [_V]),
     "mc_fit_windows": (_I, [_V, _I32, _I32, _I32, _V, _V, _V, _D, _I32, _V, _I32]),
+    "mc_fit_windows_noalloc": (_I, [_V, _I32, _I32, _I32, _V, _V, _V, _D, _I32, _V]),
     "mc_map_fuse": (_I, [_V, _I32, _V, _I32, _P(_V)]),
     "mc_blosc_decompress": (_I, [_V, _I64, _V, _I64, _P(_I64)]),
     "mc_blosc_decompress_many": (_I, [_V, _V, _V, _I64, _I32, _I32, _V]),

@@ -25,6 +25,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "mc_internal.h"
 
@@ -468,6 +469,68 @@ __global__ void k_nonzero(const float *__restrict__ cakes, int n_frames, int n_r
 }  // namespace mc
 
 using namespace mc;
+
+// ---- the allocation-free window fits ----
+// Scratch of mc_fit_windows_noalloc: the window-relative x values of every
+// window, [n_windows][n_points] float, kMaxScratchFloats at most (32 MiB).
+// Each GPU holds its own copy for the life of the process; host code uses
+// only the per-device address from hipGetSymbolAddress, never the name.
+namespace {
+constexpr int64_t kMaxScratchFloats = int64_t(8) << 20;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+static __device__ __align__(256) float scratch[kMaxScratchFloats];
+
+// The device address of this device's scratch array (looked up once per
+// device, with the device current).
+static int scratch_ptr(int device, float **out) {
+  MC_CHECK(device >= 0 && device < kMaxDevices, MC_ERR_ARG, "device %d", device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void *p = nullptr;
+    MC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MC_CHECK(p, MC_ERR_HIP, "no device address for the fit scratch array");
+    g_scratch_dev[device] = static_cast<float *>(p);
+  }
+  *out = g_scratch_dev[device];
+  return MC_OK;
+}
+
+extern "C" int mc_fit_windows_noalloc(mc_ctx *ctx, int32_t model, int32_t n_windows,
+                                      int32_t n_points, const float *x_dev, const float *y_dev,
+                                      const float *guess_dev, double rstep, int32_t max_iter,
+                                      double *out_dev) {
+  MC_CHECK(ctx && x_dev && y_dev && guess_dev && out_dev, MC_ERR_ARG, "null argument");
+  MC_CHECK(model == MC_FIT_PV || model == MC_FIT_TCH || model == MC_FIT_PV2, MC_ERR_ARG,
+           "model %d", model);
+  MC_CHECK(n_windows >= 0 && n_points >= 2 && n_points <= 4096 && max_iter >= 1, MC_ERR_ARG,
+           "bad sizes n_windows=%d n_points=%d", n_windows, n_points);
+  MC_CHECK((int64_t)n_windows * n_points <= kMaxScratchFloats, MC_ERR_ARG,
+           "%d windows of %d points need %lld scratch floats, the allocation-free fit holds "
+           "%lld (use mc_fit_windows)",
+           n_windows, n_points, (long long)n_windows * n_points, (long long)kMaxScratchFloats);
+  if (n_windows == 0) return MC_OK;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  MC_HIP(hipSetDevice(ctx->device));
+  float *xr = nullptr;
+  int rc = scratch_ptr(ctx->device, &xr);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  const unsigned blocks = (unsigned)((n_windows + 63) / 64);
+  if (model == MC_FIT_PV)
+    hipLaunchKernelGGL(k_fit_windows<MC_FIT_PV>, dim3(blocks), dim3(64), 0, s, n_windows,
+                       n_points, x_dev, y_dev, guess_dev, rstep, max_iter, xr, out_dev);
+  else if (model == MC_FIT_PV2)
+    hipLaunchKernelGGL(k_fit_windows<MC_FIT_PV2>, dim3(blocks), dim3(64), 0, s, n_windows,
+                       n_points, x_dev, y_dev, guess_dev, rstep, max_iter, xr, out_dev);
+  else
+    hipLaunchKernelGGL(k_fit_windows<MC_FIT_TCH>, dim3(blocks), dim3(64), 0, s, n_windows,
+                       n_points, x_dev, y_dev, guess_dev, rstep, max_iter, xr, out_dev);
+  MC_HIP(hipGetLastError());
+  return MC_OK;
+}
 
 extern "C" int mc_fit_windows(mc_ctx *ctx, int32_t model, int32_t n_windows, int32_t n_points,
                               const float *x, const float *y, const float *peak_guess,
