@@ -145,6 +145,57 @@ def fit_bench(g, P, frames, log, n_frames=32):
     return res
 
 
+def qmode_bench(log, steps=5, warmup=2):
+    """The Q-mode workload (2880x2880 float32, 11500 x 720 Q x eta bins,
+    SubPixelLevel 2, tilts, parallax, gradient mode): map build on the GPU,
+    64-frame batched passes and the one-frame kernel, with the roofline of
+    the batched launch (frames 4 bytes a pixel)."""
+    import numpy as np
+    import torch
+
+    import midas_amd as ma
+    from midas_amd import synth
+
+    P = synth.qmode_2880()
+    log("qmode_2880: building the map on the GPU")
+    t0 = time.perf_counter()
+    g = ma.build_map(P)
+    t_map = time.perf_counter() - t0
+    nnz = g.info().nnz
+    frames, dark = synth.torch_qmode_frames(P, 64, seed=3)
+    g.set_dark(dark)
+    t0 = time.perf_counter()
+    layout = g.layout_info(mode="gradient", dtype=np.float32)
+    t_layout = time.perf_counter() - t0
+    cakes = torch.empty((64, P.n_r_bins, P.n_eta_bins), dtype=torch.float32, device="cuda")
+    lo = torch.empty((64, 3, P.n_r_bins), dtype=torch.float32, device="cuda")
+    ms_l, ms_i = ma.time_integrate(g, frames, kind="batched", mode="gradient", iters=steps,
+                                   warmup=warmup, cakes_dev=cakes, lineouts_dev=lo)
+    ms_p, _ = ma.time_integrate(g, frames, kind="plain", mode="gradient", iters=1, warmup=1,
+                                cakes_dev=cakes, lineouts_dev=lo)
+    npx, nb = P.NrPixelsY * P.NrPixelsZ, P.n_r_bins * P.n_eta_bins
+    bpf = algorithmic_bytes_per_frame(npx, nb, P.n_r_bins, nnz, 64, frame_bytes=4)
+    achieved = bpf * 64 / (ms_l / 1e3) / 1e9
+    fps = 64 / (ms_i / 1e3)
+    log(f"qmode_2880: {nnz} map records ({t_map:.1f} s), layout {t_layout:.1f} s, "
+        f"{fps:,.0f} frames/s batched ({ms_l * 1e3:.0f} us/launch), "
+        f"one-frame {ms_p * 1e3:.0f} us, roofline {achieved / HBM_PEAK_GBS:.3f}")
+    out = {"workload": f"qmode_2880: 64 x 2880x2880 float32 frames, {P.n_r_bins}x"
+                       f"{P.n_eta_bins} Q x eta bins (Q 0.5-12 by 0.001, eta 0.5 deg), "
+                       f"SubPixelLevel 2, tilts, parallax, gradient mode, map nnz {nnz}",
+           "frames_per_s": fps, "ms_per_step": ms_i, "plain_fps": 1e3 / ms_p,
+           "batched_over_plain": fps * ms_p / 1e3,
+           "kernel_us": {"batched_per_launch": ms_l * 1e3, "plain_per_launch": ms_p * 1e3},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "bytes_per_frame": bpf},
+           "map_build_s": t_map, "layout_s": t_layout, "layout": layout}
+    g.close()
+    del frames, cakes, lo
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +214,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-frames", type=int, default=1500)
     ap.add_argument("--no-fits", action="store_true")
+    ap.add_argument("--no-qmode", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -269,6 +321,7 @@ def main():
     serving = None
     if not args.no_serving:
         serving = serving_bench((500, 2000, 5000), args.serving_frames, log)
+    qmode = None if args.no_qmode else qmode_bench(log)
 
     layout = g.layout_info(args.mode)
     traffic = None
@@ -321,6 +374,7 @@ def main():
         "cpu_baseline": cpu,
         "fits": fits,
         "serving": serving,
+        "qmode_2880": qmode,
         "map_build_s": t_map,
     }
     print(json.dumps(out))

@@ -12,6 +12,14 @@ Configurations (all distances in micrometres, pixels and degrees):
   pilatus     : 1679x1475 int32 (Pilatus 6x8 modules of 243x195, gaps Y
                 1 7 1 7 1, Z 17 as FF_HEDM/Example/Calibration/parameters.txt),
                 gap pixels masked.
+  qmode_2880  : 2880x2880 float32, px 150, Lsd 900 mm, BC (1440.3, 1439.6),
+                tilts ty 0.2 deg, tz -0.15 deg, parallax 200 um, lambda
+                0.172979; Q 0.5-12 A^-1 by 0.001 (11500 bins), eta by 0.5 deg
+                (720 bins), SubPixelLevel 2, GradientCorrection (about 9 map
+                entries per pixel); 16 Gaussian-blurred rings at Q 1.0-8.0.
+  qmode_crop  : a 320x288 crop of qmode_2880 (pixels y 1900-2219, z
+                1300-1587 of the full detector) with the same binning: the
+                case the reference mapper maps in seconds.
 """
 from __future__ import annotations
 
@@ -42,6 +50,74 @@ def pilatus() -> IntegrationParams:
                              EtaMin=-180.0, EtaMax=180.0, EtaBinSize=5.0, Lam=0.172979,
                              NPanelsY=6, NPanelsZ=8, PanelSizeY=243, PanelSizeZ=195,
                              PanelGapsY=[1, 7, 1, 7, 1], PanelGapsZ=[17] * 7)
+
+
+def qmode_2880() -> IntegrationParams:
+    return IntegrationParams(NrPixelsY=2880, NrPixelsZ=2880, pxY=150.0, pxZ=150.0, Lsd=900000.0,
+                             BC_y=1440.3, BC_z=1439.6, ty=0.2, tz=-0.15, Parallax=200.0,
+                             QMin=0.5, QMax=12.0, QBinSize=0.001, Wavelength=0.172979,
+                             Lam=0.172979, EtaMin=-180.0, EtaMax=180.0, EtaBinSize=0.5,
+                             SubPixelLevel=2, GradientCorrection=1, RMin=1.0, RMax=2100.0)
+
+
+QMODE_CROP = (1900, 1300, 320, 288)  # y0, z0, width, height in the full detector
+
+
+def qmode_crop() -> IntegrationParams:
+    P = qmode_2880()
+    y0, z0, w, h = QMODE_CROP
+    P.NrPixelsY, P.NrPixelsZ = w, h
+    P.BC_y, P.BC_z = P.BC_y - y0, P.BC_z - z0
+    return P
+
+
+def ring_q() -> np.ndarray:
+    """The 16 ring positions of the Q-mode workload (A^-1)."""
+    return np.linspace(1.0, 8.0, 16)
+
+
+def qmode_ring_image(P: IntegrationParams, sigma: float = 2.0, amplitude: float = 3000.0):
+    """Noise-free Gaussian-blurred rings at the Q positions of ring_q(),
+    placed at R = Lsd tan(2 theta) / px on a flat detector (the tilts move
+    them by a fraction of a pixel), x (1 + 0.3 cos 2 eta)."""
+    z, y = np.mgrid[0:P.NrPixelsZ, 0:P.NrPixelsY].astype(np.float64)
+    dy, dz = y - P.BC_y, z - P.BC_z
+    R = np.hypot(dy, dz)
+    eta = np.arctan2(dy, dz)
+    img = np.zeros_like(R)
+    for q in ring_q():
+        r0 = P.Lsd * math.tan(2.0 * math.asin(q * P.Wavelength / (4.0 * math.pi))) / P.pxY
+        img += amplitude * np.exp(-0.5 * ((R - r0) / sigma) ** 2)
+    return img * (1.0 + 0.3 * np.cos(2.0 * eta))
+
+
+def qmode_frames(P: IntegrationParams, n: int, seed: int = 0):
+    """(frames [n, NZ, NY] float32, dark [NZ, NY] float32): dark = 100 +
+    N(0, 2); frame = dark + 40 + rings + N(0, sqrt(40 + rings)), seeded per
+    frame."""
+    rng = np.random.default_rng(seed)
+    dark = (100.0 + rng.normal(0.0, 2.0, (P.NrPixelsZ, P.NrPixelsY))).astype(np.float32)
+    base = 40.0 + qmode_ring_image(P)
+    sd = np.sqrt(base)
+    out = np.empty((n, P.NrPixelsZ, P.NrPixelsY), dtype=np.float32)
+    for i in range(n):
+        fr = np.random.default_rng(seed * 1000003 + i + 1)
+        out[i] = dark + base + fr.normal(0.0, 1.0, base.shape) * sd
+    return out, dark
+
+
+def torch_qmode_frames(P: IntegrationParams, n: int, seed: int = 0, device="cuda"):
+    """The same generator on the GPU with torch (for large batches)."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    base = torch.as_tensor(40.0 + qmode_ring_image(P), device=device, dtype=torch.float32)
+    sd = base.sqrt()
+    dark = 100.0 + 2.0 * torch.randn(base.shape, generator=g, device=device)
+    out = torch.empty((n,) + tuple(base.shape), dtype=torch.float32, device=device)
+    for i in range(n):
+        out[i] = dark + base + torch.randn(base.shape, generator=g, device=device) * sd
+    return out, dark
 
 
 def pilatus_gap_mask(P: IntegrationParams) -> np.ndarray:

@@ -10,6 +10,7 @@ maskMap.bin content, a seeded frame and the reference cake; for the
 summary values.
 
     python tests/golden/make_golden.py batch_2048   # only the 2048^2 record
+    python tests/golden/make_golden.py qmode_crop   # only the Q-mode crop record
 """
 from __future__ import annotations
 
@@ -154,12 +155,48 @@ def make_batch_2048():
     print("batch_2048", rec["nnz"], "records")
 
 
+def make_qmode_crop():
+    """The Q-mode workload's binning (Q 0.5-12 by 0.001, eta by 0.5 deg,
+    SubPixelLevel 2, tilts, parallax) on a 320x288 crop of its detector:
+    membership digests, area sums, and the reference's gradient-mode cake of
+    a seeded float32 frame (IntegratorZarrOMP.c:1724-1732)."""
+    P = synth.qmode_crop()
+    frames, dark = synth.qmode_frames(P, 1, seed=0)
+    m = orc.ref_build_map(to_oracle(P))
+    img = frames[0].astype(np.float64) - dark.astype(np.float64)
+    cake, _ = orc.ref_integrate(m, img, P.NrPixelsY, P.NrPixelsZ, bc=(P.BC_y, P.BC_z),
+                                gradient=True)
+    flat = cake.reshape(-1)
+    nz = np.flatnonzero(np.nan_to_num(flat) != 0)
+    rec = {
+        "workload": "qmode_crop (midas_amd.synth.qmode_crop, synth.qmode_frames seed 0, "
+                    "gradient mode)",
+        "nnz": int(m.entries.size),
+        "sha256_pairs": digest(m.pairs),
+        "sha256_flags": digest(m.flags),
+        "sha256_y": digest(m.entries["y"]),
+        "sha256_z": digest(m.entries["z"]),
+        "sum_frac": float(np.sum(m.entries["frac"], dtype=np.float64)),
+        "sum_area": float(np.sum(m.entries["areaWeight"], dtype=np.float64)),
+        "sum_deltaR": float(np.sum(m.entries["deltaR"], dtype=np.float64)),
+        "sha256_frame": digest(frames[0]),
+        "sha256_dark": digest(dark),
+        "cake_nansum": float(np.nansum(cake)),
+        "cake_n_nonzero": int(nz.size),
+        "cake_sample": {str(int(i)): float(flat[i]) for i in nz[::max(1, nz.size // 40)]},
+    }
+    (HERE / "qmode_crop.json").write_text(json.dumps(rec, indent=1))
+    print("qmode_crop", rec["nnz"], "records")
+
+
 if __name__ == "__main__":
     if orc.ref_lib() is None:
         sys.exit("the reference sources are needed to write the fixtures")
-    if sys.argv[1:] == ["batch_2048"]:
-        make_batch_2048()
+    only = {"batch_2048": make_batch_2048, "qmode_crop": make_qmode_crop}
+    if len(sys.argv) == 2 and sys.argv[1] in only:
+        only[sys.argv[1]]()
         sys.exit(0)
     make_tiny()
     make_plain_1024()
     make_batch_2048()
+    make_qmode_crop()
