@@ -218,6 +218,13 @@ int mc_map_set_tuning(mc_map *m, int32_t batch, int32_t rows_per_group,
                       int32_t lds_bytes);
 int mc_map_get_tuning(mc_map *m, int32_t *batch, int32_t *rows_per_group,
                       int32_t *lds_bytes);
+/* Tile shape of the batched kernel: how many bin records (multiple of 16,
+ * 32..1024) and staged map entries (multiple of 64, 256..16384) a tile
+ * holds in LDS; the rest of the LDS budget holds the tile's pixel rows.
+ * 0 keeps a value.  Defaults 256 / 2048.  No counterpart in the reference
+ * (its GPU kernel is one thread per bin, .cu:975-1050). */
+int mc_map_set_tile_shape(mc_map *m, int32_t bins, int32_t entries);
+int mc_map_get_tile_shape(mc_map *m, int32_t *bins, int32_t *entries);
 /* Layout of the batched kernel for (mode, dtype) under the current tuning
  * (built if needed): out[9] = tiles, chunks (workgroups), direct bins,
  * staged pixels per frame, entries, LDS bytes per workgroup, bands,
@@ -233,11 +240,39 @@ int mc_map_autotune(mc_map *m, int32_t dtype, double *best_fps);
  * [n_frames][3][n_r] float32 (rows MC_LINEOUT_*).  *_on_device selects
  * whether each pointer is a device or host address.  n_frames == 1 uses
  * the one-frame kernel; larger counts use the batched kernel in passes of
- * the tuned batch size.  Synchronous. */
+ * the tuned batch size.  Synchronous.
+ * MC_DTYPE_HYBRID: frames holds n_frames messages back to back, each a u32
+ * overflow count C, the uint16 base image and C (i32 pixel, i64 value)
+ * patches (the wire format of IntegratorFitPeaksGPUStream.cu:568-689); they
+ * are decoded on the GPU (.cu:1468-1494) to int32 and integrated as such
+ * (exact in FP32 up to 2^24 counts, as the reference's FP32 conversion). */
 int mc_integrate(mc_map *m, const void *frames, int32_t dtype,
                  int32_t n_frames, int32_t frames_on_device, int32_t mode,
                  uint32_t flags, float *cakes, int32_t cakes_on_device,
                  float *lineouts, int32_t lineouts_on_device);
+/* mc_integrate on device buffers, queued on the map's context stream
+ * without waiting (the stream/event form of IntegratorFitPeaksGPUStream.cu
+ * :2511-2555): the cakes are complete after mc_ctx_sync, after an event
+ * recorded behind the call is done (mc_event_*), or when a host callback
+ * queued behind it runs (mc_ctx_callback).  Calls on one context run in
+ * order.  The first call for a (mode, dtype) builds the map's layout and
+ * may allocate; later calls allocate only when a buffer must grow.  The
+ * hybrid dtype is refused (it reads the messages' counts on the host). */
+int mc_integrate_async(mc_map *m, const void *frames_dev, int32_t dtype,
+                       int32_t n_frames, int32_t mode, uint32_t flags,
+                       float *cakes_dev, float *lineouts_dev);
+/* Completion of work queued on a context's stream: an event recorded at
+ * the current end of the stream, polled (done = 1 when everything queued
+ * before it has finished), waited for, destroyed; or a host function the
+ * runtime calls once everything queued before it has finished (it must not
+ * call into this library). */
+typedef struct mc_event mc_event;
+int mc_event_record(mc_ctx *ctx, mc_event **out);
+int mc_event_query(mc_event *ev, int32_t *done);
+int mc_event_wait(mc_event *ev);
+int mc_event_destroy(mc_event *ev);
+typedef void (*mc_host_fn)(void *user);
+int mc_ctx_callback(mc_ctx *ctx, mc_host_fn fn, void *user);
 
 /* ---- the batched passes captured in a HIP graph ----
  * For a caller that integrates the same device buffers over and over (the

@@ -12,8 +12,9 @@ from .bin_io import (MAP_HEADER_MAGIC, MAP_HEADER_SIZE, PXLIST_DTYPE, MapHeader,
                      canonical_string, header_for, load_map, write_map)
 from .params import (IntegrationParams, bin_edges, parse_integrator_params,
                      parse_mapper_params, parse_params)
-from .caking import (AREA_THRESHOLD, CSRGeometry, Context, build_csr, build_map,
-                     IntegrationGraph, default_context, integrate, integrate_batch,
+from .caking import (AREA_THRESHOLD, CSRGeometry, Context, Event, build_csr, build_map,
+                     IntegrationGraph, default_context, hybrid_message, integrate,
+                     integrate_async, integrate_batch, integrate_hybrid,
                      integrate_with_variance,
                      load_map_gpu, profile_1d, time_integrate)
 from .outputs import ResultFiles
@@ -27,7 +28,8 @@ __all__ = [
     "IntegrationParams", "bin_edges", "parse_integrator_params", "parse_mapper_params",
     "parse_params", "AREA_THRESHOLD", "CSRGeometry", "Context", "build_csr", "build_map",
     "IntegrationGraph", "default_context", "integrate", "integrate_batch",
-    "integrate_with_variance",
+    "integrate_with_variance", "Event", "integrate_async", "integrate_hybrid",
+    "hybrid_message",
     "load_map_gpu", "profile_1d", "time_integrate", "ResultFiles", "gauss_smooth_eta",
     "median_filter_eta", "build_fused_geometry", "enrich_zarr",
 ]

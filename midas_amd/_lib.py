@@ -95,6 +95,7 @@ MC_OUT_LINEOUT, MC_OUT_STREAM, MC_OUT_INT2D, MC_OUT_PEAKS = 0x10, 0x20, 0x40, 0x
 MC_OUT_CAKED_ZARR = 0x100
 
 
+HOST_FN = C.CFUNCTYPE(None, C.c_void_p)
 RESULT_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.c_int32,
                         C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32)
 
@@ -154,6 +155,14 @@ SIGNATURES = {
     "mc_server_acquire_buffer": (_I, [_V, _I32, _P(_V)]),
     "mc_server_submit_buffer": (_I, [_V, _I32, _V, _I64, _I32]),
     "mc_server_release_buffer": (_I, [_V, _I32, _V]),
+    "mc_map_set_tile_shape": (_I, [_V, _I32, _I32]),
+    "mc_map_get_tile_shape": (_I, [_V, _P(_I32), _P(_I32)]),
+    "mc_integrate_async": (_I, [_V, _V, _I32, _I32, _I32, _U32, _V, _V]),
+    "mc_event_record": (_I, [_V, _P(_V)]),
+    "mc_event_query": (_I, [_V, _P(_I32)]),
+    "mc_event_wait": (_I, [_V]),
+    "mc_event_destroy": (_I, [_V]),
+    "mc_ctx_callback": (_I, [_V, HOST_FN, _V]),
     "mc_server_drain": (_I, [_V, _I32]),
     "mc_server_set_batch_window": (_I, [_V, _I64]),
     "mc_server_stats_get": (_I, [_V, _P(ServerStats)]),

@@ -231,6 +231,16 @@ class CSRGeometry:
     def set_tuning(self, batch=0, rows_per_group=0, lds_bytes=0):
         check(lib().mc_map_set_tuning(self.handle, batch, rows_per_group, lds_bytes))
 
+    def set_tile_shape(self, bins=0, entries=0):
+        """Bin records and staged entries a batched-kernel tile holds in LDS
+        (0 keeps a value)."""
+        check(lib().mc_map_set_tile_shape(self.handle, bins, entries))
+
+    def tile_shape(self):
+        b, e = C.c_int32(), C.c_int32()
+        check(lib().mc_map_get_tile_shape(self.handle, C.byref(b), C.byref(e)))
+        return b.value, e.value
+
     def tuning(self):
         b, r, l = C.c_int32(), C.c_int32(), C.c_int32()
         check(lib().mc_map_get_tuning(self.handle, C.byref(b), C.byref(r), C.byref(l)))
@@ -300,8 +310,10 @@ def build_csr(pixmap: PixelMap, *, n_r: int, n_eta: int, n_pixels_y: int, n_pixe
     ``device`` must be a GPU: there is no CPU path."""
     if str(device).startswith("cpu"):
         raise ValueError("midas_amd integrates on the GPU only (device='cuda')")
-    if compute_variance:
-        raise NotImplementedError("variance propagation is not part of this build")
+    # compute_variance: the reference precomputes squared weights for its
+    # variance SpMV; the variance kernel here squares each weight as it
+    # reads it, so every geometry supports integrate_with_variance and the
+    # flag needs no extra buffer.
     if pixmap.n_bins != n_r * n_eta:
         raise ValueError(f"PixelMap has {pixmap.n_bins} bins; n_r * n_eta = {n_r * n_eta}")
     ctx = ctx or default_context()
@@ -362,6 +374,97 @@ def integrate_batch(frames, geom: CSRGeometry, *, mode: str = "bilinear",
     lo = np.empty((F, 3, geom.n_r), dtype=np.float32) if lineouts else None
     check(lib().mc_integrate(geom.handle, a.ctypes.data, code, F, 0, _lib.MODES[mode], fl,
                              cakes.ctypes.data, 0, lo.ctypes.data if lo is not None else None, 0))
+    return cakes, lo
+
+
+class Event:
+    """Completion of the work queued on a context's stream before it
+    (mc_event_*): ``done`` polls, ``wait`` blocks."""
+
+    def __init__(self, ctx: Context):
+        h = C.c_void_p()
+        check(lib().mc_event_record(ctx.handle, C.byref(h)))
+        self.handle = h
+
+    @property
+    def done(self) -> bool:
+        d = C.c_int32()
+        check(lib().mc_event_query(self.handle, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self):
+        check(lib().mc_event_wait(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().mc_event_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def integrate_async(frames, geom: CSRGeometry, cakes, lineouts=None, *, mode: str = "bilinear",
+                    normalize: bool = True, mask_nan: bool = True) -> Event:
+    """Queues the integration of device frames (a torch CUDA tensor
+    [F, NZ, NY]) into device cakes [F, n_r, n_eta] (and lineouts
+    [F, 3, n_r]) on the map's context stream and returns at once; the
+    returned Event completes with it (mc_integrate_async).  The frames must
+    be ready: the caller's torch stream is synchronised first."""
+    if mode not in _lib.MODES:
+        raise ValueError(f"unknown integration mode {mode!r}; expected floor|bilinear|gradient")
+    if not _is_cuda(frames) or not _is_cuda(cakes):
+        raise ValueError("integrate_async takes device tensors (frames, cakes)")
+    t = _torch()
+    npx = geom.n_pixels_y * geom.n_pixels_z
+    frames = frames.contiguous()
+    F = frames.numel() // npx
+    if frames.numel() != F * npx or cakes.numel() != F * geom.n_bins:
+        raise ValueError("frames / cakes sizes do not match the geometry")
+    if lineouts is not None and lineouts.numel() != F * 3 * geom.n_r:
+        raise ValueError("lineouts size does not match the geometry")
+    t.cuda.current_stream(frames.device).synchronize()
+    check(lib().mc_integrate_async(geom.handle, C.c_void_p(frames.data_ptr()),
+                                   _dtype_code(frames), F, _lib.MODES[mode],
+                                   _flags(normalize, mask_nan), C.c_void_p(cakes.data_ptr()),
+                                   C.c_void_p(lineouts.data_ptr()) if lineouts is not None
+                                   else None))
+    return Event(geom.ctx)
+
+
+def hybrid_message(frame: np.ndarray) -> bytes:
+    """One frame in the hybrid wire layout (dtype 6): u32 count, the frame's
+    low 16 bits as uint16, then (i32 pixel, i64 value) for every pixel
+    outside 0..65535."""
+    f = np.ascontiguousarray(frame).reshape(-1).astype(np.int64)
+    over = np.flatnonzero((f < 0) | (f > 65535)).astype(np.int32)
+    base = (f & 0xFFFF).astype(np.uint16)
+    return (np.uint32(over.size).tobytes() + base.tobytes() + over.tobytes()
+            + f[over].astype(np.int64).tobytes())
+
+
+def integrate_hybrid(messages, n_frames: int, geom: CSRGeometry, *, mode: str = "bilinear",
+                     normalize: bool = True, mask_nan: bool = True, lineouts: bool = True):
+    """Cakes (and lineouts) of n_frames hybrid messages back to back
+    (bytes, or a torch CUDA uint8 tensor), decoded on the GPU
+    (mc_integrate with MC_DTYPE_HYBRID)."""
+    if mode not in _lib.MODES:
+        raise ValueError(f"unknown integration mode {mode!r}; expected floor|bilinear|gradient")
+    cakes = np.empty((n_frames, geom.n_r, geom.n_eta), dtype=np.float32)
+    lo = np.empty((n_frames, 3, geom.n_r), dtype=np.float32) if lineouts else None
+    fl = _flags(normalize, mask_nan)
+    if _is_cuda(messages):
+        _torch().cuda.current_stream(messages.device).synchronize()
+        ptr, on_dev = C.c_void_p(messages.data_ptr()), 1
+    else:
+        buf = np.frombuffer(messages, dtype=np.uint8)
+        ptr, on_dev = buf.ctypes.data, 0
+    check(lib().mc_integrate(geom.handle, ptr, _lib.MC_DTYPE_HYBRID, n_frames, on_dev,
+                             _lib.MODES[mode], fl, cakes.ctypes.data, 0,
+                             lo.ctypes.data if lo is not None else None, 0))
     return cakes, lo
 
 

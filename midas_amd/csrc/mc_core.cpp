@@ -227,8 +227,7 @@ int ensure_csr(Map *m, int mode) {
 // LDS of the batched kernel: bin records, staged entries, the staged pixel
 // rows of the pixel-major batch (64 frames per pixel), a zero pixel row, and
 // two pixel-list buffers (the current tile's and the next one's).
-constexpr int kTileBins = 256, kTileEnts = 2048, kBinRecBytes = 32, kWaves = 16, kTileInts = 16,
-              kXtFrames = 64;
+constexpr int kBinRecBytes = 32, kWaves = 16, kTileInts = 16, kXtFrames = 64;
 
 // Tiles of the batched kernel.  Bands of H cake rows; inside a band, tiles
 // are rectangles of rows x eta columns grown column by column while the
@@ -243,8 +242,10 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   int rc = ensure_csr(m, mode);
   if (rc) return rc;
   const int B = m->batch, H = m->rows_per_group;
+  const int kTileBins = m->tile_bins, kTileEnts = m->tile_ents;
   for (auto &t : m->tiles[mode])
-    if (t->batch == B && t->rows == H && t->lds_bytes == m->lds_bytes && t->in_size == in_size) {
+    if (t->batch == B && t->rows == H && t->lds_bytes == m->lds_bytes && t->in_size == in_size &&
+        t->bin_cap == kTileBins && t->ent_cap == kTileEnts) {
       *out = t.get();
       return MC_OK;
     }
@@ -481,6 +482,8 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   for (int x = 0; x < 8; x++) per_xcd = std::max(per_xcd, xcd[x + 1] - xcd[x]);
   auto t = std::make_unique<Tiles>();
   t->batch = B;
+  t->bin_cap = kTileBins;
+  t->ent_cap = kTileEnts;
   t->grid = 8 * per_xcd;
   t->rows = H;
   t->lds_bytes = m->lds_bytes;
@@ -738,6 +741,77 @@ int mc_ctx_sync(mc_ctx *ctx) {
 }
 
 void *mc_ctx_stream(mc_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+struct mc_event {
+  hipEvent_t e = nullptr;
+};
+
+int mc_event_record(mc_ctx *ctx, mc_event **out) {
+  MC_CHECK(ctx && out, MC_ERR_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  MC_HIP(hipSetDevice(ctx->device));
+  auto *ev = new mc_event();
+  hipError_t e = hipEventCreateWithFlags(&ev->e, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(ev->e, ctx->stream);
+  if (e != hipSuccess) {
+    if (ev->e) (void)hipEventDestroy(ev->e);
+    delete ev;
+    set_error("event record: %s", hipGetErrorString(e));
+    return MC_ERR_HIP;
+  }
+  *out = ev;
+  return MC_OK;
+}
+
+int mc_event_query(mc_event *ev, int32_t *done) {
+  MC_CHECK(ev && done, MC_ERR_ARG, "null argument");
+  const hipError_t e = hipEventQuery(ev->e);
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return MC_OK;
+  }
+  MC_HIP(e);
+  *done = 1;
+  return MC_OK;
+}
+
+int mc_event_wait(mc_event *ev) {
+  MC_CHECK(ev, MC_ERR_ARG, "null event");
+  MC_HIP(hipEventSynchronize(ev->e));
+  return MC_OK;
+}
+
+int mc_event_destroy(mc_event *ev) {
+  if (!ev) return MC_OK;
+  (void)hipEventDestroy(ev->e);
+  delete ev;
+  return MC_OK;
+}
+
+namespace {
+struct HostCb {
+  mc_host_fn fn;
+  void *user;
+};
+void run_host_cb(void *p) {
+  std::unique_ptr<HostCb> cb(static_cast<HostCb *>(p));
+  cb->fn(cb->user);
+}
+}  // namespace
+
+int mc_ctx_callback(mc_ctx *ctx, mc_host_fn fn, void *user) {
+  MC_CHECK(ctx && fn, MC_ERR_ARG, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  MC_HIP(hipSetDevice(ctx->device));
+  auto *cb = new HostCb{fn, user};
+  const hipError_t e = hipLaunchHostFunc(ctx->stream, run_host_cb, cb);
+  if (e != hipSuccess) {
+    delete cb;
+    set_error("hipLaunchHostFunc: %s", hipGetErrorString(e));
+    return MC_ERR_HIP;
+  }
+  return MC_OK;
+}
 
 int mc_dev_alloc(mc_ctx *ctx, uint64_t bytes, void **dptr) {
   MC_CHECK(ctx && dptr, MC_ERR_ARG, "null");
@@ -1232,6 +1306,29 @@ int mc_map_set_tuning(mc_map *m, int32_t batch, int32_t rows_per_group, int32_t 
   return MC_OK;
 }
 
+int mc_map_set_tile_shape(mc_map *m, int32_t bins, int32_t entries) {
+  MC_CHECK(m, MC_ERR_ARG, "null map");
+  std::lock_guard<std::mutex> lk(m->mu);
+  if (bins) {
+    MC_CHECK(bins >= 32 && bins <= 1024 && bins % 16 == 0, MC_ERR_ARG,
+             "bins per tile %d not a multiple of 16 in 32..1024", bins);
+    m->tile_bins = bins;
+  }
+  if (entries) {
+    MC_CHECK(entries >= 256 && entries <= 16384 && entries % 64 == 0, MC_ERR_ARG,
+             "entries per tile %d not a multiple of 64 in 256..16384", entries);
+    m->tile_ents = entries;
+  }
+  return MC_OK;
+}
+
+int mc_map_get_tile_shape(mc_map *m, int32_t *bins, int32_t *entries) {
+  MC_CHECK(m, MC_ERR_ARG, "null map");
+  if (bins) *bins = m->tile_bins;
+  if (entries) *entries = m->tile_ents;
+  return MC_OK;
+}
+
 int mc_map_get_tuning(mc_map *m, int32_t *batch, int32_t *rows_per_group, int32_t *lds_bytes) {
   MC_CHECK(m, MC_ERR_ARG, "null map");
   if (batch) *batch = m->batch;
@@ -1257,13 +1354,48 @@ int mc_map_layout_info(mc_map *m, int32_t mode, int32_t dtype, int64_t *out) {
   return MC_OK;
 }
 
+// Hybrid frames: F messages back to back (u32 count, u16 base[npx], i32
+// index[count], i64 value[count]); the counts are read on the host (copied
+// from the device for device messages), the frames decoded on the GPU to
+// int32 and integrated as such.
+static int integrate_hybrid(mc_map *m, const void *msgs, int32_t n_frames, int32_t on_device,
+                            int32_t mode, uint32_t flags, float *cd, float *ld) {
+  hipStream_t s = m->ctx->stream;
+  const int64_t npx = m->npx();
+  std::vector<int64_t> off(n_frames + 1, 0), cnt(n_frames, 0);
+  for (int f = 0; f < n_frames; f++) {
+    uint32_t c = 0;
+    const char *p = (const char *)msgs + off[f];
+    if (on_device) MC_HIP(hipMemcpy(&c, p, 4, hipMemcpyDeviceToHost));
+    else memcpy(&c, p, 4);
+    MC_CHECK((int64_t)c <= npx, MC_ERR_FORMAT, "hybrid frame %d: %u overflow patches for %lld "
+             "pixels", f, c, (long long)npx);
+    cnt[f] = c;
+    off[f + 1] = off[f] + 4 + 2 * npx + 12 * (int64_t)c;
+  }
+  DBuf dmsg, dec;
+  int rc;
+  const char *dm = (const char *)msgs;
+  if (!on_device) {
+    if ((rc = dmsg.alloc(off[n_frames]))) return rc;
+    MC_HIP(hipMemcpyAsync(dmsg.p, msgs, off[n_frames], hipMemcpyHostToDevice, s));
+    dm = dmsg.as<char>();
+  }
+  if ((rc = dec.alloc((size_t)n_frames * npx * 4))) return rc;
+  for (int f = 0; f < n_frames; f++)
+    if ((rc = launch_hybrid_decode(s, dm + off[f], npx, cnt[f], dec.as<int32_t>() + f * npx)))
+      return rc;
+  return run_integrate(m, s, dec.p, MC_DTYPE_I32, n_frames, mode, flags, cd, ld, false, false);
+}
+
 int mc_integrate(mc_map *m, const void *frames, int32_t dtype, int32_t n_frames,
                  int32_t frames_on_device, int32_t mode, uint32_t flags, float *cakes,
                  int32_t cakes_on_device, float *lineouts, int32_t lineouts_on_device) {
   MC_CHECK(m && frames && cakes && n_frames >= 1, MC_ERR_ARG, "bad arguments");
   MC_CHECK(mode >= 0 && mode <= 2, MC_ERR_ARG, "mode %d", mode);
+  const bool hybrid = dtype == MC_DTYPE_HYBRID;
   const size_t es = dtype_size(dtype);
-  MC_CHECK(es > 0, MC_ERR_ARG, "unsupported frame dtype %d", dtype);
+  MC_CHECK(es > 0 || hybrid, MC_ERR_ARG, "unsupported frame dtype %d", dtype);
   std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
   std::lock_guard<std::mutex> lk2(m->mu);
   MC_HIP(hipSetDevice(m->ctx->device));
@@ -1274,7 +1406,7 @@ int mc_integrate(mc_map *m, const void *frames, int32_t dtype, int32_t n_frames,
   DBuf df, dc, dl;
   int rc;
   const void *fd = frames;
-  if (!frames_on_device) {
+  if (!frames_on_device && !hybrid) {
     if ((rc = df.alloc(fbytes))) return rc;
     MC_HIP(hipMemcpyAsync(df.p, frames, fbytes, hipMemcpyHostToDevice, s));
     fd = df.p;
@@ -1289,13 +1421,27 @@ int mc_integrate(mc_map *m, const void *frames, int32_t dtype, int32_t n_frames,
     if ((rc = dl.alloc(lbytes))) return rc;
     ld = dl.as<float>();
   }
-  if ((rc = run_integrate(m, s, fd, dtype, n_frames, mode, flags, cd, ld, false, false)))
-    return rc;
+  rc = hybrid ? integrate_hybrid(m, frames, n_frames, frames_on_device, mode, flags, cd, ld)
+              : run_integrate(m, s, fd, dtype, n_frames, mode, flags, cd, ld, false, false);
+  if (rc) return rc;
   if (!cakes_on_device) MC_HIP(hipMemcpyAsync(cakes, cd, cbytes, hipMemcpyDeviceToHost, s));
   if (lineouts && !lineouts_on_device)
     MC_HIP(hipMemcpyAsync(lineouts, ld, lbytes, hipMemcpyDeviceToHost, s));
   MC_HIP(hipStreamSynchronize(s));
   return MC_OK;
+}
+
+int mc_integrate_async(mc_map *m, const void *frames_dev, int32_t dtype, int32_t n_frames,
+                       int32_t mode, uint32_t flags, float *cakes_dev, float *lineouts_dev) {
+  MC_CHECK(m && frames_dev && cakes_dev && n_frames >= 1, MC_ERR_ARG, "bad arguments");
+  MC_CHECK(mode >= 0 && mode <= 2, MC_ERR_ARG, "mode %d", mode);
+  MC_CHECK(dtype_size(dtype) > 0, MC_ERR_ARG,
+           "unsupported frame dtype %d (hybrid frames: mc_integrate)", dtype);
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  std::lock_guard<std::mutex> lk2(m->mu);
+  MC_HIP(hipSetDevice(m->ctx->device));
+  return run_integrate(m, m->ctx->stream, frames_dev, dtype, n_frames, mode, flags, cakes_dev,
+                       lineouts_dev, false, false);
 }
 
 int mc_integrate_variance(mc_map *m, const void *frames, int32_t dtype, int32_t n_frames,

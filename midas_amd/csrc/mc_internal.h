@@ -100,6 +100,7 @@ struct Tiles {
   bool built = false;
   uint64_t id = 0;   // unique per layout built (captured graphs check it)
   int batch = 0, rows = 0, lds_bytes = 0;  // frames per pass, band rows (16 * G), LDS budget
+  int bin_cap = 0, ent_cap = 0;  // bin records and staged entries a tile holds in LDS
   int in_size = 0;   // bytes per element of the pixel-major batch (2 or 4)
   int n_chunks = 0, n_bands = 0;
   size_t lds_needed = 0;
@@ -172,6 +173,7 @@ struct Map {
   uint64_t binfo_version[12] = {};  // bumped whenever a table is rebuilt
   // tuning
   int batch = 64, rows_per_group = 16, lds_bytes = 81920;
+  int tile_bins = 256, tile_ents = 2048;  // LDS slots of a tile: bin records, entries
   Work work;  // scratch of the calls made through the map's context stream
   int64_t nb() const { return (int64_t)n_r * n_eta; }
   int64_t npx() const { return (int64_t)ny * nz; }
@@ -220,6 +222,10 @@ int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *bi
 size_t batch_xt_bytes(const Map &m, int dtype);
 int launch_profile(hipStream_t s, const Map &m, const float *cakes, int n_frames,
                    int mask_nan, float *lineouts);
+// One hybrid message (u32 count, u16 base[npx], i32 index[count], i64
+// value[count]) in device memory at msg, decoded to int32 out[npx].
+int launch_hybrid_decode(hipStream_t s, const void *msg, int64_t npx, int64_t count,
+                         int32_t *out);
 
 // ---- mapper (mc_mapper.hip) ----
 int map_build_gpu(Ctx *ctx, const mc_geometry *g, const double *mask,

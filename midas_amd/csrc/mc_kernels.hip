@@ -222,6 +222,34 @@ __global__ __launch_bounds__(256) void k_frame_dark(const T *__restrict__ frames
 }
 
 // ---------------------------------------------------------------------
+// Hybrid frames (wire dtype 6, IntegratorFitPeaksGPUStream.cu:1468-1494):
+// a uint16 base image, then count (int32 pixel, int64 value) patches for
+// the pixels whose count overflowed 16 bits.  Decoded to int32 (values
+// clamped to the int32 range).  The patch arrays follow the base image
+// unaligned, so they are read byte by byte.
+// ---------------------------------------------------------------------
+__global__ void k_hybrid_base(const uint16_t *__restrict__ base, int64_t npx,
+                              int32_t *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < npx) out[i] = base[i];
+}
+
+__global__ void k_hybrid_patch(const uint8_t *__restrict__ idx, const uint8_t *__restrict__ val,
+                               int64_t count, int64_t npx, int32_t *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint32_t p = 0;
+  uint64_t v = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) p |= (uint32_t)idx[4 * i + b] << (8 * b);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) v |= (uint64_t)val[8 * i + b] << (8 * b);
+  const int64_t px = (int32_t)p, x = (int64_t)v;
+  if (px >= 0 && px < npx)
+    out[px] = (int32_t)(x > INT32_MAX ? INT32_MAX : (x < INT32_MIN ? INT32_MIN : x));
+}
+
+// ---------------------------------------------------------------------
 // Lineouts of existing cakes: workgroup = one R row of one frame.
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_profile(
@@ -298,8 +326,6 @@ __global__ __launch_bounds__(256) void k_profile(
 // share its L2.
 // ---------------------------------------------------------------------
 constexpr int kWaves = 16;
-constexpr int kTileBins = 256;    // bin records per tile (LDS)
-constexpr int kTileEnts = 2048;   // staged map entries per tile (LDS)
 constexpr int kTileInts = 16;
 constexpr int kXtFrames = 64;     // frames per pixel row of the batch
 
@@ -511,20 +537,19 @@ __device__ __forceinline__ void stage_pixels(const int32_t *__restrict__ tpix, i
 // Stages one tile into LDS with gfx950 global->LDS copies (16 bytes per
 // lane; a wave fills 1 KB of consecutive LDS, so the copies need no
 // registers): the rows of the tile's pixels (its footprint, listed in LDS
-// at ltp), the bin records, the entries (a tile's entries start at a
-// multiple of 8), and the zero pixel row after the staged rows (plain
-// stores; the padding entries point at it).  The caller waits for the
-// copies (s_waitcnt) and then for every wave (barrier).
+// at ltp), the bin records (at lds), the entries (at lent; a tile's entries
+// start at a multiple of 8), and the zero pixel row after the staged rows
+// (plain stores; the padding entries point at it).  The caller waits for
+// the copies (s_waitcnt) and then for every wave (barrier).
 template <typename TL>
 __device__ __forceinline__ void tile_stage(const TileHdr &h, int wave, int lane, char *lds,
+                                           char *lent, char *plane,
                                            const BinRec *__restrict__ recs,
                                            const uint2 *__restrict__ ents,
                                            const TL *__restrict__ xt,
                                            const int32_t *__restrict__ ltp) {
   constexpr int RB = kXtFrames * (int)sizeof(TL);  // bytes per pixel row
   constexpr int VPP = RB / 16;                     // 16-byte vectors per pixel row
-  char *lent = lds + kTileBins * sizeof(BinRec);
-  char *plane = lent + kTileEnts * sizeof(uint2);
   const int jobs = h.npx * VPP;
   for (int ch = wave; ch * 64 < jobs; ch += kWaves) {
     const int j = ch * 64 + lane;
@@ -599,8 +624,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
     const int32_t *__restrict__ chunks, const int32_t *__restrict__ xcd,
     const int32_t *__restrict__ tiles, const BinRec *__restrict__ recs,
     const uint2 *__restrict__ ents, const int32_t *__restrict__ tpix, const TL *__restrict__ xt,
-    const uint16_t *__restrict__ dint, int n_frames, int tp_off, int tp_len, int64_t nb,
-    float *__restrict__ cakes, float *__restrict__ part) {
+    const uint16_t *__restrict__ dint, int n_frames, int ent_off, int plane_off, int tp_off,
+    int tp_len, int64_t nb, float *__restrict__ cakes, float *__restrict__ part) {
   extern __shared__ __attribute__((aligned(16))) char lds_raw[];
   typedef typename Quad<TL>::type QT;
   constexpr int QB = 4 * (int)sizeof(TL);  // bytes of one lane's 4 frames
@@ -608,8 +633,11 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
   constexpr int LG = 8;  // cake bins per granule: 32-byte stores
   typedef typename GranVec<LG>::type VT;
   BinRec *lbin = reinterpret_cast<BinRec *>(lds_raw);
-  uint2 *lent = reinterpret_cast<uint2 *>(lds_raw + kTileBins * sizeof(BinRec));
-  char *plane = lds_raw + kTileBins * sizeof(BinRec) + kTileEnts * sizeof(uint2);
+  // LDS: the tile's bin records, its entries at ent_off, its pixel rows at
+  // plane_off (offsets of the layout's tile shape), the two pixel lists at
+  // tp_off.
+  uint2 *lent = reinterpret_cast<uint2 *>(lds_raw + ent_off);
+  char *plane = lds_raw + plane_off;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int grp = lane >> 4, l16 = lane & 15;
@@ -648,8 +676,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
     const TileHdr h = tile_hdr(tiles, t);
     const int cur = (t - t_beg) & 1;
     if (t > t_beg) __syncthreads();  // every wave is done with the previous tile's LDS
-    tile_stage<TL>(h, wave, lane, lds_raw, recs, ents, xt,
-                   reinterpret_cast<const int32_t *>(lds_raw + tp_off + cur * tp_len));
+    tile_stage<TL>(h, wave, lane, lds_raw, reinterpret_cast<char *>(lent), plane, recs, ents,
+                   xt, reinterpret_cast<const int32_t *>(lds_raw + tp_off + cur * tp_len));
     if (t + 1 < t_end) {
       const int32_t *Tn = tiles + kTileInts * (t + 1);
       stage_pixels(tpix, Tn[0], Tn[1], wave, lane, lds_raw + tp_off + (cur ^ 1) * tp_len);
@@ -1081,6 +1109,21 @@ int launch_profile(hipStream_t s, const Map &m, const float *cakes, int n_frames
   return MC_OK;
 }
 
+int launch_hybrid_decode(hipStream_t s, const void *msg, int64_t npx, int64_t count,
+                         int32_t *out) {
+  const uint8_t *b = static_cast<const uint8_t *>(msg) + 4;  // after the u32 count
+  hipLaunchKernelGGL(k_hybrid_base, dim3(blocks_for(npx, 256)), dim3(256), 0, s,
+                     reinterpret_cast<const uint16_t *>(b), npx, out);
+  MC_HIP(hipGetLastError());
+  if (count > 0) {
+    const uint8_t *idx = b + 2 * npx, *val = idx + 4 * count;
+    hipLaunchKernelGGL(k_hybrid_patch, dim3(blocks_for(count, 256)), dim3(256), 0, s, idx, val,
+                       count, npx, out);
+    MC_HIP(hipGetLastError());
+  }
+  return MC_OK;
+}
+
 int launch_tile_recs(hipStream_t s, const Map &m, Tiles &t, const float2 *binfo) {
   if (!t.recs.p) {
     int rc = t.recs.alloc((size_t)t.n_tbins * sizeof(BinRec));
@@ -1138,7 +1181,9 @@ static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mas
   hipLaunchKernelGGL(kern, dim3((unsigned)t.grid), dim3(64 * kWaves), lds, s,
                      t.chunks.as<int32_t>(), t.xcd.as<int32_t>(), t.tiles.as<int32_t>(),
                      t.recs.as<BinRec>(), t.ents.as<uint2>(), t.tpix.as<int32_t>(),
-                     (const TL *)xt, dhi, n_frames, t.tp_off, t.tp_len, m.nb(), cakes, part);
+                     (const TL *)xt, dhi, n_frames, t.bin_cap * (int)sizeof(BinRec),
+                     t.bin_cap * (int)sizeof(BinRec) + t.ent_cap * 8, t.tp_off, t.tp_len, m.nb(),
+                     cakes, part);
   MC_HIP(hipGetLastError());
   // 3. lineouts
   if (lineouts) {

@@ -146,6 +146,7 @@ class FrameServer:
         srv = socket.create_server((host, port))
         self._tcp = srv
         self._conn_id = 0
+        self.wire_errors = []  # (connection, message index, reason) of refused streams
 
         def client(conn, cid):
             n = 0
@@ -158,6 +159,13 @@ class FrameServer:
                     g = geometry_for_dataset(dataset)
                     geom, dt = self._geoms[g]
                     npx = geom.n_pixels_y * geom.n_pixels_z
+                    why = wire_refusal(dtype, dt)
+                    if why:
+                        # a frame the geometry's pixel type cannot hold
+                        # exactly: refuse the stream rather than wrap or
+                        # truncate its values
+                        self.wire_errors.append((cid, n, why))
+                        return
                     if dtype == 6:
                         raw = _recv_exact(conn, 4)
                         if raw is None:
@@ -224,6 +232,25 @@ def _recv_exact(conn, n):
             return None
         buf += chunk
     return bytes(buf)
+
+
+def wire_refusal(code: int, dt: np.dtype) -> str | None:
+    """Why a wire dtype cannot feed a geometry registered with pixel type
+    dt, or None.  The frame is converted to dt on the host, so the
+    conversion must keep every value: numpy's safe casting for the fixed
+    types; for hybrid frames (uint16 base plus int64 overflow values) a
+    signed or floating type of at least 32 bits."""
+    if code == 6:
+        if dt.kind in "if" and dt.itemsize >= 4:
+            return None
+        return (f"hybrid frames (dtype 6) carry overflow counts above 65535: register the "
+                f"geometry as int32, int64, float32 or float64, not {dt}")
+    if code not in WIRE_NUMPY:
+        return f"unknown wire dtype {code}"
+    src = np.dtype(WIRE_NUMPY[code])
+    if src == dt or np.can_cast(src, dt, casting="safe"):
+        return None
+    return f"wire dtype {code} ({src}) does not convert exactly to the geometry's {dt}"
 
 
 def decode_hybrid(body: bytes, npx: int, count: int) -> np.ndarray:

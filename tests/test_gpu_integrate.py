@@ -246,3 +246,20 @@ def test_unused_pixel_blocks_are_skipped(ny, nz):
         c1, l1 = ma.integrate_batch(frames[i], g, mode="bilinear", mask_nan=True)
         np.testing.assert_allclose(cb[i], c1[0], rtol=3e-6, atol=1e-4, equal_nan=True)
         np.testing.assert_allclose(lb[i], l1[0], rtol=3e-6, atol=1e-4)
+
+
+@pytest.mark.parametrize("bins,ents", [(192, 2560), (128, 1536), (320, 3072)])
+def test_tile_shapes_are_bit_identical(bins, ents):
+    """Tiles of other bin/entry capacities group bins differently but pad
+    each bin's entries the same way: cakes equal the default shape's bit for
+    bit; lineouts sum a row's chunk partials, which other chunks reassociate
+    (a few ulp)."""
+    P, mask, frames, dark = _setup("tilt", n_frames=9, seed=8)
+    g = ma.build_map(P, mask=mask)
+    g.set_dark(dark)
+    cd, ld = ma.integrate_batch(frames, g, mode="bilinear", mask_nan=True)
+    g.set_tile_shape(bins, ents)
+    assert g.tile_shape() == (bins, ents)
+    cs, ls = ma.integrate_batch(frames, g, mode="bilinear", mask_nan=True)
+    np.testing.assert_array_equal(cs, cd)
+    np.testing.assert_allclose(ls, ld, rtol=1e-6, atol=0)
