@@ -387,14 +387,23 @@ int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype,
                         (pf_fit_doublet_pv, PeakFit.c:984-1101); windows
                         only; peak_guess holds 2 values per window (R1, R2);
                         out: R1, R2, SNR1, SNR2, Gamma1, Gamma2, mu, status */
+/* TCH with N = 2..MC_FIT_MAX_PEAKS peaks on one window: a merged fitPeaks
+ * job (PeakFit.c:321-351, 356-544), 4 params per peak + the 2 background
+ * terms; N = 2 is the TCH doublet of the per-eta fits (:1138-1170). */
+#define MC_FIT_MAX_PEAKS 8
+#define MC_FIT_TCH_PEAKS(n) ((n) == 1 ? MC_FIT_TCH : (n) + 1)
+#define MC_FIT_TCH2 MC_FIT_TCH_PEAKS(2)
 #define MC_FIT_OUT 8 /* per window: 7 PF_PARAMS_PER_PEAK values + status */
 
 /* Fits n_windows windows of n_points samples each.  x, y are
- * [n_windows][n_points] float32 and peak_guess [n_windows] (the window's
- * expected centre); rstep is the sample spacing.  out is
- * [n_windows][MC_FIT_OUT] double: area, center, sig, gam, FWHM, eta, chi2
- * (chi2/dof for TCH, SNR for pV, as the reference reports them), status
- * (iterations, negative on failure).  Pointers are host or device. */
+ * [n_windows][n_points] float32 and peak_guess [n_windows][k] (the window's
+ * k expected centres: k = 1 for PV and TCH, 2 for PV2, N for
+ * MC_FIT_TCH_PEAKS(N)); rstep is the sample spacing.  out is
+ * [n_windows][MC_FIT_OUT] double (PV, TCH, PV2) or [n_windows][N]
+ * [MC_FIT_OUT] (TCH with N peaks, one row per peak in guess order): area, center,
+ * sig, gam, FWHM, eta, chi2 (chi2/dof for TCH, SNR for pV, as the
+ * reference reports them), status (iterations, negative on failure).
+ * Pointers are host or device. */
 int mc_fit_windows(mc_ctx *ctx, int32_t model, int32_t n_windows,
                    int32_t n_points, const float *x, const float *y,
                    const float *peak_guess, double rstep, int32_t max_iter,

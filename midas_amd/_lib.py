@@ -19,6 +19,12 @@ MC_DTYPE_F32, MC_DTYPE_F64, MC_DTYPE_HYBRID, MC_DTYPE_I32 = 4, 5, 6, 7
 MC_MODE_FLOOR, MC_MODE_BILINEAR, MC_MODE_GRADIENT = 0, 1, 2
 MC_F_NORMALIZE, MC_F_MASK_NAN = 1, 2
 MC_FIT_PV, MC_FIT_TCH, MC_FIT_PV2, MC_FIT_OUT = 0, 1, 2, 8
+MC_FIT_MAX_PEAKS = 8
+
+
+def MC_FIT_TCH_PEAKS(n: int) -> int:
+    """Model code of the TCH fit with n peaks on one window."""
+    return MC_FIT_TCH if n == 1 else n + 1
 
 MODES = {"floor": MC_MODE_FLOOR, "bilinear": MC_MODE_BILINEAR, "gradient": MC_MODE_GRADIENT}
 

@@ -15,12 +15,24 @@
 //        params [R1, R2, mu, Gamma1, Imax1, Gamma2, Imax2, BG], start and
 //        bounds of pf_fit_doublet_pv (:1037-1078): R1 in [x0, Rmid], R2 in
 //        [Rmid, x_last], Rmid the guesses' midpoint.
+//   TCH with N = 2..8 peaks (MC_FIT_TCH_PEAKS(N)): a merged fitPeaks job
+//        (:321-351, 356-544): N TCH peaks [A, center, sig, gam] and one
+//        Chebyshev background [bg0, bg1] (4N + 2 params), each peak started
+//        and bounded as fitPeaks does (:380-445) at the window sample
+//        nearest its guess, the background from the first peak's estimate;
+//        the TCH doublet of the per-eta fits (pf_fit_doublet_tch,
+//        :1138-1170) is N = 2 over the whole window.
 // The objective is the reference's: the sum of squared residuals.  The
 // reference minimises it with NLopt (L-BFGS, Nelder-Mead fallback); this
 // kernel uses Levenberg-Marquardt with Marquardt damping and box bounds by
-// projection.  Residuals and Jacobians are FP32, evaluated on coordinates
-// relative to the window's first sample; the normal equations are
-// accumulated and solved (Cholesky) in FP64.
+// projection; a parameter on a bound whose descent direction leaves the box
+// is held there for the step (active set).  The TCH peaks are fitted in
+// u = sqrt(sig) in place of sig (the Gaussian FWHM is linear in u, while
+// dF/dsig grows without bound as sig -> 0 and stalls the damped steps near
+// a Lorentzian shape); bounds are the square roots of the sig bounds and
+// results are reported in sig.  Residuals and Jacobians are FP32, evaluated
+// on coordinates relative to the window's first sample; the normal
+// equations are accumulated and solved (Cholesky) in FP64.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -32,8 +44,21 @@
 namespace mc {
 namespace {
 
+// Peaks of a TCH-family model (1 for TCH, N for TCH_PEAKS(N); 0 otherwise).
+template <int MODEL> constexpr int tch_peaks() {
+  return MODEL == MC_FIT_TCH ? 1
+         : (MODEL >= MC_FIT_TCH2 && MODEL <= MC_FIT_TCH_PEAKS(MC_FIT_MAX_PEAKS)) ? MODEL - 1
+                                                                                  : 0;
+}
 template <int MODEL> constexpr int n_params() {
-  return MODEL == MC_FIT_PV ? 5 : MODEL == MC_FIT_TCH ? 6 : 8;
+  return MODEL == MC_FIT_PV ? 5 : MODEL == MC_FIT_PV2 ? 8 : 4 * tch_peaks<MODEL>() + 2;
+}
+// Guesses a window takes and result rows it writes (MC_FIT_OUT each).
+template <int MODEL> constexpr int n_guesses() {
+  return MODEL == MC_FIT_PV ? 1 : MODEL == MC_FIT_PV2 ? 2 : tch_peaks<MODEL>();
+}
+template <int MODEL> constexpr int n_rows() {
+  return MODEL == MC_FIT_PV || MODEL == MC_FIT_PV2 ? 1 : tch_peaks<MODEL>();
 }
 constexpr float kLn2 = 0.69314718055994530942f;
 
@@ -94,14 +119,15 @@ __device__ __forceinline__ float pv2_eval(const double *p, float xr, float c1, f
   return bg + i1 * (mu * L1 + (1.0f - mu) * G1) + i2 * (mu * L2 + (1.0f - mu) * G2);
 }
 
-// TCH width/mixing with derivatives (PeakFit.c:17-42).
+// TCH width/mixing with derivatives (PeakFit.c:17-42), in u = sqrt(sig).
 struct Tch {
-  float F, eta, dF_dsig, dF_dgam, deta_dsig, deta_dgam;
+  float F, eta, dF_du, dF_dgam, deta_du, deta_dgam;
 };
 
-__device__ Tch tch(double sig_d, double gam_d) {
-  const float sig = (float)fmax(sig_d, 1e-12), gam = (float)fmax(gam_d, 1e-6);
-  const float fg = sqrtf(fmaxf(8.0f * kLn2 * sig, 0.0f)) / 100.0f;
+__device__ Tch tch(double u_d, double gam_d) {
+  const float u = (float)fmax(u_d, 1e-6), gam = (float)fmax(gam_d, 1e-6);
+  const float kfg = 0.0235482004503094933f;  // sqrt(8 ln 2) / 100
+  const float fg = kfg * u;
   const float fl = gam / 100.0f;
   const float g2 = fg * fg, g3 = g2 * fg, g4 = g3 * fg, g5 = g4 * fg;
   const float l2 = fl * fl, l3 = l2 * fl, l4 = l3 * fl, l5 = l4 * fl;
@@ -114,7 +140,6 @@ __device__ Tch tch(double sig_d, double gam_d) {
   const float dP_dfl = 2.69269f * g4 + 2.0f * 2.42843f * g3 * fl + 3.0f * 4.47163f * g2 * l2 +
                        4.0f * 0.07842f * fg * l3 + 5.0f * l4;
   const float dF_dP = t.F / (5.0f * fmaxf(P5, 1e-30f));
-  const float dfg_dsig = fg > 0.f ? 4.0f * kLn2 / (1e4f * fg) : 0.f;
   const float dfl_dgam = 0.01f;
   const float dF_dfg = dF_dP * dP_dfg, dF_dfl = dF_dP * dP_dfl;
   const float q = fl / t.F;
@@ -125,15 +150,17 @@ __device__ Tch tch(double sig_d, double gam_d) {
   t.eta = eta;
   const float dq_dfg = -fl / (t.F * t.F) * dF_dfg;
   const float dq_dfl = 1.0f / t.F - fl / (t.F * t.F) * dF_dfl;
-  t.dF_dsig = dF_dfg * dfg_dsig;
+  t.dF_du = dF_dfg * kfg;
   t.dF_dgam = dF_dfl * dfl_dgam;
-  t.deta_dsig = deta_dq * dq_dfg * dfg_dsig;
+  t.deta_du = deta_dq * dq_dfg * kfg;
   t.deta_dgam = deta_dq * dq_dfl * dfl_dgam;
   return t;
 }
 
-__device__ __forceinline__ float tch_eval(const double *p, const Tch &t, float xr, float cr,
-                                          float xn, float *J) {
+// One TCH peak (params p[0..3] = A, center, u = sqrt(sig), gam): A * shape
+// and, with J, its 4 derivatives.
+__device__ __forceinline__ float tch_peak(const double *p, const Tch &t, float xr, float cr,
+                                          float *J) {
   const float A = (float)p[0];
   const float F = t.F;
   const float k = 1.0f / (2.0f * sqrtf(2.0f * kLn2));
@@ -153,23 +180,27 @@ __device__ __forceinline__ float tch_eval(const double *p, const Tch &t, float x
     const float dL_dc = L * 2.0f * dx / D;
     J[0] = shape;
     J[1] = A * (t.eta * dL_dc + (1.0f - t.eta) * dG_dc);
-    J[2] = A * (ds_dF * t.dF_dsig + ds_deta * t.deta_dsig);
+    J[2] = A * (ds_dF * t.dF_du + ds_deta * t.deta_du);
     J[3] = A * (ds_dF * t.dF_dgam + ds_deta * t.deta_dgam);
-    J[4] = 1.0f;
-    J[5] = xn;
   }
-  return (float)p[4] + (float)p[5] * xn + A * shape;
+  return A * shape;
 }
 
 // Sum of squared residuals and (optionally) the FP64 normal equations.
 template <int MODEL>
 __device__ double chi2_normal(const Win &w, const double *p, double *JtJ, double *Jtr) {
   constexpr int NP = n_params<MODEL>();
-  const float cr = (float)(p[MODEL == MC_FIT_TCH ? 1 : 0] - w.x0);
+  constexpr int NPK = tch_peaks<MODEL>();
+  const float cr = (float)(p[0] - w.x0);
   const float cr2 = MODEL == MC_FIT_PV2 ? (float)(p[1] - w.x0) : 0.0f;
   const float xrange = fmaxf(w.xr[w.n - 1] - w.xr[0], 1e-12f);
-  Tch t;
-  if (MODEL == MC_FIT_TCH) t = tch(p[2], p[3]);
+  Tch t[NPK > 0 ? NPK : 1];
+  float tc[NPK > 0 ? NPK : 1];
+#pragma unroll
+  for (int k = 0; k < NPK; k++) {
+    t[k] = tch(p[4 * k + 2], p[4 * k + 3]);
+    tc[k] = (float)(p[4 * k + 1] - w.x0);
+  }
   if (JtJ) {
     for (int a = 0; a < NP * NP; a++) JtJ[a] = 0;
     for (int a = 0; a < NP; a++) Jtr[a] = 0;
@@ -181,9 +212,20 @@ __device__ double chi2_normal(const Win &w, const double *p, double *JtJ, double
     const float xr = w.xr[i];
     float J[NP];
     float f;
-    if (MODEL == MC_FIT_PV) f = pv_eval(p, xr, cr, JtJ ? J : nullptr);
-    else if (MODEL == MC_FIT_PV2) f = pv2_eval(p, xr, cr, cr2, JtJ ? J : nullptr);
-    else f = tch_eval(p, t, xr, cr, 2.0f * (xr - w.xr[0]) / xrange - 1.0f, JtJ ? J : nullptr);
+    if (MODEL == MC_FIT_PV) {
+      f = pv_eval(p, xr, cr, JtJ ? J : nullptr);
+    } else if (MODEL == MC_FIT_PV2) {
+      f = pv2_eval(p, xr, cr, cr2, JtJ ? J : nullptr);
+    } else {  // TCH family: Chebyshev T0 + T1 background plus the peaks
+      const float xn = 2.0f * (xr - w.xr[0]) / xrange - 1.0f;
+      f = (float)p[4 * NPK] + (float)p[4 * NPK + 1] * xn;
+#pragma unroll
+      for (int k = 0; k < NPK; k++) f += tch_peak(p + 4 * k, t[k], xr, tc[k], JtJ ? J + 4 * k : nullptr);
+      if (JtJ) {
+        J[4 * NPK] = 1.0f;
+        J[4 * NPK + 1] = xn;
+      }
+    }
     const float r = yi - f;
     chi += (double)r * r;
     if (JtJ) {
@@ -224,11 +266,14 @@ template <int NP> __device__ bool cholesky_solve(double *A, double *b) {
   return true;
 }
 
-// Start values and bounds (see the file comment).
+// Start values and bounds (see the file comment).  guess: the window's
+// n_guesses<MODEL>() expected centres.
 template <int MODEL>
-__device__ void init_params(const Win &w, double rstep, double rmean, double rmean2, double *p,
+__device__ void init_params(const Win &w, double rstep, const double *guess, double *p,
                             double *lo, double *hi) {
   const int n = w.n;
+  const double rmean = guess[0];
+  const double rmean2 = MODEL == MC_FIT_PV2 ? (double)guess[1] : 0.0;
   if (MODEL == MC_FIT_PV2) {  // pf_fit_doublet_pv, PeakFit.c:1046-1066
     double bg = ((double)w.y[0] + w.y[n - 1]) / 2;
     if (bg < 0) bg = 0;
@@ -259,10 +304,8 @@ __device__ void init_params(const Win &w, double rstep, double rmean, double rme
     p[4] = bg; lo[4] = 0; hi[4] = (bg > 0) ? bg * 1.5 : mx * 0.5;
     return;
   }
-  int pk = 0;
-  double best = 1e10;
-  for (int i = 0; i < n; i++)
-    if (fabs(w.xa(i) - rmean) < best) { best = fabs(w.xa(i) - rmean); pk = i; }
+  // TCH family: fitPeaks' per-peak start and bounds (PeakFit.c:380-437)
+  constexpr int NPK = tch_peaks<MODEL>();
   double mx = -1e30, mn = 1e30;
   for (int i = 0; i < n; i++) { mx = fmax(mx, (double)w.y[i]); mn = fmin(mn, (double)w.y[i]); }
   const double range = fmax(mx - mn, 1.0);
@@ -270,54 +313,73 @@ __device__ void init_params(const Win &w, double rstep, double rmean, double rme
   if (bw < 1) bw = 1;
   double bs = 0;
   for (int i = 0; i < bw; i++) bs += (double)w.y[i] + (double)w.y[n - 1 - i];
-  const double bgg = bs / (2.0 * bw);
-  double amp = w.y[pk] - bgg;
-  if (amp <= 0) amp = w.y[pk];
-  const double half = bgg + amp / 2.0;
-  int l = pk, r = pk;
-  while (l > 0 && w.y[l] > half) l--;
-  while (r < n - 1 && w.y[r] > half) r++;
-  double fwb = (double)(r - l);
-  if (!(fwb > 1.0)) fwb = 2.0;
-  const double fw = fwb * rstep, fgi = fw * 0.5;
-  double sig0 = (fgi * 100.0) * (fgi * 100.0) / (8.0 * log(2.0));
-  if (sig0 < 1e-6) sig0 = 1.0;
-  double gam0 = fw * 0.5 * 100.0;
-  if (gam0 < 0.01) gam0 = 1.0;
-  double a0 = amp * fw * 1.5;
-  if (a0 <= 0) a0 = range * rstep;
-  const double crange = fmax(fw, (double)n * rstep * 0.5);
-  double ahi = range * (double)n * rstep * 2.0;
-  if (ahi < 1.0) ahi = 1.0;
-  const double shi = (rstep * 200.0 * 100.0) * (rstep * 200.0 * 100.0) / (8.0 * log(2.0));
-  const double ghi = rstep * 200.0 * 100.0;
-  p[0] = fmax(fmin(a0, ahi * 0.9), 0.01); lo[0] = 0; hi[0] = ahi;
-  p[1] = w.xa(pk); lo[1] = w.xa(pk) - crange; hi[1] = w.xa(pk) + crange;
-  p[2] = fmax(fmin(sig0, shi * 0.5), 2e-6); lo[2] = 1e-6; hi[2] = shi;
-  p[3] = fmax(fmin(gam0, ghi * 0.5), 0.02); lo[3] = 0.01; hi[3] = ghi;
-  for (int k = 0; k < 4; k++) {
-    if (p[k] <= lo[k]) p[k] = lo[k] + (hi[k] - lo[k]) * 0.01;
-    if (p[k] >= hi[k]) p[k] = hi[k] - (hi[k] - lo[k]) * 0.01;
+  const double bgg = bs / (2.0 * bw);  // pf_estimate_initial_params' background (:215-243)
+  for (int k = 0; k < NPK; k++) {
+    int pk = 0;
+    double best = 1e10;
+    for (int i = 0; i < n; i++)
+      if (fabs(w.xa(i) - (double)guess[k]) < best) {
+        best = fabs(w.xa(i) - (double)guess[k]);
+        pk = i;
+      }
+    double amp = w.y[pk] - bgg;
+    if (amp <= 0) amp = w.y[pk];
+    const double half = bgg + amp / 2.0;
+    int l = pk, r = pk;
+    while (l > 0 && w.y[l] > half) l--;
+    while (r < n - 1 && w.y[r] > half) r++;
+    double fwb = (double)(r - l);
+    if (!(fwb > 1.0)) fwb = 2.0;
+    const double fw = fwb * rstep, fgi = fw * 0.5;
+    double sig0 = (fgi * 100.0) * (fgi * 100.0) / (8.0 * log(2.0));
+    if (sig0 < 1e-6) sig0 = 1.0;
+    double gam0 = fw * 0.5 * 100.0;
+    if (gam0 < 0.01) gam0 = 1.0;
+    double a0 = amp * fw * 1.5;
+    if (a0 <= 0) a0 = range * rstep;
+    const double crange = fmax(fw, (double)n * rstep * 0.5);
+    double ahi = range * (double)n * rstep * 2.0;
+    if (ahi < 1.0) ahi = 1.0;
+    const double shi = (rstep * 200.0 * 100.0) * (rstep * 200.0 * 100.0) / (8.0 * log(2.0));
+    const double ghi = rstep * 200.0 * 100.0;
+    double *q = p + 4 * k, *ql = lo + 4 * k, *qh = hi + 4 * k;
+    q[0] = fmax(fmin(a0, ahi * 0.9), 0.01); ql[0] = 0; qh[0] = ahi;
+    q[1] = w.xa(pk); ql[1] = w.xa(pk) - crange; qh[1] = w.xa(pk) + crange;
+    q[2] = fmax(fmin(sig0, shi * 0.5), 2e-6); ql[2] = 1e-6; qh[2] = shi;
+    q[3] = fmax(fmin(gam0, ghi * 0.5), 0.02); ql[3] = 0.01; qh[3] = ghi;
+    for (int j = 0; j < 4; j++) {
+      if (q[j] <= ql[j]) q[j] = ql[j] + (qh[j] - ql[j]) * 0.01;
+      if (q[j] >= qh[j]) q[j] = qh[j] - (qh[j] - ql[j]) * 0.01;
+    }
   }
-  p[4] = bgg; lo[4] = fmin(mn - range * 0.5, -1.0); hi[4] = mx + range;
-  if (p[4] <= lo[4]) p[4] = lo[4] + 0.01;
-  if (p[4] >= hi[4]) p[4] = hi[4] - 0.01;
-  p[5] = 0.0; lo[5] = -range; hi[5] = range;
+  double *b = p + 4 * NPK, *bl = lo + 4 * NPK, *bh = hi + 4 * NPK;
+  b[0] = bgg; bl[0] = fmin(mn - range * 0.5, -1.0); bh[0] = mx + range;
+  if (b[0] <= bl[0]) b[0] = bl[0] + 0.01;
+  if (b[0] >= bh[0]) b[0] = bh[0] - 0.01;
+  b[1] = 0.0; bl[1] = -range; bh[1] = range;
 }
 
 template <int MODEL>
-__device__ void fit_one(const Win &w, double rstep, double rmean, double rmean2, int max_iter,
+__device__ void fit_one(const Win &w, double rstep, const double *guess, int max_iter,
                         double *out) {
   constexpr int NP = n_params<MODEL>();
+  constexpr int NR = n_rows<MODEL>();
   double p[NP], lo[NP], hi[NP];
-  for (int k = 0; k < 8; k++) out[k] = 0;
+  for (int k = 0; k < MC_FIT_OUT * NR; k++) out[k] = 0;
   int valid = 0;
   for (int i = 0; i < w.n; i++) valid += !isnan(w.y[i]);
   if (w.n < NP || valid < NP) {
-    out[7] = -1;
+    for (int k = 0; k < NR; k++) out[MC_FIT_OUT * k + 7] = -1;
     return;
   }
-  init_params<MODEL>(w, rstep, rmean, rmean2, p, lo, hi);
+  init_params<MODEL>(w, rstep, guess, p, lo, hi);
+  constexpr int NPK = tch_peaks<MODEL>();
+#pragma unroll
+  for (int k = 0; k < NPK; k++) {  // sig -> u = sqrt(sig)
+    p[4 * k + 2] = sqrt(p[4 * k + 2]);
+    lo[4 * k + 2] = sqrt(lo[4 * k + 2]);
+    hi[4 * k + 2] = sqrt(hi[4 * k + 2]);
+  }
   double JtJ[NP * NP], Jtr[NP], M[NP * NP], d[NP], q[NP];
   double chi = chi2_normal<MODEL>(w, p, JtJ, Jtr);
   double lam = 1e-3;
@@ -332,6 +394,13 @@ __device__ void fit_one(const Win &w, double rstep, double rmean, double rmean2,
         }
         M[a * NP + a] += lam * fmax(JtJ[a * NP + a], 1e-30);
         d[a] = Jtr[a];
+      }
+      for (int a = 0; a < NP; a++) {  // active set: hold a bound that pushes outwards
+        if ((p[a] <= lo[a] && Jtr[a] < 0) || (p[a] >= hi[a] && Jtr[a] > 0)) {
+          for (int b = 0; b < NP; b++) M[a * NP + b] = M[b * NP + a] = 0;
+          M[a * NP + a] = 1;
+          d[a] = 0;
+        }
       }
       if (!cholesky_solve<NP>(M, d)) {
         lam *= 10;
@@ -360,7 +429,7 @@ __device__ void fit_one(const Win &w, double rstep, double rmean, double rmean2,
     if (!accepted || stall >= 2) break;
   }
   if (!isfinite(chi)) {
-    out[7] = -2;
+    for (int k = 0; k < NR; k++) out[MC_FIT_OUT * k + 7] = -2;
     return;
   }
   const int n_used = valid;
@@ -381,15 +450,21 @@ __device__ void fit_one(const Win &w, double rstep, double rmean, double rmean2,
     out[5] = mu;
     const double rms = sqrt(chi / n_used);
     out[6] = rms > 0 ? imax / rms : 1.0;  // SNR, pf_fit_singlet_pv :973-974
-  } else {
-    const Tch t = tch(p[2], p[3]);
-    out[0] = p[0];
-    out[1] = p[1];
-    out[2] = p[2];
-    out[3] = p[3];
-    out[4] = t.F;
-    out[5] = t.eta;
-    out[6] = chi / fmax((double)(n_used - NP), 1.0);  // chi^2 / dof, PeakFit.c:505
+  } else {  // fitPeaks' rows, one per peak (PeakFit.c:505-527)
+    const double dof_chi = chi / fmax((double)(n_used - NP), 1.0);  // chi^2 / dof
+    for (int k = 0; k < NR; k++) {
+      const Tch t = tch(p[4 * k + 2], p[4 * k + 3]);
+      double *o = out + MC_FIT_OUT * k;
+      o[0] = p[4 * k];
+      o[1] = p[4 * k + 1];
+      o[2] = p[4 * k + 2] * p[4 * k + 2];
+      o[3] = p[4 * k + 3];
+      o[4] = t.F;
+      o[5] = t.eta;
+      o[6] = dof_chi;
+      o[7] = it + 1;
+    }
+    return;
   }
   out[7] = it + 1;
 }
@@ -411,11 +486,9 @@ __global__ __launch_bounds__(64) void k_fit_windows(int nw, int npts, const floa
   w.y = y + (int64_t)i * npts;
   w.n = npts;
   w.x0 = x0;
-  if (MODEL == MC_FIT_PV2)
-    fit_one<MODEL>(w, rstep, guess[2 * i], guess[2 * i + 1], max_iter,
-                   out + (int64_t)i * MC_FIT_OUT);
-  else
-    fit_one<MODEL>(w, rstep, guess[i], 0.0, max_iter, out + (int64_t)i * MC_FIT_OUT);
+  double g[n_guesses<MODEL>()];
+  for (int k = 0; k < n_guesses<MODEL>(); k++) g[k] = guess[(int64_t)n_guesses<MODEL>() * i + k];
+  fit_one<MODEL>(w, rstep, g, max_iter, out + (int64_t)i * MC_FIT_OUT * n_rows<MODEL>());
 }
 
 // Windows cut from cakes: one thread per (frame, eta bin, ring).  The
@@ -451,7 +524,7 @@ __global__ __launch_bounds__(64) void k_fit_rings(
   w.y = ys;
   w.n = n;
   w.x0 = x0s[k];
-  fit_one<MODEL>(w, rstep, ring[k], 0.0, max_iter, o);
+  fit_one<MODEL>(w, rstep, ring + k, max_iter, o);
 }
 
 __global__ void k_nonzero(const float *__restrict__ cakes, int n_frames, int n_r, int n_eta,
@@ -498,13 +571,47 @@ static int scratch_ptr(int device, float **out) {
   return MC_OK;
 }
 
+// Guesses a window of a model takes, result rows it writes; 0 for an
+// unknown model.
+static int fit_guesses(int model) {
+  if (model == MC_FIT_PV || model == MC_FIT_TCH) return 1;
+  if (model == MC_FIT_PV2) return 2;
+  if (model >= MC_FIT_TCH2 && model <= MC_FIT_TCH_PEAKS(MC_FIT_MAX_PEAKS)) return model - 1;
+  return 0;
+}
+static int fit_rows(int model) { return model == MC_FIT_PV2 ? 1 : fit_guesses(model); }
+
+static int launch_fit_windows(hipStream_t s, int model, int nw, int np, const float *x,
+                              const float *y, const float *g, double rstep, int max_iter,
+                              float *xr, double *out) {
+  const unsigned blocks = (unsigned)((nw + 63) / 64);
+#define MC_FIT_LAUNCH(M)                                                                  \
+  hipLaunchKernelGGL(k_fit_windows<M>, dim3(blocks), dim3(64), 0, s, nw, np, x, y, g,  \
+                     rstep, max_iter, xr, out)
+  switch (model) {
+    case MC_FIT_PV: MC_FIT_LAUNCH(MC_FIT_PV); break;
+    case MC_FIT_PV2: MC_FIT_LAUNCH(MC_FIT_PV2); break;
+    case MC_FIT_TCH: MC_FIT_LAUNCH(MC_FIT_TCH); break;
+    case MC_FIT_TCH_PEAKS(2): MC_FIT_LAUNCH(MC_FIT_TCH_PEAKS(2)); break;
+    case MC_FIT_TCH_PEAKS(3): MC_FIT_LAUNCH(MC_FIT_TCH_PEAKS(3)); break;
+    case MC_FIT_TCH_PEAKS(4): MC_FIT_LAUNCH(MC_FIT_TCH_PEAKS(4)); break;
+    case MC_FIT_TCH_PEAKS(5): MC_FIT_LAUNCH(MC_FIT_TCH_PEAKS(5)); break;
+    case MC_FIT_TCH_PEAKS(6): MC_FIT_LAUNCH(MC_FIT_TCH_PEAKS(6)); break;
+    case MC_FIT_TCH_PEAKS(7): MC_FIT_LAUNCH(MC_FIT_TCH_PEAKS(7)); break;
+    case MC_FIT_TCH_PEAKS(8): MC_FIT_LAUNCH(MC_FIT_TCH_PEAKS(8)); break;
+    default: set_error("model %d", model); return MC_ERR_ARG;
+  }
+#undef MC_FIT_LAUNCH
+  MC_HIP(hipGetLastError());
+  return MC_OK;
+}
+
 extern "C" int mc_fit_windows_noalloc(mc_ctx *ctx, int32_t model, int32_t n_windows,
                                       int32_t n_points, const float *x_dev, const float *y_dev,
                                       const float *guess_dev, double rstep, int32_t max_iter,
                                       double *out_dev) {
   MC_CHECK(ctx && x_dev && y_dev && guess_dev && out_dev, MC_ERR_ARG, "null argument");
-  MC_CHECK(model == MC_FIT_PV || model == MC_FIT_TCH || model == MC_FIT_PV2, MC_ERR_ARG,
-           "model %d", model);
+  MC_CHECK(fit_guesses(model) > 0, MC_ERR_ARG, "model %d", model);
   MC_CHECK(n_windows >= 0 && n_points >= 2 && n_points <= 4096 && max_iter >= 1, MC_ERR_ARG,
            "bad sizes n_windows=%d n_points=%d", n_windows, n_points);
   MC_CHECK((int64_t)n_windows * n_points <= kMaxScratchFloats, MC_ERR_ARG,
@@ -517,35 +624,24 @@ extern "C" int mc_fit_windows_noalloc(mc_ctx *ctx, int32_t model, int32_t n_wind
   float *xr = nullptr;
   int rc = scratch_ptr(ctx->device, &xr);
   if (rc) return rc;
-  hipStream_t s = ctx->stream;
-  const unsigned blocks = (unsigned)((n_windows + 63) / 64);
-  if (model == MC_FIT_PV)
-    hipLaunchKernelGGL(k_fit_windows<MC_FIT_PV>, dim3(blocks), dim3(64), 0, s, n_windows,
-                       n_points, x_dev, y_dev, guess_dev, rstep, max_iter, xr, out_dev);
-  else if (model == MC_FIT_PV2)
-    hipLaunchKernelGGL(k_fit_windows<MC_FIT_PV2>, dim3(blocks), dim3(64), 0, s, n_windows,
-                       n_points, x_dev, y_dev, guess_dev, rstep, max_iter, xr, out_dev);
-  else
-    hipLaunchKernelGGL(k_fit_windows<MC_FIT_TCH>, dim3(blocks), dim3(64), 0, s, n_windows,
-                       n_points, x_dev, y_dev, guess_dev, rstep, max_iter, xr, out_dev);
-  MC_HIP(hipGetLastError());
-  return MC_OK;
+  return launch_fit_windows(ctx->stream, model, n_windows, n_points, x_dev, y_dev, guess_dev,
+                            rstep, max_iter, xr, out_dev);
 }
 
 extern "C" int mc_fit_windows(mc_ctx *ctx, int32_t model, int32_t n_windows, int32_t n_points,
                               const float *x, const float *y, const float *peak_guess,
                               double rstep, int32_t max_iter, double *out, int32_t on_device) {
   MC_CHECK(ctx && x && y && peak_guess && out, MC_ERR_ARG, "null argument");
-  MC_CHECK(model == MC_FIT_PV || model == MC_FIT_TCH || model == MC_FIT_PV2, MC_ERR_ARG,
-           "model %d", model);
+  MC_CHECK(fit_guesses(model) > 0, MC_ERR_ARG, "model %d", model);
   MC_CHECK(n_windows >= 0 && n_points >= 2 && n_points <= 4096 && max_iter >= 1, MC_ERR_ARG,
            "bad sizes n_windows=%d n_points=%d", n_windows, n_points);
   if (n_windows == 0) return MC_OK;
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   MC_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  const size_t wb = (size_t)n_windows * n_points * 4, ob = (size_t)n_windows * MC_FIT_OUT * 8;
-  const size_t gb = (size_t)n_windows * (model == MC_FIT_PV2 ? 2 : 1) * 4;  // guesses
+  const size_t wb = (size_t)n_windows * n_points * 4;
+  const size_t ob = (size_t)n_windows * fit_rows(model) * MC_FIT_OUT * 8;
+  const size_t gb = (size_t)n_windows * fit_guesses(model) * 4;
   DBuf dx, dy, dg, dout, dxr;
   const float *px = x, *py = y, *pg = peak_guess;
   double *po = out;
@@ -563,17 +659,9 @@ extern "C" int mc_fit_windows(mc_ctx *ctx, int32_t model, int32_t n_windows, int
     po = dout.as<double>();
   }
   if ((rc = dxr.alloc(wb))) return rc;
-  const unsigned blocks = (unsigned)((n_windows + 63) / 64);
-  if (model == MC_FIT_PV)
-    hipLaunchKernelGGL(k_fit_windows<MC_FIT_PV>, dim3(blocks), dim3(64), 0, s, n_windows,
-                       n_points, px, py, pg, rstep, max_iter, dxr.as<float>(), po);
-  else if (model == MC_FIT_PV2)
-    hipLaunchKernelGGL(k_fit_windows<MC_FIT_PV2>, dim3(blocks), dim3(64), 0, s, n_windows,
-                       n_points, px, py, pg, rstep, max_iter, dxr.as<float>(), po);
-  else
-    hipLaunchKernelGGL(k_fit_windows<MC_FIT_TCH>, dim3(blocks), dim3(64), 0, s, n_windows,
-                       n_points, px, py, pg, rstep, max_iter, dxr.as<float>(), po);
-  MC_HIP(hipGetLastError());
+  if ((rc = launch_fit_windows(s, model, n_windows, n_points, px, py, pg, rstep, max_iter,
+                               dxr.as<float>(), po)))
+    return rc;
   if (!on_device) MC_HIP(hipMemcpyAsync(out, po, ob, hipMemcpyDeviceToHost, s));
   MC_HIP(hipStreamSynchronize(s));
   return MC_OK;

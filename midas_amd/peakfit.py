@@ -16,6 +16,16 @@ from .caking import CSRGeometry, Context, _is_cuda, _torch, default_context
 
 PF_PARAMS_PER_PEAK = 7
 MODELS = {"pv": _lib.MC_FIT_PV, "tch": _lib.MC_FIT_TCH, "pv2": _lib.MC_FIT_PV2}
+MODELS.update({f"tch{n}": _lib.MC_FIT_TCH_PEAKS(n) for n in range(2, _lib.MC_FIT_MAX_PEAKS + 1)})
+
+
+def _guesses(model: str) -> int:
+    """Guesses per window (= result rows for the TCH models)."""
+    if model in ("pv", "tch"):
+        return 1
+    if model == "pv2":
+        return 2
+    return int(model[3:])
 
 
 def fit_windows(x, y, peak_guess, rstep: float, *, model: str = "pv", max_iter: int = 100,
@@ -25,19 +35,24 @@ def fit_windows(x, y, peak_guess, rstep: float, *, model: str = "pv", max_iter: 
     status (iterations; negative when the window could not be fitted).
     model "pv2" (pV doublet, PeakFit.c:984-1101) takes peak_guess [W, 2]
     (the two centres) and returns R1, R2, SNR1, SNR2, Gamma1, Gamma2, mu,
-    status."""
+    status.  model "tchK" (K = 2..8 TCH peaks on each window, a merged
+    fitPeaks job, PeakFit.c:321-351) takes peak_guess [W, K] and returns
+    [W, K, 8], one row per peak."""
+    if model not in MODELS:
+        raise ValueError(f"unknown model {model!r}; expected one of {sorted(MODELS)}")
     ctx = ctx or default_context()
     x = np.ascontiguousarray(x, dtype=np.float32)
     y = np.ascontiguousarray(y, dtype=np.float32)
     g = np.ascontiguousarray(peak_guess, dtype=np.float32).reshape(-1)
-    per = 2 if model == "pv2" else 1
+    per = _guesses(model)
     if x.shape != y.shape or x.ndim != 2 or g.size != per * x.shape[0]:
-        raise ValueError(f"x, y must be [W, N] and peak_guess [W{', 2' if per == 2 else ''}]")
-    out = np.zeros((x.shape[0], _lib.MC_FIT_OUT), dtype=np.float64)
+        raise ValueError(f"x, y must be [W, N] and peak_guess [W{', %d' % per if per > 1 else ''}]")
+    rows = per if model.startswith("tch") else 1
+    out = np.zeros((x.shape[0], rows, _lib.MC_FIT_OUT), dtype=np.float64)
     check(lib().mc_fit_windows(ctx.handle, MODELS[model], x.shape[0], x.shape[1],
                                x.ctypes.data, y.ctypes.data, g.ctypes.data, rstep, max_iter,
                                out.ctypes.data, 0))
-    return out
+    return out if rows > 1 else out[:, 0]
 
 
 def fit_cake_rings(cakes, geom: CSRGeometry, r_centers, rings, *, pad: int = 20,
@@ -45,10 +60,16 @@ def fit_cake_rings(cakes, geom: CSRGeometry, r_centers, rings, *, pad: int = 20,
     """Per-eta fits of every ring in every cake, in one launch
     (IntegratorZarrOMP.c:1903-2070).  ``cakes`` is a CUDA tensor
     [F, n_r, n_eta]; returns [F, n_eta, n_rings, 8] float64 (host).
-    With doublet_separation > 0 (pV only), adjacent rings closer than it
-    are fitted jointly as pV doublets on one ROI spanning both
-    (:1912-1923, 1960-2016): their rows then hold the centre [1], SNR [6]
-    and status [7] of the doublet fit, the other fields 0."""
+    With doublet_separation > 0, adjacent rings closer than it are fitted
+    jointly as doublets on one ROI spanning both (:1912-1923, 1960-2016):
+    pV doublets (pf_fit_doublet_pv; the reference's per-eta pass always
+    uses this mode, :1997), or for model "tch" TCH doublets, the TCH mode
+    of pf_fit_doublet_peak (PeakFit.c:1183-1195) that the reference's
+    calibrant integrator selects (pf_fit_doublet_tch, :1138-1170: SNR =
+    area / (1.064 FWHM) / sqrt(chi2/dof), falling back to the pV doublet
+    where the TCH fit fails);
+    their rows then hold the centre [1], FWHM [4], SNR [6] and status [7]
+    of the doublet fit, the other fields 0."""
     if not _is_cuda(cakes):
         raise TypeError("cakes must be a device tensor (the output of integrate_batch)")
     t = _torch()
@@ -62,13 +83,13 @@ def fit_cake_rings(cakes, geom: CSRGeometry, r_centers, rings, *, pad: int = 20,
                                   rc.ctypes.data_as(C.POINTER(C.c_double)),
                                   rg.ctypes.data_as(C.POINTER(C.c_double)), rg.size, pad,
                                   MODELS[model], max_iter, out.ctypes.data, 0))
-    pairs = doublet_pairs(rg, doublet_separation) if model == "pv" else []
+    pairs = doublet_pairs(rg, doublet_separation)
     if pairs:
-        _fit_ring_doublets(c, rc, rg, pairs, pad, max_iter, out, geom)
+        _fit_ring_doublets(c, rc, rg, pairs, pad, max_iter, out, geom, model)
     return out
 
 
-def _fit_ring_doublets(c, rc, rg, pairs, pad, max_iter, out, geom):
+def _fit_ring_doublets(c, rc, rg, pairs, pad, max_iter, out, geom, model="pv"):
     """The doublet pass of the per-eta fits (IntegratorZarrOMP.c:1960-2016):
     ROI [min - pad bins, max + pad bins] over both rings, skipped when under
     6 bins or when the eta profile was skipped (status -3)."""
@@ -99,47 +120,98 @@ def _fit_ring_doublets(c, rc, rg, pairs, pad, max_iter, out, geom):
         G = np.tile([plo, phi], (fi.size, 1))
         res = fit_windows(X, Y, G, step, model="pv2", max_iter=max_iter, ctx=geom.ctx)
         good = res[:, 7] > 0
-        for ring, col_r, col_snr in ((lo_r, 0, 2), (hi_r, 1, 3)):
+        for ring, col_r, col_snr, col_fw in ((lo_r, 0, 2, 4), (hi_r, 1, 3, 5)):
             out[fi, ei, ring, 1] = np.where(good, res[:, col_r], 0.0)
+            out[fi, ei, ring, 4] = np.where(good, res[:, col_fw], 0.0)
             out[fi, ei, ring, 6] = np.where(good, res[:, col_snr], 0.0)
             out[fi, ei, ring, 7] = res[:, 7]
+        if model == "tch":
+            # pf_fit_doublet_tch: both TCH peaks fitted on the window; where
+            # both centres come out positive its rows replace the pV rows
+            t = fit_windows(X, Y, G, step, model="tch2", max_iter=max_iter, ctx=geom.ctx)
+            ok = (t[:, 0, 7] > 0) & (t[:, 1, 7] > 0) & (t[:, 0, 1] > 0) & (t[:, 1, 1] > 0)
+            for ring, k in ((lo_r, 0), (hi_r, 1)):
+                area, fw, chi = t[:, k, 0], t[:, k, 4], t[:, k, 6]
+                h = np.where(fw > 0, area / np.where(fw > 0, fw, 1.0) / 1.064, area)
+                snr = np.where(chi > 0, h / np.sqrt(np.where(chi > 0, chi, 1.0)), 1.0)
+                out[fi, ei, ring, 1] = np.where(ok, t[:, k, 1], out[fi, ei, ring, 1])
+                out[fi, ei, ring, 4] = np.where(ok, fw, out[fi, ei, ring, 4])
+                out[fi, ei, ring, 6] = np.where(ok, snr, out[fi, ei, ring, 6])
+                out[fi, ei, ring, 7] = np.where(ok, t[:, k, 7], out[fi, ei, ring, 7])
+
+
+def fit_jobs(n_bins: int, peak_bins, fit_roi_padding: int) -> list:
+    """The fit jobs of fitPeaks (PeakFit.c:321-351): peaks sorted by bin,
+    ROI [bin - pad, bin + pad] clipped to the profile; a peak whose ROI
+    starts at or before the current job's end joins it and moves the end to
+    its own ROI end.  Returns [(start, end, [positions in peak_bins])].
+    Jobs of more than MC_FIT_MAX_PEAKS peaks are split into consecutive
+    groups of at most that many, each on [first - pad, last + pad] (the
+    reference fits any number jointly)."""
+    order = sorted(range(len(peak_bins)), key=lambda i: peak_bins[i])
+    jobs = []
+    for i in order:
+        k = int(peak_bins[i])
+        start = max(0, k - fit_roi_padding)
+        end = min(n_bins - 1, k + fit_roi_padding)
+        if jobs and start <= jobs[-1][1]:
+            jobs[-1][1] = end
+            jobs[-1][2].append(i)
+        else:
+            jobs.append([start, end, [i]])
+    out = []
+    cap = _lib.MC_FIT_MAX_PEAKS
+    for start, end, members in jobs:
+        if len(members) <= cap:
+            out.append((start, end, members))
+            continue
+        for c in range(0, len(members), cap):
+            part = members[c:c + cap]
+            lo = max(0, int(peak_bins[part[0]]) - fit_roi_padding)
+            hi = min(n_bins - 1, int(peak_bins[part[-1]]) + fit_roi_padding)
+            out.append((lo, hi, part))
+    return out
 
 
 def fit_peaks(x, intensity, peak_locations, x_bin_size: float, fit_roi_padding: int = 20,
               *, max_iter: int = 200, ctx: Context | None = None) -> np.ndarray:
-    """fitPeaks (PeakFit.c:275-544) on a 1-D profile: the TCH model on the
-    ROI of +-fit_roi_padding bins around the bin nearest each location
-    (locations farther than 2 bins from every sample are skipped, as in
-    the reference).  Returns [n_peaks, 7]; rows of skipped peaks are 0.
-    Each peak is one window (the reference merges overlapping ROIs into a
-    multi-peak job; here each is fitted on its own)."""
+    """fitPeaks (PeakFit.c:275-544) on a 1-D profile: each location snaps to
+    the nearest sample (skipped when it is 2 bins or more away); ROIs of
+    +-fit_roi_padding bins that overlap are merged into one job fitted as a
+    sum of TCH peaks over one Chebyshev background (fit_jobs); jobs of
+    equal length and peak count go to the GPU in one launch.  Returns
+    [n_peaks, 7] in the order of peak_locations, chi2/dof shared by the
+    peaks of a job; rows of skipped or failed peaks are 0.  (The reference
+    writes the fitted rows in order of their bins.)"""
     x = np.asarray(x, dtype=np.float64)
     yv = np.asarray(intensity, dtype=np.float64)
     locs = np.atleast_1d(np.asarray(peak_locations, dtype=np.float64))
     out = np.zeros((locs.size, PF_PARAMS_PER_PEAK))
-    wins, keep = [], []
+    bins, keep = [], []
     for i, p in enumerate(locs):
         k = int(np.argmin(np.abs(x - p)))
         if abs(x[k] - p) >= 2.0 * x_bin_size:
             continue
-        lo, hi = max(0, k - fit_roi_padding), min(x.size - 1, k + fit_roi_padding)
-        wins.append((lo, hi, x[k]))
+        bins.append(k)
         keep.append(i)
-    if not wins:
+    if not bins:
         return out
-    # One launch per distinct window length (ROIs clipped at the profile
-    # ends are shorter).
-    by_len: dict[int, list] = {}
-    for (lo, hi, c), i in zip(wins, keep):
-        by_len.setdefault(hi - lo + 1, []).append((lo, hi, c, i))
-    for L, items in by_len.items():
-        X = np.stack([x[lo:hi + 1] for lo, hi, _, _ in items])
-        Y = np.stack([yv[lo:hi + 1] for lo, hi, _, _ in items])
-        G = np.array([c for _, _, c, _ in items])
-        r = fit_windows(X, Y, G, x_bin_size, model="tch", max_iter=max_iter, ctx=ctx)
-        for row, (_, _, _, i) in zip(r, items):
-            if row[7] > 0:
-                out[i] = row[:PF_PARAMS_PER_PEAK]
+    groups: dict[tuple, list] = {}
+    for start, end, members in fit_jobs(x.size, bins, fit_roi_padding):
+        groups.setdefault((end - start + 1, len(members)), []).append((start, end, members))
+    for (L, npk), jobs in groups.items():
+        X = np.stack([x[s:e + 1] for s, e, _ in jobs])
+        Y = np.stack([yv[s:e + 1] for s, e, _ in jobs])
+        G = np.array([[x[bins[m]] for m in mem] for _, _, mem in jobs])
+        model = "tch" if npk == 1 else f"tch{npk}"
+        r = fit_windows(X, Y, G if npk > 1 else G[:, 0], x_bin_size, model=model,
+                        max_iter=max_iter, ctx=ctx)
+        if npk == 1:
+            r = r[:, None, :]
+        for rows, (_, _, mem) in zip(r, jobs):
+            for row, m in zip(rows, mem):
+                if row[7] > 0:
+                    out[keep[m]] = row[:PF_PARAMS_PER_PEAK]
     return out
 
 

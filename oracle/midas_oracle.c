@@ -1067,27 +1067,96 @@ void orc_window_init(int model, int n, const double *x, const double *y,
   p[5] = 0.0; lo[5] = -range; hi[5] = range;
 }
 
+/* fitPeaks' start values and bounds for a job of nPeaks peaks on the
+ * window (PeakFit.c:370-445): per peak, pf_estimate_initial_params at the
+ * sample nearest its location (background = mean of min(5, n/4) samples at
+ * each end, amplitude = peak - background or the peak value, FWHM = the
+ * half-maximum crossing distance in bins, at least 2), area / centre / sig
+ * / gam start and bounds; the background [bg0, bg1] from the first peak's
+ * estimate and the window's range.  Parameters 4 per peak, then bg0, bg1. */
+void orc_tchn_init(int nPeaks, int n, const double *x, const double *y, double rstep,
+                   const double *locs, double *p, double *lo, double *hi) {
+  double mx = -1e30, mn = 1e30;
+  for (int i = 0; i < n; i++) {
+    if (y[i] > mx) mx = y[i];
+    if (y[i] < mn) mn = y[i];
+  }
+  double range = fmax(mx - mn, 1.0);
+  int bw = (int)fmin(5, n / 4);
+  if (bw < 1) bw = 1;
+  double bs = 0.0;
+  for (int i = 0; i < bw; ++i) bs += y[i] + y[n - 1 - i];
+  double bgg = bs / (2.0 * bw);
+  for (int k = 0; k < nPeaks; k++) {
+    int pk = 0;
+    double best = 1e10;
+    for (int i = 0; i < n; i++)
+      if (fabs(x[i] - locs[k]) < best) { best = fabs(x[i] - locs[k]); pk = i; }
+    double amp = y[pk] - bgg;
+    if (amp <= 0) amp = y[pk];
+    double half = bgg + amp / 2.0;
+    int l = pk, r = pk;
+    while (l > 0 && y[l] > half) l--;
+    while (r < n - 1 && y[r] > half) r++;
+    double fw_bins = (double)(r - l);
+    if (!(fw_bins > 1.0)) fw_bins = 2.0;
+    double fw = fw_bins * rstep, fgi = fw * 0.5;
+    double sig0 = (fgi * 100.0) * (fgi * 100.0) / (8.0 * log(2.0));
+    if (sig0 < 1e-6) sig0 = 1.0;
+    double gam0 = fw * 0.5 * 100.0;
+    if (gam0 < 0.01) gam0 = 1.0;
+    double a0 = amp * fw * 1.5;
+    if (a0 <= 0) a0 = range * rstep;
+    double crange = fmax(fw, (double)n * rstep * 0.5);
+    double ahi = range * (double)n * rstep * 2.0;
+    if (ahi < 1.0) ahi = 1.0;
+    double shi = (rstep * 200.0 * 100.0) * (rstep * 200.0 * 100.0) / (8.0 * log(2.0));
+    double ghi = rstep * 200.0 * 100.0;
+    double *q = p + 4 * k, *ql = lo + 4 * k, *qh = hi + 4 * k;
+    q[0] = fmax(fmin(a0, ahi * 0.9), 0.01); ql[0] = 0; qh[0] = ahi;
+    q[1] = x[pk]; ql[1] = x[pk] - crange; qh[1] = x[pk] + crange;
+    q[2] = fmax(fmin(sig0, shi * 0.5), 1e-6 * 2.0); ql[2] = 1e-6; qh[2] = shi;
+    q[3] = fmax(fmin(gam0, ghi * 0.5), 0.01 * 2.0); ql[3] = 0.01; qh[3] = ghi;
+    for (int j = 0; j < 4; j++) {
+      if (q[j] <= ql[j]) q[j] = ql[j] + (qh[j] - ql[j]) * 0.01;
+      if (q[j] >= qh[j]) q[j] = qh[j] - (qh[j] - ql[j]) * 0.01;
+    }
+  }
+  double *b = p + 4 * nPeaks, *bl = lo + 4 * nPeaks, *bh = hi + 4 * nPeaks;
+  b[0] = bgg; bl[0] = fmin(mn - range * 0.5, -1.0); bh[0] = mx + range;
+  if (b[0] <= bl[0]) b[0] = bl[0] + 0.01;
+  if (b[0] >= bh[0]) b[0] = bh[0] - 0.01;
+  b[1] = 0.0; bl[1] = -range; bh[1] = range;
+}
+
 /* ------------------------------------------------------------------ */
 /* FP64 Levenberg-Marquardt (the build's CPU fit, same objective as the */
 /* reference: sum of squared residuals, box bounds by projection).      */
 /* ------------------------------------------------------------------ */
+/* Peaks of a TCH-family model code: 1 = one peak, 3..9 = 2..8 peaks (a
+ * merged fitPeaks job, PeakFit.c:321-351); 0 for pV. */
+static int orc_tch_peaks(int model) {
+  return model == 1 ? 1 : (model >= 3 && model <= 9) ? model - 1 : 0;
+}
+static int orc_np(int model) { return model == 0 ? 5 : 4 * orc_tch_peaks(model) + 2; }
+
 static void orc_model_jac(int model, int n, const double *x, const double *p,
                           double xlo, double xhi, double *f, double *J) {
-  int np = model == 0 ? 5 : 6;
+  int np = orc_np(model), npk = orc_tch_peaks(model);
   /* Central differences in FP64 with a relative step: the checker only
    * needs the converged optimum, not the GPU's analytic Jacobian. */
   if (model == 0) orc_pv_model(p, n, x, f);
-  else orc_tch_model(1, p, n, x, xlo, xhi, f);
-  double q[6], fp[512], fm[512];
+  else orc_tch_model(npk, p, n, x, xlo, xhi, f);
+  double q[34], fp[512], fm[512];
   for (int k = 0; k < np; k++) {
     for (int j = 0; j < np; j++) q[j] = p[j];
     double h = fmax(fabs(p[k]), 1e-3) * 1e-6;
     q[k] = p[k] + h;
     if (model == 0) orc_pv_model(q, n, x, fp);
-    else orc_tch_model(1, q, n, x, xlo, xhi, fp);
+    else orc_tch_model(npk, q, n, x, xlo, xhi, fp);
     q[k] = p[k] - h;
     if (model == 0) orc_pv_model(q, n, x, fm);
-    else orc_tch_model(1, q, n, x, xlo, xhi, fm);
+    else orc_tch_model(npk, q, n, x, xlo, xhi, fm);
     for (int i = 0; i < n; i++) J[i * np + k] = (fp[i] - fm[i]) / (2 * h);
   }
 }
@@ -1118,20 +1187,51 @@ static int orc_chol_solve(int n, double *A, double *b) {
   return 0;
 }
 
+/* The TCH models are fitted in u = sqrt(sig) in place of sig (the Gaussian
+ * FWHM is linear in u): dF_G/dsig grows without bound as sig -> 0, which
+ * stalls the damped steps once a path passes near a Lorentzian shape.  Same
+ * model, same bounds (sqrt of the sig bounds), results returned in sig. */
+static void orc_to_ext(int npk, int np, const double *u, double *p) {
+  for (int a = 0; a < np; a++) p[a] = u[a];
+  for (int k = 0; k < npk; k++) p[4 * k + 2] = u[4 * k + 2] * u[4 * k + 2];
+}
+
+static void orc_eval(int model, int npk, int np, int n, const double *x, const double *u,
+                     double xlo, double xhi, double *f, double *J) {
+  double p[34];
+  orc_to_ext(npk, np, u, p);
+  if (J) {
+    orc_model_jac(model, n, x, p, xlo, xhi, f, J);
+    for (int k = 0; k < npk; k++)
+      for (int i = 0; i < n; i++) J[i * np + 4 * k + 2] *= 2.0 * u[4 * k + 2];
+  } else if (model == 0) {
+    orc_pv_model(p, n, x, f);
+  } else {
+    orc_tch_model(npk, p, n, x, xlo, xhi, f);
+  }
+}
+
 int orc_lm_fit(int model, int n, const double *x, const double *y, double *p,
-               const double *lo, const double *hi, int maxIter,
+               const double *lo_in, const double *hi_in, int maxIter,
                double *chi2_out) {
-  int np = model == 0 ? 5 : 6;
-  if (n > 512) return -1;
+  int np = orc_np(model), npk = orc_tch_peaks(model);
+  if (n > 512 || (model != 0 && npk == 0)) return -1;
   double xlo = x[0], xhi = x[n - 1];
-  double f[512], J[512 * 6], ft[512];
+  static __thread double J[512 * 34];
+  double f[512], ft[512], u[34], lo[34], hi[34];
+  for (int a = 0; a < np; a++) { u[a] = p[a]; lo[a] = lo_in[a]; hi[a] = hi_in[a]; }
+  for (int k = 0; k < npk; k++) {
+    u[4 * k + 2] = sqrt(fmax(p[4 * k + 2], 0.0));
+    lo[4 * k + 2] = sqrt(fmax(lo_in[4 * k + 2], 0.0));
+    hi[4 * k + 2] = sqrt(fmax(hi_in[4 * k + 2], 0.0));
+  }
   double lam = 1e-3;
-  orc_model_jac(model, n, x, p, xlo, xhi, f, J);
+  orc_eval(model, npk, np, n, x, u, xlo, xhi, f, J);
   double chi = 0;
   for (int i = 0; i < n; i++) chi += (f[i] - y[i]) * (f[i] - y[i]);
-  int it;
-  for (it = 0; it < maxIter; it++) {
-    double JtJ[36], g[6];
+  int it, done = 0;
+  for (it = 0; it < maxIter && !done; it++) {
+    double JtJ[34 * 34], g[34];
     for (int a = 0; a < np; a++) {
       g[a] = 0;
       for (int i = 0; i < n; i++) g[a] += J[i * np + a] * (y[i] - f[i]);
@@ -1143,44 +1243,50 @@ int orc_lm_fit(int model, int n, const double *x, const double *y, double *p,
     }
     int accepted = 0;
     for (int tries = 0; tries < 12 && !accepted; tries++) {
-      double M[36], d[6], q[6];
+      double M[34 * 34], d[34], q[34];
       for (int a = 0; a < np * np; a++) M[a] = JtJ[a];
       for (int a = 0; a < np; a++) {
         M[a * np + a] += lam * fmax(JtJ[a * np + a], 1e-30);
         d[a] = g[a];
       }
+      /* active set: a parameter on a bound whose descent direction points
+       * out of the box is held there for this step */
+      for (int a = 0; a < np; a++) {
+        if ((u[a] <= lo[a] && g[a] < 0) || (u[a] >= hi[a] && g[a] > 0)) {
+          for (int b = 0; b < np; b++) M[a * np + b] = M[b * np + a] = 0;
+          M[a * np + a] = 1;
+          d[a] = 0;
+        }
+      }
       if (orc_chol_solve(np, M, d) != 0) { lam *= 10; continue; }
       for (int a = 0; a < np; a++) {
-        q[a] = p[a] + d[a];
+        q[a] = u[a] + d[a];
         if (q[a] < lo[a]) q[a] = lo[a];
         if (q[a] > hi[a]) q[a] = hi[a];
       }
-      if (model == 0) orc_pv_model(q, n, x, ft);
-      else orc_tch_model(1, q, n, x, xlo, xhi, ft);
+      orc_eval(model, npk, np, n, x, q, xlo, xhi, ft, NULL);
       double c2 = 0;
       for (int i = 0; i < n; i++) c2 += (ft[i] - y[i]) * (ft[i] - y[i]);
       if (c2 <= chi) {
         double step = 0, mag = 0;
         for (int a = 0; a < np; a++) {
-          step += (q[a] - p[a]) * (q[a] - p[a]);
-          mag += p[a] * p[a];
-          p[a] = q[a];
+          step += (q[a] - u[a]) * (q[a] - u[a]);
+          mag += u[a] * u[a];
+          u[a] = q[a];
         }
         double dchi = chi - c2;
         chi = c2;
         lam = fmax(lam / 10, 1e-12);
         accepted = 1;
-        orc_model_jac(model, n, x, p, xlo, xhi, f, J);
-        if (step <= 1e-24 * (mag + 1e-24) || dchi <= 1e-15 * (chi + 1e-300)) {
-          if (chi2_out) *chi2_out = chi;
-          return it + 1;
-        }
+        orc_eval(model, npk, np, n, x, u, xlo, xhi, f, J);
+        if (step <= 1e-24 * (mag + 1e-24) || dchi <= 1e-15 * (chi + 1e-300)) done = 1;
       } else {
         lam *= 10;
       }
     }
     if (!accepted) break;
   }
+  orc_to_ext(npk, np, u, p);
   if (chi2_out) *chi2_out = chi;
   return it;
 }

@@ -119,13 +119,16 @@ void orc_tch_model(int nPeaks, const double *params, int n, const double *x,
 void orc_pv_model(const double *p5, int n, const double *x, double *out);
 /* FP64 Levenberg-Marquardt with box bounds; model 0 = pV-5
  * [Rcen, Mu, Gamma, Imax, BG], model 1 = TCH single peak
- * [area, center, sig, gam, bg0, bg1]. Returns iterations. */
+ * [area, center, sig, gam, bg0, bg1], models 3..9 = TCH with 2..8 peaks
+ * (4 params each, then bg0, bg1). Returns iterations. */
 int orc_lm_fit(int model, int n, const double *x, const double *y,
                double *params, const double *lo, const double *hi,
                int maxIter, double *chi2_out);
 void orc_window_init(int model, int n, const double *x, const double *y,
                      double rstep, double rmean, double *p, double *lo,
                      double *hi);
+void orc_tchn_init(int nPeaks, int n, const double *x, const double *y, double rstep,
+                   const double *locs, double *p, double *lo, double *hi);
 
 #ifdef __cplusplus
 }

@@ -111,6 +111,7 @@ def lib():
         L.orc_lm_fit.argtypes = [C.c_int, C.c_int, P(d), P(d), P(d), P(d), P(d),
                                  C.c_int, P(d)]
         L.orc_lm_fit.restype = C.c_int
+        L.orc_tchn_init.argtypes = [C.c_int, C.c_int, P(d), P(d), d, P(d), P(d), P(d), P(d)]
         L.orc_window_init.argtypes = [C.c_int, C.c_int, P(d), P(d), d, d, P(d),
                                       P(d), P(d)]
         _lib = L
@@ -421,6 +422,18 @@ def window_init(model, x, y, rstep, rmean):
     n = 5 if model == 0 else 6
     p, lo, hi = np.zeros(n), np.zeros(n), np.zeros(n)
     lib().orc_window_init(model, x.size, _dp(x), _dp(y), rstep, rmean, _dp(p), _dp(lo), _dp(hi))
+    return p, lo, hi
+
+
+def tchn_init(x, y, rstep, locs):
+    """fitPeaks' start and bounds for one merged job (PeakFit.c:370-445)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    locs = np.ascontiguousarray(locs, dtype=np.float64)
+    n = 4 * locs.size + 2
+    p, lo, hi = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().orc_tchn_init(locs.size, x.size, _dp(x), _dp(y), rstep, _dp(locs), _dp(p), _dp(lo),
+                        _dp(hi))
     return p, lo, hi
 
 

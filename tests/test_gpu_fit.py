@@ -1,5 +1,12 @@
 """Batched LM fits on the GPU vs the FP64 CPU LM of the oracle, and exact
-recovery of noise-free peaks.  Centres must agree within 1e-4 px."""
+recovery of noise-free peaks.  Centres must agree within 1e-4 px in every
+window; the only windows excused are named by cause in each test (a local
+minimum of the bounded objective that the FP64 fit reaches as well, or a
+noisy window whose GPU sum of squares is no more than 1e-6 relative above
+the FP64 fit's: both points are then equally good minima at the FP32
+resolution of the residuals, and the data do not fix the centre to
+1e-4 px), and the tests print the worst window and every excused one."""
+SUMSQ_TIE = 1e-6
 import numpy as np
 import pytest
 
@@ -26,16 +33,34 @@ def _pv_windows(n_win, n_pts, seed, noise):
     return x, y, c
 
 
+def _worst(name, err):
+    i = int(np.argmax(err))
+    print(f"[fit] {name}: worst window {i}: {err[i]:.3g} px over {err.size}", flush=True)
+
+
+def _active(p, lo, hi):
+    return np.nonzero((p <= lo + 1e-9 * (np.abs(lo) + 1)) | (p >= hi - 1e-9 * (np.abs(hi) + 1)))[0]
+
+
+def _tch_code(n):
+    return 1 if n == 1 else n + 1
+
+
 def test_pv_noise_free_recovery():
     x, y, c = _pv_windows(2000, 41, 0, noise=False)
     out = peakfit.fit_windows(x, y, c + 0.1, 0.25, model="pv", max_iter=200)
     assert (out[:, 7] > 0).all()
     err = np.abs(out[:, 1] - c)
-    assert np.quantile(err, 0.99) < CENTER_TOL, np.sort(err)[-5:]
+    _worst("pV noise-free", err)
+    assert np.all(err < CENTER_TOL), np.sort(err)[-5:]
 
 
 @pytest.mark.parametrize("model", ["pv", "tch"])
 def test_gpu_lm_matches_fp64_oracle(model, oracle_lib):
+    """Noisy single peaks (SNR 20-200) sampled at 3-10 points per FWHM: the
+    GPU centre matches the FP64 LM's in every window, except where the GPU
+    sum of squares is within SUMSQ_TIE (relative) of the FP64 one or below
+    it (equally good minima of a flat objective)."""
     m = 0 if model == "pv" else 1
     rng = np.random.default_rng(4)
     n_win, n_pts = 300, 41
@@ -48,24 +73,41 @@ def test_gpu_lm_matches_fp64_oracle(model, oracle_lib):
             yy = oracle_lib.pv_model(p, x[i])
             amp = p[3]
         else:
-            p = [rng.uniform(500, 3000), ctr, rng.uniform(50, 400), rng.uniform(5, 60), 40.0, 3.0]
+            p = [rng.uniform(500, 3000), ctr, rng.uniform(2000, 20000), rng.uniform(50, 250),
+                 40.0, 3.0]
             yy = oracle_lib.tch_model(p, x[i])
             amp = yy.max() - 40
         yy = yy + rng.normal(0, amp / rng.uniform(20, 200), n_pts)
         xs.append(x[i]); ys.append(yy); gs.append(x[i, 20])
     X, Y, G = np.array(xs), np.array(ys), np.array(gs)
     out = peakfit.fit_windows(X, Y, G, 0.25, model=model, max_iter=300)
-    cpu = []
+    cpu, flat = [], []
+    assert np.all(out[:, 7] > 0)
     for i in range(n_win):
-        p0, lo, hi = oracle_lib.window_init(m, X[i], Y[i].astype(np.float32).astype(np.float64),
-                                            0.25, G[i])
-        p, chi, it = oracle_lib.lm_fit(m, X[i], Y[i].astype(np.float32).astype(np.float64),
-                                       p0, lo, hi, max_iter=300)
+        yy = Y[i].astype(np.float32).astype(np.float64)
+        p0, lo, hi = oracle_lib.window_init(m, X[i], yy, 0.25, G[i])
+        p, chi, it = oracle_lib.lm_fit(m, X[i], yy, p0, lo, hi, max_iter=300)
         cpu.append(p[0] if model == "pv" else p[1])
+        flat.append((_active(p, lo, hi).size > 0 or it >= 300, chi))
     d = np.abs(out[:, 1] - np.array(cpu))
-    ok = out[:, 7] > 0
-    assert ok.mean() > 0.99
-    assert np.quantile(d[ok], 0.98) < CENTER_TOL, np.sort(d[ok])[-8:]
+    _worst(f"{model} GPU vs FP64 LM", d)
+    for i in np.nonzero(d >= CENTER_TOL)[0]:
+        on_bound, chi = flat[i]
+        g_chi = _gpu_sumsq(model, out[i], X.shape[1])
+        print(f"[fit]   window {i}: {d[i]:.3g} px, FP64 fit on a bound/cap: {on_bound}, "
+              f"sum of squares GPU {g_chi:.8g} FP64 {chi:.8g}", flush=True)
+        assert g_chi <= chi * (1 + SUMSQ_TIE), i
+
+
+def _gpu_sumsq(model, row, n_pts):
+    """Sum of squared residuals from a result row.  pV: SNR = Imax / rms,
+    rms = sqrt(S / n), Imax from the area, FWHM and mu (pf_fit_singlet_pv,
+    PeakFit.c:960-974); TCH: chi2/dof times the degrees of freedom."""
+    if model == "pv":
+        G, mu = row[4], row[5]
+        imax = row[0] / (G * (mu * np.pi / 2 + (1 - mu) * np.sqrt(np.pi / (4 * np.log(2)))))
+        return n_pts * (imax / row[6]) ** 2
+    return row[6] * max(n_pts - 6, 1)
 
 
 def test_fit_cake_rings_finds_the_rings():
@@ -107,25 +149,70 @@ def _lineout(x, locs, slope=0.05, seed=None):
     return y
 
 
-def test_fit_peaks_auto_finds_the_peaks():
+def _oracle_fit_peaks(O, x, y, locs, step, pad):
+    """fitPeaks (PeakFit.c:275-544) with the FP64 LM: snap, sort, merge
+    ROIs, fit each job from fitPeaks' start; rows in the callers' order."""
+    y32 = np.asarray(y, dtype=np.float32).astype(np.float64)
+    out = np.zeros((len(locs), 7))
+    pk = []
+    for i, c in enumerate(locs):
+        k = int(np.argmin(np.abs(x - c)))
+        if abs(x[k] - c) < 2 * step:
+            pk.append((k, i))
+    pk.sort()
+    jobs = []
+    for k, i in pk:
+        s0, e0 = max(0, k - pad), min(x.size - 1, k + pad)
+        if jobs and s0 <= jobs[-1][1]:
+            jobs[-1][1] = e0
+            jobs[-1][2].append((k, i))
+        else:
+            jobs.append([s0, e0, [(k, i)]])
+    for s0, e0, mem in jobs:
+        xs, ys = x[s0:e0 + 1], y32[s0:e0 + 1]
+        n = len(mem)
+        p0, lo, hi = O.tchn_init(xs, ys, step, np.array([x[k] for k, _ in mem]))
+        p, chi, it = O.lm_fit(_tch_code(n), xs, ys, p0, lo, hi, max_iter=200)
+        if it <= 0:
+            continue
+        for j, (_, i) in enumerate(mem):
+            f, eta = O.tch_eta_fwhm(p[4 * j + 2], p[4 * j + 3])
+            out[i] = [p[4 * j], p[4 * j + 1], p[4 * j + 2], p[4 * j + 3], f, eta,
+                      chi / max(xs.size - (4 * n + 2), 1)]
+    return out
+
+
+def test_fit_peaks_auto_finds_the_peaks(oracle_lib):
     """fitPeaksAutoDetect: SNIP removes the sloped background, detection
-    finds the three peaks, the fits put the centres on them."""
+    finds the three peaks (and one SNIP edge bump), the merged TCH fits put
+    the centres on them.  Noise-free, the selected rows equal those of the
+    same pipeline run with the FP64 LM (centres 1e-4 px, areas 1e-4).  With
+    noise, candidates on noise bumps share jobs with the peaks; such a
+    candidate can settle outside its window with a large area and no effect
+    on the sum of squares (the reference's objective and bounds allow it),
+    so the noisy checks ask only that the peaks are among the rows kept."""
+    from oracle import peak_detect as PD
     print("[fit] auto-detect", flush=True)
     x = 10.0 + 0.25 * np.arange(400)
     locs = [30.0, 55.5, 80.25]
-    y = _lineout(x, locs, seed=4)
+    y = _lineout(x, locs)
     out, n = peakfit.fit_peaks_auto(x, y, 3, 0.25, 20, 50)
     assert out.shape == (3, 7) and n == 3  # the largest areas: the three peaks
-    np.testing.assert_allclose(out[:, 1], locs, atol=0.05)
-    # over-selection also fits noise bumps; the peaks are among the five kept
-    out5, n5 = peakfit.fit_peaks_auto(x, y, 5, 0.25, 20, 50)
-    assert 3 <= n5 <= 5 and np.all(out5[n5:] == 0)
-    assert np.all(np.diff(out5[:n5, 1]) > 0)
+    corr = np.maximum(y - PD.snip_background(y, 50), 0.0)
+    cand = PD.detect_peaks(x, corr, 9, 0.75)
+    rows = _oracle_fit_peaks(oracle_lib, x, corr, cand, 0.25, 20)
+    keep = PD.select_fits(rows, 3, 0.25)
+    want = rows[keep]
+    np.testing.assert_allclose(out[:, 1], want[:, 1], atol=CENTER_TOL)
+    np.testing.assert_allclose(out[:, 0], want[:, 0], rtol=1e-4)
+    np.testing.assert_allclose(out[:, 1], locs, atol=0.05)  # TCH fitted to pV peaks
+    yn = _lineout(x, locs, seed=4)
+    out5, n5 = peakfit.fit_peaks_auto(x, yn, 5, 0.25, 20, 50)
+    assert 3 <= n5 <= 5 and np.all(out5[n5:] == 0)  # rows in candidate order
     for c in locs:
         assert np.min(np.abs(out5[:n5, 1] - c)) < 0.05
-    out1, n1 = peakfit.fit_peaks_auto(x, _lineout(x, locs, seed=4), 2, 0.25, 20, 50)
-    assert n1 == 2  # the two largest areas (all equal here: any two), in x order
-    assert out1[0, 1] < out1[1, 1]
+    out1, n1 = peakfit.fit_peaks_auto(x, yn, 2, 0.25, 20, 50)
+    assert n1 == 2 and out1[0, 1] < out1[1, 1]  # the two largest areas, in x order
 
 
 def test_stream_fitter_batches_frames(tmp_path):
@@ -170,8 +257,9 @@ def test_pv_doublet_recovers_both_centres():
     guess = np.stack([c1 - 0.3, c2 + 0.3], axis=1)
     out = peakfit.fit_windows(x, y, guess, step, model="pv2", max_iter=200)
     assert np.all(out[:, 7] > 0)
-    np.testing.assert_allclose(out[:, 0], c1, atol=2e-3)
-    np.testing.assert_allclose(out[:, 1], c2, atol=2e-3)
+    _worst("pV doublet", np.maximum(np.abs(out[:, 0] - c1), np.abs(out[:, 1] - c2)))
+    np.testing.assert_allclose(out[:, 0], c1, atol=CENTER_TOL)
+    np.testing.assert_allclose(out[:, 1], c2, atol=CENTER_TOL)
     np.testing.assert_allclose(out[:, 4], g1, rtol=2e-2)
     np.testing.assert_allclose(out[:, 5], g2, rtol=2e-2)
     np.testing.assert_allclose(out[:, 6], mu, atol=2e-2)
@@ -204,8 +292,154 @@ def test_fit_cake_rings_with_a_doublet():
                                   doublet_separation=8 * step)
     np.testing.assert_array_equal(both[..., 2, :], single[..., 2, :])
     assert np.all(both[..., :2, 7] > 0)
-    np.testing.assert_allclose(both[..., 0, 1], r1, atol=2e-3)
-    np.testing.assert_allclose(both[..., 1, 1], r2, atol=2e-3)
+    # the cake holds the profile rounded to FP32: centres within 1e-4 px
+    np.testing.assert_allclose(both[..., 0, 1], r1, atol=CENTER_TOL)
+    np.testing.assert_allclose(both[..., 1, 1], r2, atol=CENTER_TOL)
     assert np.all(both[..., :2, 6] > 1.0)
     # the singlet fits of the close pair are pulled by the neighbour
     assert np.abs(single[..., 0, 1] - r1).max() > np.abs(both[..., 0, 1] - r1).max()
+
+
+def _tch_jobs(n_peaks, n_win, seed, sig=(50, 400), gam=(5, 60), spacing=4.0):
+    """Windows holding n_peaks TCH peaks spacing +-0.4 x-units apart over a
+    sloped Chebyshev background; guesses are the samples nearest the
+    centres (fitPeaks' snapped bins)."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(seed)
+    step = 0.25
+    L = 41 + int(round(spacing / step)) * (n_peaks - 1)
+    X = 100.0 + step * np.arange(L)[None, :] + rng.uniform(0, 500, (n_win, 1)).round(2)
+    Y = np.zeros_like(X)
+    C = np.zeros((n_win, n_peaks))
+    for i in range(n_win):
+        cs = X[i, 20] + spacing * np.arange(n_peaks) + rng.uniform(-0.4, 0.4, n_peaks)
+        p = []
+        for c in cs:
+            p += [rng.uniform(500, 3000), c, rng.uniform(*sig), rng.uniform(*gam)]
+        Y[i] = O.tch_model(p + [40.0, 3.0], X[i])
+        C[i] = cs
+    G = np.take_along_axis(X, np.abs(X[:, None, :] - C[:, :, None]).argmin(axis=2), axis=1)
+    return X, Y, C, G, step
+
+
+def _fit_tch(X, Y, G, step, n_peaks, max_iter):
+    model = "tch" if n_peaks == 1 else f"tch{n_peaks}"
+    out = peakfit.fit_windows(X, Y, G if n_peaks > 1 else G[:, 0], step, model=model,
+                              max_iter=max_iter)
+    return out[:, None, :] if n_peaks == 1 else out
+
+
+@pytest.mark.parametrize("n_peaks", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_tch_merged_jobs_noise_free(n_peaks, oracle_lib):
+    """Merged fitPeaks jobs of 1-8 TCH peaks (PeakFit.c:356-544), noise-free,
+    peaks of 1-2 samples FWHM: every centre within 1e-4 px of the truth.
+    Excused: windows where the FP64 LM from the same start ends in the same
+    local minimum (GPU and FP64 centres within 1e-4 px) with a peak's
+    Gaussian width on its lower bound; at most 1 % of the windows."""
+    print(f"[fit] TCH merged jobs of {n_peaks}", flush=True)
+    X, Y, C, G, step = _tch_jobs(n_peaks, 256, 100 + n_peaks)
+    out = _fit_tch(X, Y, G, step, n_peaks, 500)
+    assert out.shape == (256, n_peaks, 8) and np.all(out[..., 7] > 0)
+    err = np.abs(out[..., 1] - C).max(axis=1)
+    _worst(f"TCH x{n_peaks} noise-free", err)
+    bad = np.nonzero(err >= CENTER_TOL)[0]
+    for i in bad:
+        yy = Y[i].astype(np.float32).astype(np.float64)
+        p0, lo, hi = oracle_lib.tchn_init(X[i], yy, step, G[i])
+        p, chi, it = oracle_lib.lm_fit(_tch_code(n_peaks), X[i], yy, p0, lo, hi, max_iter=500)
+        act = _active(p, lo, hi)
+        d = np.abs(out[i, :, 1] - p[1:4 * n_peaks:4]).max()
+        print(f"[fit]   window {i}: {err[i]:.3g} px from the truth; FP64 LM {d:.3g} px from "
+              f"the GPU, bounds active {act.tolist()}, sum of squares {chi:.4g}", flush=True)
+        assert d < CENTER_TOL and any(a % 4 == 2 and a < 4 * n_peaks for a in act), i
+    assert bad.size <= 0.01 * err.size
+
+
+@pytest.mark.parametrize("n_peaks", [2, 3, 5])
+def test_tch_merged_jobs_match_fp64_oracle(n_peaks, oracle_lib):
+    """Noisy merged jobs (SNR 30-300, peaks 6 x-units apart and 4-10
+    samples FWHM, so that the data resolve each peak): GPU centres
+    match the FP64 LM's from the same start in every window, except where
+    the GPU sum of squares is within SUMSQ_TIE (relative) of the FP64 one or
+    below it."""
+    print(f"[fit] TCH merged jobs of {n_peaks} vs FP64", flush=True)
+    X, Y, C, G, step = _tch_jobs(n_peaks, 200, 300 + n_peaks, sig=(2000, 8000), gam=(40, 120),
+                                 spacing=6.0)
+    rng = np.random.default_rng(n_peaks)
+    amp = Y.max(axis=1, keepdims=True) - 40
+    Y = Y + rng.normal(0, 1, Y.shape) * amp / rng.uniform(30, 300, (Y.shape[0], 1))
+    out = _fit_tch(X, Y, G, step, n_peaks, 500)
+    assert np.all(out[..., 7] > 0)
+    d = np.zeros(X.shape[0])
+    info = []
+    for i in range(X.shape[0]):
+        yy = Y[i].astype(np.float32).astype(np.float64)
+        p0, lo, hi = oracle_lib.tchn_init(X[i], yy, step, G[i])
+        p, chi, it = oracle_lib.lm_fit(_tch_code(n_peaks), X[i], yy, p0, lo, hi, max_iter=500)
+        d[i] = np.abs(out[i, :, 1] - p[1:4 * n_peaks:4]).max()
+        info.append((_active(p, lo, hi).size > 0 or it >= 500, chi))
+    _worst(f"TCH x{n_peaks} GPU vs FP64 LM", d)
+    dof = max(X.shape[1] - (4 * n_peaks + 2), 1)
+    for i in np.nonzero(d >= CENTER_TOL)[0]:
+        on_bound, chi = info[i]
+        g_chi = out[i, 0, 6] * dof
+        print(f"[fit]   window {i}: {d[i]:.3g} px, FP64 fit on a bound/cap: {on_bound}, "
+              f"sum of squares GPU {g_chi:.8g} FP64 {chi:.8g}", flush=True)
+        assert g_chi <= chi * (1 + SUMSQ_TIE), i
+
+
+def test_fit_peaks_merges_overlapping_rois(oracle_lib):
+    """fitPeaks on a lineout: peaks 3 and 4 x-units apart share one job (ROIs
+    of +-20 bins overlap), a far peak is fitted alone; rows come back in the
+    caller's order, matching the FP64 LM of each job and the truth."""
+    print("[fit] fit_peaks merging", flush=True)
+    step = 0.25
+    x = 10.0 + step * np.arange(600)
+    truth = [(1800.0, 40.1, 3000.0, 120.0), (1500.0, 43.05, 5000.0, 90.0),
+             (2400.0, 47.2, 2500.0, 150.0), (2000.0, 120.33, 4000.0, 100.0)]
+    y = 25.0 + 0.02 * (x - x[0])
+    for t in truth:
+        y = y + oracle_lib.tch_model(list(t) + [0.0, 0.0], x)
+    order = [3, 0, 2, 1]  # the caller's order
+    locs = [truth[k][1] for k in order]
+    out = peakfit.fit_peaks(x, y, locs, step, 20)
+    assert out.shape == (4, 7)
+    np.testing.assert_allclose(out[:, 1], locs, atol=CENTER_TOL)
+    jobs = peakfit.fit_jobs(x.size, [int(np.argmin(np.abs(x - c))) for c in locs], 20)
+    assert sorted(len(m) for _, _, m in jobs) == [1, 3]
+    for start, end, mem in jobs:
+        xs, ys = x[start:end + 1], y[start:end + 1].astype(np.float32).astype(np.float64)
+        g = np.array([xs[np.argmin(np.abs(xs - locs[m]))] for m in mem])
+        p0, lo, hi = oracle_lib.tchn_init(xs, ys, step, g)
+        p, chi, it = oracle_lib.lm_fit(_tch_code(len(mem)), xs, ys, p0, lo, hi, max_iter=500)
+        np.testing.assert_allclose(out[mem, 1], p[1:4 * len(mem):4], atol=CENTER_TOL)
+        np.testing.assert_allclose(out[mem, 0], p[0:4 * len(mem):4], rtol=1e-4)
+        if len(mem) > 1:  # chi2/dof shared by the job's rows
+            assert np.all(out[mem, 6] == out[mem[0], 6])
+
+
+def test_fit_cake_rings_with_a_tch_doublet(oracle_lib):
+    """pf_fit_doublet_tch (PeakFit.c:1138-1170) in the per-eta pass: two TCH
+    rings 1.5 x-units apart fitted jointly on one ROI; their rows carry the
+    doublet's centres, FWHM and SNR."""
+    torch = pytest.importorskip("torch")
+    print("[fit] cake rings with a TCH doublet", flush=True)
+    P = synth.plain_1024()
+    g = ma.build_map(P)
+    RLo, RHi, _, _ = ma.bin_edges(P)
+    rc = (RLo + RHi) / 2
+    nr, ne = P.n_r_bins, P.n_eta_bins
+    step = float(rc[1] - rc[0])
+    r1, r2 = rc[nr // 3] + 0.1, rc[nr // 3] + 0.1 + 1.5
+    prof = 30.0 + oracle_lib.tch_model([900.0, r1, 4000.0, 60.0, 0.0, 0.0], rc) \
+        + oracle_lib.tch_model([700.0, r2, 5000.0, 80.0, 0.0, 0.0], rc)
+    cakes = np.repeat(prof[None, :, None], ne, axis=2).repeat(2, axis=0).astype(np.float32)
+    out = peakfit.fit_cake_rings(torch.from_numpy(cakes).cuda(), g, rc, [r1, r2], pad=20,
+                                 model="tch", doublet_separation=8 * step)
+    assert np.all(out[..., :2, 7] > 0)
+    _worst("TCH doublet", np.abs(out[..., 0, 1] - r1).ravel())
+    np.testing.assert_allclose(out[..., 0, 1], r1, atol=CENTER_TOL)
+    np.testing.assert_allclose(out[..., 1, 1], r2, atol=CENTER_TOL)
+    f1, _ = oracle_lib.tch_eta_fwhm(4000.0, 60.0)
+    np.testing.assert_allclose(out[..., 0, 4], f1, rtol=1e-3)
+    assert np.all(out[..., :2, 6] > 100.0)  # noise-free: large SNR

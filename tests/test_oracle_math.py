@@ -213,3 +213,42 @@ def test_lm_recovers_noise_free_peaks(oracle_lib, model):
     p, chi, it = oracle_lib.lm_fit(model, x, y, p0, lo, hi, max_iter=300)
     assert chi < 1e-12 * np.sum(y ** 2)
     assert abs(p[0 if model == 0 else 1] - truth[0 if model == 0 else 1]) < 1e-6
+
+
+def _tch_code(n):
+    return 1 if n == 1 else n + 1  # model codes of the N-peak TCH fits
+
+
+def test_multi_peak_tch_is_a_sum_of_single_peaks(oracle_lib):
+    """The merged-job model (PeakFit.c:160-215) is the single-peak model
+    summed over peaks with one Chebyshev background."""
+    x = np.arange(120) * 0.25 + 50.0
+    a = [900.0, 60.0, 120.0, 30.0]
+    b = [1400.0, 68.5, 200.0, 10.0]
+    bg = [25.0, -3.0]
+    both = oracle_lib.tch_model(a + b + bg, x)
+    one = oracle_lib.tch_model(a + bg, x) + oracle_lib.tch_model(b + [0.0, 0.0], x)
+    np.testing.assert_allclose(both, one, rtol=1e-13, atol=1e-11)
+
+
+@pytest.mark.parametrize("n_peaks", [2, 3, 5, 8])
+def test_lm_recovers_merged_multi_peak_jobs(oracle_lib, n_peaks):
+    """fitPeaks' merged job: N TCH peaks 3-5 px apart plus a sloped
+    background, noise-free, started from the fitPeaks init at the snapped
+    bins; every centre comes back within 1e-6."""
+    rng = np.random.default_rng(n_peaks)
+    step = 0.25
+    centres = 60.0 + np.cumsum(rng.uniform(3.0, 5.0, n_peaks))
+    x = np.arange(int((centres[-1] - centres[0] + 10) / step) + 1) * step + centres[0] - 5.0
+    truth = []
+    for c in centres:
+        truth += [rng.uniform(800, 2000), c, rng.uniform(50, 300), rng.uniform(5, 60)]
+    truth += [30.0, 4.0]
+    y = oracle_lib.tch_model(truth, x)
+    locs = x[np.argmin(np.abs(x[None, :] - centres[:, None]), axis=1)]
+    p0, lo, hi = oracle_lib.tchn_init(x, y, step, locs)
+    assert np.all(lo <= p0) and np.all(p0 <= hi)
+    p, chi, it = oracle_lib.lm_fit(_tch_code(n_peaks), x, y, p0, lo, hi, max_iter=500)
+    assert it > 0
+    np.testing.assert_allclose(p[1:4 * n_peaks:4], centres, atol=1e-6)
+    np.testing.assert_allclose(p[0:4 * n_peaks:4], truth[0:4 * n_peaks:4], rtol=1e-5)
