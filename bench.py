@@ -212,7 +212,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="finish with a 1 GiB device copy (counter calibration)")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serving-frames", type=int, default=1500)
+    ap.add_argument("--serving-frames", type=int, default=4000,
+                    help="frames per offered rate (5 rates x 4000 = 20,000 frames)")
+    ap.add_argument("--serving-rates", default="100,500,1000,2000,5000")
     ap.add_argument("--no-fits", action="store_true")
     ap.add_argument("--no-qmode", action="store_true")
     args = ap.parse_args()
@@ -320,7 +322,8 @@ def main():
     fits = None if args.no_fits else fit_bench(g, P, frames, log)
     serving = None
     if not args.no_serving:
-        serving = serving_bench((500, 2000, 5000), args.serving_frames, log)
+        rates = tuple(float(r) for r in args.serving_rates.split(",") if r)
+        serving = serving_bench(rates, args.serving_frames, log)
     qmode = None if args.no_qmode else qmode_bench(log)
 
     layout = g.layout_info(args.mode)

@@ -155,3 +155,50 @@ def test_concurrent_batches_of_one_geometry():
             alone, lo = ma.integrate_batch(frames[i], g, mode="bilinear", mask_nan=True)
             np.testing.assert_allclose(r.cake, alone[0], rtol=1e-5, atol=1e-3, equal_nan=True)
             np.testing.assert_allclose(r.lineouts, lo[0], rtol=1e-5, atol=0)
+
+
+def test_mix_equals_alone_real_geometries():
+    """The serving workload's three geometries at size: 1024^2 uint16 with
+    0.1 % hot pixels masked, 2048^2 uint16 (solid angle + polarisation),
+    Pilatus 1679x1475 int32 with the module gaps masked; each with its own
+    resident map and dark.  240 frames in a shuffled mix through the server
+    (batching window 2 ms, batches of up to 64); every cake and lineout
+    equals the same frame integrated alone by the one-frame kernel within
+    1e-5 relative (no absolute floor), masked bins NaN in both."""
+    from midas_amd import synth
+    from helpers import rel_err
+    print("[server] three real geometries: generating frames", flush=True)
+    cases = []
+    for P, dt, seed in ((synth.plain_1024(), np.uint16, 1), (synth.batch_2048(), np.uint16, 2),
+                        (synth.pilatus(), np.int32, 3)):
+        fr, dark, mask = synth.frames(P, 80, seed=seed, n_rings=8, dtype=dt)
+        if P.NPanelsY:
+            mask = np.maximum(mask, synth.pilatus_gap_mask(P))
+        g = ma.build_map(P, mask=mask)
+        g.set_dark(dark.astype(dt))
+        cases.append((g, dt, fr))
+    rng = np.random.default_rng(7)
+    order = [(k, i) for k in range(3) for i in range(80)]
+    rng.shuffle(order)
+    print("[server] 240 frames through the server", flush=True)
+    with FrameServer(hbm_budget_bytes=64 << 30, max_batch=64, queue_depth=512,
+                     batch_window_ms=2.0) as srv:
+        gids = [srv.add_geometry(g, dt) for g, dt, _ in cases]
+        for rid, (k, i) in enumerate(order):
+            srv.submit(gids[k], cases[k][2][i], rid)
+        srv.drain()
+        st = srv.stats()
+        print(f"[server] {st['batches']} batches, p50 {st['p50_ms']:.2f} ms, "
+              f"p99 {st['p99_ms']:.2f} ms", flush=True)
+        assert st["frames_done"] == len(order) and st["batches"] < len(order)
+        worst = [0.0, 0.0, 0.0]
+        for rid, (k, i) in enumerate(order):
+            r = srv.results[rid]
+            assert r.status == 0 and r.geometry == gids[k]
+            alone, lo = ma.integrate_batch(cases[k][2][i], cases[k][0], mode="bilinear",
+                                           mask_nan=True)
+            e = rel_err(r.cake, alone[0], alone[0])
+            el = rel_err(r.lineouts, lo[0], lo[0])
+            worst[k] = max(worst[k], float(e.max()), float(el.max()))
+        print(f"[server] worst relative difference per geometry: {worst}", flush=True)
+        assert max(worst) <= 1e-5
