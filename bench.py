@@ -202,9 +202,13 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--frames", type=int, default=64)
-    ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--rows", type=int, default=0)
-    ap.add_argument("--lds", type=int, default=0)
+    ap.add_argument("--batch", type=int, default=0, help="manual tuning (skips the plan)")
+    ap.add_argument("--rows", type=int, default=0, help="manual tuning (skips the plan)")
+    ap.add_argument("--lds", type=int, default=0, help="manual tuning (skips the plan)")
+    ap.add_argument("--plan-dir", default=str(ROOT / "midas_amd" / "plans"),
+                    help="tuned plans (mc_map_autotune_plan): applied if present, else "
+                         "tuned and written")
+    ap.add_argument("--no-plan", action="store_true", help="built-in default tuning")
     ap.add_argument("--mode", default="bilinear")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -239,7 +243,16 @@ def main():
     log(f"map: {info.nnz} entries, {P.n_r_bins}x{P.n_eta_bins} bins, {t_map:.2f} s")
     frames, dark = synth.torch_frames(P, args.frames, seed=0, n_rings=12, device=dev)
     g.set_dark(dark)
-    g.set_tuning(batch=args.batch, rows_per_group=args.rows, lds_bytes=args.lds)
+    plan = None
+    if args.batch or args.rows or args.lds:
+        g.set_tuning(batch=args.batch, rows_per_group=args.rows, lds_bytes=args.lds)
+    elif not args.no_plan:
+        t0 = time.perf_counter()
+        plan = g.autotune_plan(args.plan_dir, np.uint16)
+        log(f"plan {plan['key']} {'loaded' if plan['loaded'] else 'tuned'} in "
+            f"{time.perf_counter() - t0:.1f} s: batch {plan['batch']}, rows "
+            f"{plan['rows_per_group']}, LDS {plan['lds_bytes']}, tile {plan['tile_bins']}/"
+            f"{plan['tile_entries']}, one-frame lanes {plan['plain_lanes']}")
     B, H, L = g.tuning()
     n_bins = P.n_r_bins * P.n_eta_bins
     npx = P.NrPixelsY * P.NrPixelsZ
@@ -363,7 +376,7 @@ def main():
                                f"{P.n_r_bins}x{P.n_eta_bins} bins (R 0.25 px, eta 1 deg), "
                                f"solid angle + polarisation, {args.mode}, map nnz {info.nnz}",
                    "batch": B, "rows_per_group": H, "lds_bytes": L,
-                   "frames_per_step": args.frames, "layout": layout},
+                   "frames_per_step": args.frames, "layout": layout, "plan": plan},
         "graph": {"frames_per_s": fps_g, "ms_per_step": ms_iter_g,
                   "what": "the step's batched passes replayed as one HIP graph (mc_graph_launch)"},
         "plain_fps": fps_p,

@@ -231,9 +231,28 @@ int mc_map_get_tile_shape(mc_map *m, int32_t *bins, int32_t *entries);
  * bytes per batch element (2 or 4), and 64-pixel blocks holding pixels of
  * the map (the blocks the transposes rewrite). */
 int mc_map_layout_info(mc_map *m, int32_t mode, int32_t dtype, int64_t *out);
-/* Times candidate tunings on synthetic frames of the map's shape and keeps
- * the fastest; writes the winner's figures. */
+/* Lanes per bin of the one-frame kernels (4, 8 or 16; default 8). */
+int mc_map_set_plain_lanes(mc_map *m, int32_t lanes);
+int mc_map_get_plain_lanes(mc_map *m, int32_t *lanes);
+/* Times candidate plans on synthetic frames of the map's shape and keeps
+ * the fastest: batch 32/64, rows per band 16/32, LDS budget 80/160 KiB and
+ * three tile shapes for the batched kernel (128 frames a timing), then 4/8/
+ * 16 lanes per bin for the one-frame kernel; writes the winner's batched
+ * frames/s.  No counterpart in the reference (it has no tuner). */
 int mc_map_autotune(mc_map *m, int32_t dtype, double *best_fps);
+/* Key of a tuned plan: 16 hex digits of a 64-bit FNV-1a hash of the map's
+ * size and bins, its per-bin record counts and offsets and mask flags, the
+ * pixel type, the device (architecture name, CU count) and key_extra (may
+ * be NULL; e.g. the map header's parameter hash, MapHeader.h:44-55).
+ * out holds at least 17 bytes. */
+int mc_map_plan_key(mc_map *m, int32_t dtype, const char *key_extra, char *out, int32_t cap);
+/* mc_map_autotune with a persistent plan: plan_dir/<key>.plan, when it
+ * exists, is applied without timing (*loaded = 1); otherwise the candidates
+ * are timed, the winner applied and written there with the table of every
+ * candidate (*loaded = 0; plan_dir is created if missing).  *best_fps: the
+ * plan's batched frames/s as measured when it was tuned. */
+int mc_map_autotune_plan(mc_map *m, int32_t dtype, const char *plan_dir, const char *key_extra,
+                         double *best_fps, int32_t *loaded);
 
 /* Integrates n_frames frames (each NrPixelsZ x NrPixelsY, row-major) into
  * cakes [n_frames][n_r][n_eta] float32 and, when lineouts != NULL,

@@ -247,11 +247,50 @@ class CSRGeometry:
         return b.value, r.value, l.value
 
     def autotune(self, dtype=np.uint16) -> float:
-        """Times the candidate tunings of the batched kernel on frames of
-        this geometry, keeps the fastest and returns its frames/s."""
+        """Times the candidate plans (batched kernel: batch, rows per band,
+        LDS budget, tile shape; one-frame kernel: lanes per bin) on frames
+        of this geometry, keeps the fastest and returns its frames/s."""
         fps = C.c_double()
         check(lib().mc_map_autotune(self.handle, NP_TO_CODE[np.dtype(dtype)], C.byref(fps)))
         return fps.value
+
+    def plan_key(self, dtype=np.uint16) -> str:
+        """Key of this geometry's tuned plan (the map's content, the pixel
+        type, the device and the map header's parameter hash)."""
+        out = C.create_string_buffer(32)
+        check(lib().mc_map_plan_key(self.handle, NP_TO_CODE[np.dtype(dtype)],
+                                    self._header_hash(), out, 32))
+        return out.value.decode()
+
+    def autotune_plan(self, plan_dir, dtype=np.uint16) -> dict:
+        """mc_map_autotune_plan: applies plan_dir/<key>.plan when it exists,
+        else tunes and writes it.  Returns {key, loaded, frames_per_s,
+        batch, rows_per_group, lds_bytes, tile_bins, tile_entries,
+        plain_lanes}."""
+        fps, loaded = C.c_double(), C.c_int32()
+        check(lib().mc_map_autotune_plan(self.handle, NP_TO_CODE[np.dtype(dtype)],
+                                         str(plan_dir).encode(), self._header_hash(),
+                                         C.byref(fps), C.byref(loaded)))
+        b, r, l = self.tuning()
+        tb, te = self.tile_shape()
+        return {"key": self.plan_key(dtype), "loaded": bool(loaded.value),
+                "frames_per_s": fps.value, "batch": b, "rows_per_group": r, "lds_bytes": l,
+                "tile_bins": tb, "tile_entries": te, "plain_lanes": self.plain_lanes()}
+
+    def _header_hash(self):
+        if self.params is None:
+            return None
+        from .bin_io import header_for
+        return header_for(self.params).param_hash.hex().encode()
+
+    def set_plain_lanes(self, lanes: int):
+        """Lanes per bin of the one-frame kernels: 4, 8 or 16."""
+        check(lib().mc_map_set_plain_lanes(self.handle, lanes))
+
+    def plain_lanes(self) -> int:
+        v = C.c_int32()
+        check(lib().mc_map_get_plain_lanes(self.handle, C.byref(v)))
+        return v.value
 
     def layout_info(self, mode: str = "bilinear", dtype=np.uint16) -> dict:
         """Tile layout of the batched kernel under the current tuning."""

@@ -17,6 +17,8 @@
 #include <thread>
 #include <vector>
 
+#include <sys/stat.h>
+
 #include "mc_internal.h"
 
 namespace mc {
@@ -1329,6 +1331,21 @@ int mc_map_get_tile_shape(mc_map *m, int32_t *bins, int32_t *entries) {
   return MC_OK;
 }
 
+int mc_map_set_plain_lanes(mc_map *m, int32_t lanes) {
+  MC_CHECK(m, MC_ERR_ARG, "null map");
+  MC_CHECK(lanes == 4 || lanes == 8 || lanes == 16, MC_ERR_ARG,
+           "lanes per bin %d not in {4, 8, 16}", lanes);
+  std::lock_guard<std::mutex> lk(m->mu);
+  m->plain_lanes = lanes;
+  return MC_OK;
+}
+
+int mc_map_get_plain_lanes(mc_map *m, int32_t *lanes) {
+  MC_CHECK(m && lanes, MC_ERR_ARG, "null argument");
+  *lanes = m->plain_lanes;
+  return MC_OK;
+}
+
 int mc_map_get_tuning(mc_map *m, int32_t *batch, int32_t *rows_per_group, int32_t *lds_bytes) {
   MC_CHECK(m, MC_ERR_ARG, "null map");
   if (batch) *batch = m->batch;
@@ -1729,50 +1746,206 @@ int mc_time_integrate(mc_map *m, const void *frames_dev, int32_t dtype, int32_t 
 // Times the batched kernel for each candidate (batch, rows per workgroup,
 // LDS budget) on device frames of the map's shape (constant content: the
 // kernel's work does not depend on pixel values) and keeps the fastest.
-int mc_map_autotune(mc_map *m, int32_t dtype, double *best_fps) {
-  MC_CHECK(m, MC_ERR_ARG, "null map");
+}  // extern "C"
+
+namespace {
+
+struct Plan {
+  int batch, rows, lds, bins, ents, lanes;
+  double fps, plain_fps;
+};
+
+// Times the candidate plans on synthetic frames of the map's shape (128
+// frames per batched timing, 16 per one-frame timing) and applies the
+// fastest; the table of every candidate goes to *table.
+int autotune_run(mc_map *m, int32_t dtype, Plan *best_out, std::string *table) {
   const size_t es = dtype_size(dtype);
   MC_CHECK(es > 0, MC_ERR_ARG, "unsupported dtype %d", dtype);
-  const int keep_b = m->batch, keep_h = m->rows_per_group, keep_l = m->lds_bytes;
   DBuf frames, cakes;
   int rc;
-  if ((rc = frames.alloc((size_t)128 * m->npx() * es)) ||
-      (rc = cakes.alloc((size_t)128 * m->nb() * 4)))
+  constexpr int kFrames = 128, kPlainFrames = 16;
+  if ((rc = frames.alloc((size_t)kFrames * m->npx() * es)) ||
+      (rc = cakes.alloc((size_t)kFrames * m->nb() * 4)))
     return rc;
   MC_HIP(hipMemset(frames.p, 0x11, frames.bytes));
-  const int Bs[] = {32, 64, 128}, Hs[] = {16, 32}, Ls[] = {81920, 163840};
-  double best = -1;
-  int bb = keep_b, bh = keep_h, bl = keep_l;
+  MC_HIP(hipDeviceSynchronize());
+  const int Bs[] = {32, 64}, Hs[] = {16, 32}, Ls[] = {81920, 163840};
+  const int Shapes[][2] = {{256, 2048}, {128, 1024}, {512, 4096}};
+  Plan best{m->batch, m->rows_per_group, m->lds_bytes, m->tile_bins, m->tile_ents,
+            m->plain_lanes, -1.0, -1.0};
+  char line[160];
   for (int B : Bs)
     for (int H : Hs)
-      for (int L : Ls) {
-        if (mc_map_set_tuning(m, B, H, L)) continue;
-        double per_launch = 0, per_iter = 0;
-        if (mc_time_integrate(m, frames.p, dtype, 128, MC_MODE_BILINEAR,
-                              MC_F_NORMALIZE | MC_F_MASK_NAN, 1, 3, 1, cakes.as<float>(),
-                              nullptr, &per_launch, &per_iter))
-          continue;  // e.g. an LDS budget the device does not grant
-        const double fps = 128.0 / (per_iter / 1e3);
-        if (fps > best) {
-          best = fps;
-          bb = B;
-          bh = H;
-          bl = L;
+      for (int L : Ls)
+        for (const auto &sh : Shapes) {
+          if (mc_map_set_tuning(m, B, H, L) || mc_map_set_tile_shape(m, sh[0], sh[1])) continue;
+          double per_launch = 0, per_iter = 0;
+          if (mc_time_integrate(m, frames.p, dtype, kFrames, MC_MODE_BILINEAR,
+                                MC_F_NORMALIZE | MC_F_MASK_NAN, 1, 3, 1, cakes.as<float>(),
+                                nullptr, &per_launch, &per_iter))
+            continue;  // e.g. an LDS budget the device does not grant
+          const double fps = kFrames / (per_iter / 1e3);
+          snprintf(line, sizeof line, "# batched B %d rows %d lds %d tile %d/%d: %.0f frames/s\n",
+                   B, H, L, sh[0], sh[1], fps);
+          *table += line;
+          if (fps > best.fps) best = Plan{B, H, L, sh[0], sh[1], m->plain_lanes, fps, -1.0};
         }
-      }
-  MC_CHECK(best > 0, MC_ERR_STATE, "no candidate tuning ran: %s", g_err.c_str());
-  mc_map_set_tuning(m, bb, bh, bl);
+  MC_CHECK(best.fps > 0, MC_ERR_STATE, "no candidate tuning ran: %s", g_err.c_str());
+  mc_map_set_tuning(m, best.batch, best.rows, best.lds);
+  mc_map_set_tile_shape(m, best.bins, best.ents);
+  for (int lanes : {4, 8, 16}) {
+    mc_map_set_plain_lanes(m, lanes);
+    double per_launch = 0, per_iter = 0;
+    if (mc_time_integrate(m, frames.p, dtype, kPlainFrames, MC_MODE_BILINEAR,
+                          MC_F_NORMALIZE | MC_F_MASK_NAN, 0, 3, 1, cakes.as<float>(), nullptr,
+                          &per_launch, &per_iter))
+      continue;
+    const double fps = kPlainFrames / (per_iter / 1e3);
+    snprintf(line, sizeof line, "# one-frame lanes %d: %.0f frames/s\n", lanes, fps);
+    *table += line;
+    if (fps > best.plain_fps) {
+      best.plain_fps = fps;
+      best.lanes = lanes;
+    }
+  }
+  mc_map_set_plain_lanes(m, best.lanes);
   {
     // Drop the layouts of the losers.
     std::lock_guard<std::mutex> lk(m->mu);
     for (auto &v : m->tiles) {
       std::vector<std::unique_ptr<Tiles>> keep;
       for (auto &t : v)
-        if (t->batch == bb && t->rows == bh && t->lds_bytes == bl) keep.push_back(std::move(t));
+        if (t->batch == best.batch && t->rows == best.rows && t->lds_bytes == best.lds &&
+            t->bin_cap == best.bins && t->ent_cap == best.ents)
+          keep.push_back(std::move(t));
       v.swap(keep);
     }
   }
-  if (best_fps) *best_fps = best;
+  *best_out = best;
+  return MC_OK;
+}
+
+// FNV-1a, 64 bits.
+struct Fnv {
+  uint64_t h = 1469598103934665603ull;
+  void add(const void *p, size_t n) {
+    const unsigned char *c = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+  }
+  template <typename T> void add(const T &v) { add(&v, sizeof v); }
+};
+
+int plan_key(mc_map *m, int32_t dtype, const char *key_extra, std::string *key) {
+  hipDeviceProp_t prop;
+  MC_HIP(hipGetDeviceProperties(&prop, m->ctx->device));
+  Fnv f;
+  const char tag[] = "midas_amd batched plan 1";
+  f.add(tag, sizeof tag);
+  f.add(m->ny);
+  f.add(m->nz);
+  f.add(m->n_r);
+  f.add(m->n_eta);
+  f.add(m->nnz);
+  f.add(m->q_mode);
+  f.add(m->fused);
+  f.add(m->h_pairs.data(), m->h_pairs.size() * sizeof(int32_t));
+  f.add(m->h_flags.data(), m->h_flags.size() * sizeof(int32_t));
+  f.add(dtype);
+  f.add(prop.gcnArchName, strnlen(prop.gcnArchName, sizeof prop.gcnArchName));
+  f.add(prop.multiProcessorCount);
+  if (key_extra) f.add(key_extra, strlen(key_extra));
+  char hex[17];
+  snprintf(hex, sizeof hex, "%016llx", (unsigned long long)f.h);
+  *key = hex;
+  return MC_OK;
+}
+
+bool read_plan(const std::string &path, Plan *p) {
+  FILE *fp = fopen(path.c_str(), "r");
+  if (!fp) return false;
+  Plan q{0, 0, 0, 0, 0, 0, 0.0, 0.0};
+  int seen = 0;
+  char line[256];
+  while (fgets(line, sizeof line, fp)) {
+    char k[64];
+    double v;
+    if (line[0] == '#' || sscanf(line, "%63s %lf", k, &v) != 2) continue;
+    const std::string key(k);
+    if (key == "batch") q.batch = (int)v, seen |= 1;
+    else if (key == "rows_per_group") q.rows = (int)v, seen |= 2;
+    else if (key == "lds_bytes") q.lds = (int)v, seen |= 4;
+    else if (key == "tile_bins") q.bins = (int)v, seen |= 8;
+    else if (key == "tile_entries") q.ents = (int)v, seen |= 16;
+    else if (key == "plain_lanes") q.lanes = (int)v, seen |= 32;
+    else if (key == "frames_per_s") q.fps = v;
+    else if (key == "plain_frames_per_s") q.plain_fps = v;
+  }
+  fclose(fp);
+  if (seen != 63) return false;
+  *p = q;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mc_map_autotune(mc_map *m, int32_t dtype, double *best_fps) {
+  MC_CHECK(m, MC_ERR_ARG, "null map");
+  Plan best;
+  std::string table;
+  int rc = autotune_run(m, dtype, &best, &table);
+  if (rc) return rc;
+  if (best_fps) *best_fps = best.fps;
+  return MC_OK;
+}
+
+int mc_map_plan_key(mc_map *m, int32_t dtype, const char *key_extra, char *out, int32_t cap) {
+  MC_CHECK(m && out && cap >= 17, MC_ERR_ARG, "bad arguments");
+  MC_CHECK(dtype_size(dtype) > 0, MC_ERR_ARG, "unsupported dtype %d", dtype);
+  std::string key;
+  int rc = plan_key(m, dtype, key_extra, &key);
+  if (rc) return rc;
+  snprintf(out, (size_t)cap, "%s", key.c_str());
+  return MC_OK;
+}
+
+int mc_map_autotune_plan(mc_map *m, int32_t dtype, const char *plan_dir, const char *key_extra,
+                         double *best_fps, int32_t *loaded) {
+  MC_CHECK(m && plan_dir && *plan_dir, MC_ERR_ARG, "bad arguments");
+  MC_CHECK(dtype_size(dtype) > 0, MC_ERR_ARG, "unsupported dtype %d", dtype);
+  std::string key;
+  int rc = plan_key(m, dtype, key_extra, &key);
+  if (rc) return rc;
+  const std::string path = std::string(plan_dir) + "/" + key + ".plan";
+  Plan p;
+  if (read_plan(path, &p)) {
+    if (!(rc = mc_map_set_tuning(m, p.batch, p.rows, p.lds)) &&
+        !(rc = mc_map_set_tile_shape(m, p.bins, p.ents)) &&
+        !(rc = mc_map_set_plain_lanes(m, p.lanes))) {
+      if (best_fps) *best_fps = p.fps;
+      if (loaded) *loaded = 1;
+      return MC_OK;
+    }
+    return rc;  // a plan file with values this library refuses
+  }
+  std::string table;
+  if ((rc = autotune_run(m, dtype, &p, &table))) return rc;
+  mkdir(plan_dir, 0755);  // may exist already
+  const std::string tmp = path + ".tmp";
+  FILE *fp = fopen(tmp.c_str(), "w");
+  MC_CHECK(fp, MC_ERR_IO, "cannot write %s", tmp.c_str());
+  fprintf(fp,
+          "# tuned plan of the batched and one-frame kernels, key %s\n"
+          "# (map size, bins, records per bin, mask flags, pixel type, device)\n"
+          "batch %d\nrows_per_group %d\nlds_bytes %d\ntile_bins %d\ntile_entries %d\n"
+          "plain_lanes %d\nframes_per_s %.1f\nplain_frames_per_s %.1f\n%s",
+          key.c_str(), p.batch, p.rows, p.lds, p.bins, p.ents, p.lanes, p.fps, p.plain_fps,
+          table.c_str());
+  const bool ok = fclose(fp) == 0 && rename(tmp.c_str(), path.c_str()) == 0;
+  MC_CHECK(ok, MC_ERR_IO, "cannot write %s", path.c_str());
+  if (best_fps) *best_fps = p.fps;
+  if (loaded) *loaded = 0;
   return MC_OK;
 }
 

@@ -174,6 +174,7 @@ struct Map {
   // tuning
   int batch = 64, rows_per_group = 16, lds_bytes = 81920;
   int tile_bins = 256, tile_ents = 2048;  // LDS slots of a tile: bin records, entries
+  int plain_lanes = 8;                    // lanes per bin of the one-frame kernels
   Work work;  // scratch of the calls made through the map's context stream
   int64_t nb() const { return (int64_t)n_r * n_eta; }
   int64_t npx() const { return (int64_t)ny * nz; }

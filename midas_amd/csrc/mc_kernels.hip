@@ -108,10 +108,10 @@ __device__ __forceinline__ void block_sum4(float4 &v, float4 *red) {
 
 // ---------------------------------------------------------------------
 // One-frame kernel: workgroup = one R row of one frame, LPB lanes per bin
-// (8: a bin has about 15 entries; 16 lanes leave half of them idle, 4
-// lengthen each lane's chain of dependent loads).  grid = (n_r, n_frames).
+// (4, 8 or 16, a tuning of the map; 8 by default: a bin of about 15
+// entries leaves half of 16 lanes idle, while 4 lanes lengthen each lane's
+// chain of dependent loads).  grid = (n_r, n_frames).
 // ---------------------------------------------------------------------
-constexpr int kPlainLanes = 8;
 //
 // With a dark the kernel reads frames already dark-subtracted by
 // k_frame_dark (FP32, count minus the pixel's integer dark, exact): a second
@@ -168,14 +168,14 @@ __global__ __launch_bounds__(256) void k_plain(
 // or a variance image.  With a dark the frames are k_frame_dark's output.
 // grid = (n_r, n_frames).
 // ---------------------------------------------------------------------
-template <typename T, bool POISSON>
+template <typename T, bool POISSON, int LPB>
 __global__ __launch_bounds__(256) void k_plain_var(
     const int32_t *__restrict__ row, const int32_t *__restrict__ pix,
     const float *__restrict__ w, const float2 *__restrict__ binfo,
     const float *__restrict__ d2term, const T *__restrict__ frames, const float *__restrict__ vimg,
     int64_t frame_stride, int n_eta, int64_t nb, float *__restrict__ cake,
     float *__restrict__ var) {
-  constexpr int LPB = kPlainLanes, SLOTS = 256 / LPB;  // the one-frame kernel's order
+  constexpr int SLOTS = 256 / LPB;  // the one-frame kernel's order for the same lanes
   const int r = blockIdx.x, f = blockIdx.y;
   const T *__restrict__ img = frames + (int64_t)f * frame_stride;
   const float *__restrict__ vi = POISSON ? nullptr : vimg + (int64_t)f * frame_stride;
@@ -1010,9 +1010,11 @@ int launch_binfo(hipStream_t s, const float *inv_area, const int32_t *flags,
   return MC_OK;
 }
 
-int launch_plain(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo, int mask_nan,
-                 const void *frames, int dtype, float *cake, float *lineouts, int n_frames,
-                 int64_t frame_stride, float *sub, int64_t sub_frames) {
+template <int LPB>
+static int launch_plain_l(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo,
+                          int mask_nan, const void *frames, int dtype, float *cake,
+                          float *lineouts, int n_frames, int64_t frame_stride, float *sub,
+                          int64_t sub_frames) {
   const int64_t npx = m.npx();
   if (m.has_dark) {
     if (!sub || sub_frames < 1) {
@@ -1031,7 +1033,7 @@ int launch_plain(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo,
         return MC_OK;
       });
       if (rc) return rc;
-      hipLaunchKernelGGL((k_plain<float, kPlainLanes>), dim3(m.n_r, nf), dim3(256), 0, s,
+      hipLaunchKernelGGL((k_plain<float, LPB>), dim3(m.n_r, nf), dim3(256), 0, s,
                          c.row.as<int32_t>(), c.pix.as<int32_t>(), c.w.as<float>(), binfo,
                          m.d_area_inb.as<float>(), m.d_row_static.as<float4>(), sub, npx,
                          m.n_eta, m.n_r, m.nb(), mask_nan, cake + (size_t)f0 * m.nb(),
@@ -1042,12 +1044,32 @@ int launch_plain(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo,
   }
   return dispatch_dtype(dtype, [&](auto tag) -> int {
     using T = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
-    hipLaunchKernelGGL((k_plain<T, kPlainLanes>), dim3(m.n_r, n_frames), dim3(256), 0, s,
+    hipLaunchKernelGGL((k_plain<T, LPB>), dim3(m.n_r, n_frames), dim3(256), 0, s,
                        c.row.as<int32_t>(), c.pix.as<int32_t>(), c.w.as<float>(), binfo,
                        m.d_area_inb.as<float>(), m.d_row_static.as<float4>(), (const T *)frames,
                        frame_stride, m.n_eta, m.n_r, m.nb(), mask_nan, cake, lineouts);
     MC_HIP(hipGetLastError());
     return MC_OK;
+  });
+}
+
+// Calls f(std::integral_constant<int, L>) for the map's lanes per bin.
+template <typename F> int with_plain_lanes(int lanes, F &&f) {
+  switch (lanes) {
+    case 4: return f(std::integral_constant<int, 4>());
+    case 8: return f(std::integral_constant<int, 8>());
+    case 16: return f(std::integral_constant<int, 16>());
+    default: set_error("lanes per bin %d not in {4, 8, 16}", lanes); return MC_ERR_ARG;
+  }
+}
+
+int launch_plain(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo, int mask_nan,
+                 const void *frames, int dtype, float *cake, float *lineouts, int n_frames,
+                 int64_t frame_stride, float *sub, int64_t sub_frames) {
+  return with_plain_lanes(m.plain_lanes, [&](auto lanes) -> int {
+    constexpr int LPB = decltype(lanes)::value;
+    return launch_plain_l<LPB>(s, m, c, binfo, mask_nan, frames, dtype, cake, lineouts,
+                               n_frames, frame_stride, sub, sub_frames);
   });
 }
 
@@ -1066,12 +1088,15 @@ int launch_plain_var(hipStream_t s, const Map &m, const Csr &c, const float2 *bi
   auto run = [&](auto tag, const void *fr, const float *vi, float *ck, float *vr, int nf) -> int {
     using T = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
     const dim3 grid(m.n_r, nf);
-    auto kern = vi ? k_plain_var<T, false> : k_plain_var<T, true>;
-    hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, c.row.as<int32_t>(), c.pix.as<int32_t>(),
-                       c.w.as<float>(), binfo, d2term, (const T *)fr, vi, npx, m.n_eta, nb, ck,
-                       vr);
-    MC_HIP(hipGetLastError());
-    return MC_OK;
+    return with_plain_lanes(m.plain_lanes, [&](auto lanes) -> int {
+      constexpr int LPB = decltype(lanes)::value;
+      auto kern = vi ? k_plain_var<T, false, LPB> : k_plain_var<T, true, LPB>;
+      hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, c.row.as<int32_t>(), c.pix.as<int32_t>(),
+                         c.w.as<float>(), binfo, d2term, (const T *)fr, vi, npx, m.n_eta, nb,
+                         ck, vr);
+      MC_HIP(hipGetLastError());
+      return MC_OK;
+    });
   };
   if (!m.has_dark)
     return dispatch_dtype(dtype, [&](auto tag) -> int {
