@@ -307,7 +307,18 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
     int ra = 0, nrows = 0, e0 = 0, nc = 0;
     std::vector<int32_t> px;  // footprint, sorted
     int64_t ents = 0;         // staged entries
+    bool chunk_start = false; // first tile of a workgroup's run
   };
+  // Chunks: runs of consecutive tiles of one band of about `target` cost
+  // (a few thousand workgroups for a 2048^2 map).  The cost of a tile
+  // counts its entries plus a fixed charge per bin and per tile: a bin of 2
+  // entries costs a wave about as much as one of 30.  A chunk ends only
+  // before an eta column that is a multiple of 8: the kernel writes whole
+  // 8-bin granules of a row, and a granule split between two workgroups
+  // costs one 4-byte store per bin.
+  constexpr int64_t kBinCost = 24, kTileCost = 256;
+  const int64_t target =
+      std::max<int64_t>(4096, (c.nnz + kBinCost * nb + kTileCost * (nb / kTileBins + 1)) / 2048);
   const int n_bands = (nR + H - 1) / H;
   std::vector<std::vector<TileH>> bands(n_bands);
   parallel_for(n_bands, [&](int64_t a, int64_t bq) {
@@ -320,16 +331,30 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
                union_size(T.px, add) <= cap_px;
       };
       TileH cur;
+      bool next_starts = true;  // the next tile begins a chunk
+      int64_t acc = 0;          // cost since the current chunk began
+      auto push = [&](TileH &&T) {
+        T.chunk_start = next_starts;
+        next_starts = false;
+        acc += kTileCost;
+        tv.push_back(std::move(T));
+      };
       auto flush = [&]() {
-        if (cur.nc > 0) tv.push_back(std::move(cur));
+        if (cur.nc > 0) push(std::move(cur));
         cur = TileH();
       };
       std::vector<int32_t> col;
       for (int e = 0; e < nE; e++) {
+        if (e % 8 == 0 && acc >= target) {  // end the chunk here
+          flush();
+          next_starts = true;
+          acc = 0;
+        }
         col.clear();
         int64_t ce = 0;
         for (int j = 0; j < nrows; j++) {
           const int64_t b = (int64_t)(row0 + j) * nE + e;
+          acc += cnt(b) + kBinCost;
           if (big[b]) continue;
           merge_into(col, fp[b]);
           ce += cntp(b);
@@ -361,7 +386,7 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
           const std::vector<int32_t> &add = big[b] ? none : fp[b];
           const int64_t ae = big[b] ? 0 : cntp(b);
           if (piece.nrows > 0 && !room(piece, add, ae, 1)) {
-            tv.push_back(std::move(piece));
+            push(std::move(piece));
             piece = TileH();
             piece.e0 = e;
             piece.nc = 1;
@@ -371,21 +396,12 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
           piece.ents += ae;
           piece.nrows++;
         }
-        tv.push_back(std::move(piece));
+        push(std::move(piece));
       }
       flush();
     }
   });
-  // Flatten.  Chunks: runs of consecutive tiles of one band of about
-  // `target` cost (a few thousand workgroups for a 2048^2 map).  The cost
-  // of a tile counts its entries plus a fixed charge per bin and per tile:
-  // a bin of 2 entries costs a wave about as much as one of 30.
-  constexpr int64_t kBinCost = 24, kTileCost = 256;
-  int64_t total_cost = 0;
-  for (auto &tv : bands)
-    for (auto &T : tv) total_cost += kTileCost + kBinCost * T.nrows * T.nc;
-  total_cost += c.nnz;
-  const int64_t target = std::max<int64_t>(4096, total_cost / 2048);
+  // Flatten, starting a chunk at each tile the band planner marked.
   std::vector<int64_t> chunk_cost;
   std::vector<int32_t> chunks, band_chunk(n_bands + 1, 0), tiles;
   std::vector<int4> tbins;
@@ -396,13 +412,11 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   int64_t n_direct = 0, max_npx = 0, staged = 0, real_ents = 0;
   for (int gi = 0; gi < n_bands; gi++) {
     const int row0 = gi * H, nrows_band = std::min(H, nR - row0);
-    int64_t acc = target;  // forces a new chunk at the band start
     for (auto &T : bands[gi]) {
-      if (acc >= target) {
+      if (T.chunk_start) {
         chunks.insert(chunks.end(), {(int32_t)(tiles.size() / kTileInts),
                                      (int32_t)(tiles.size() / kTileInts), row0, nrows_band});
         chunk_cost.push_back(0);
-        acc = 0;
       }
       const int32_t npx = (int32_t)T.px.size();
       const int32_t px_beg = (int32_t)tpix.size();
@@ -457,7 +471,6 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
                                  T.ra, T.nc, 0, 0, T.nrows, 0, 0, 0});
       chunks[chunks.size() - 3] = (int32_t)(tiles.size() / kTileInts);
       const int64_t tcost = all_ents + kTileCost + kBinCost * T.nrows * T.nc;
-      acc += tcost;
       chunk_cost.back() += tcost;
     }
     band_chunk[gi + 1] = (int32_t)(chunks.size() / 4);

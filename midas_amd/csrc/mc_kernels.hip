@@ -329,11 +329,14 @@ constexpr int kWaves = 16;
 constexpr int kTileInts = 16;
 constexpr int kXtFrames = 64;     // frames per pixel row of the batch
 
+// A bin's record in a tile: {cake bin, entry begin, entry end, direct},
+// {scale, dark term}, and for the lineouts the bin's in-bounds area where
+// it counts (above 1e-9; 0 otherwise) and 1/0 for the same condition.
 struct BinRec {
   int4 tb;
   float2 info;
   float a;
-  float pad;
+  float af;
 };
 
 typedef int i4v __attribute__((ext_vector_type(4)));
@@ -672,15 +675,18 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
   }
+  // The next tile's header is read while the current one is staged, and
+  // carried to the next iteration.
+  TileHdr hn = tile_hdr(tiles, t_beg);
   for (int t = t_beg; t < t_end; ++t) {
-    const TileHdr h = tile_hdr(tiles, t);
+    const TileHdr h = hn;
     const int cur = (t - t_beg) & 1;
     if (t > t_beg) __syncthreads();  // every wave is done with the previous tile's LDS
     tile_stage<TL>(h, wave, lane, lds_raw, reinterpret_cast<char *>(lent), plane, recs, ents,
                    xt, reinterpret_cast<const int32_t *>(lds_raw + tp_off + cur * tp_len));
     if (t + 1 < t_end) {
-      const int32_t *Tn = tiles + kTileInts * (t + 1);
-      stage_pixels(tpix, Tn[0], Tn[1], wave, lane, lds_raw + tp_off + (cur ^ 1) * tp_len);
+      hn = tile_hdr(tiles, t + 1);
+      stage_pixels(tpix, hn.px_beg, hn.npx, wave, lane, lds_raw + tp_off + (cur ^ 1) * tp_len);
     }
     __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
     __syncthreads();
@@ -787,14 +793,14 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
         om[g] |= 1u << gk;
         const float o = (mine - r.info.y) * r.info.x;  // NaN scale: a masked bin
         ob[g][gk] = o;
-        if (o != 0.f && !isnan(o)) {
-          lo[g][0] += o;
-          lo[g][1] += 1.f;
-        }
-        if (r.a > 1e-9f && !isnan(o)) {
-          lo[g][2] += o * r.a;
-          lo[g][3] += o;
-        }
+        // lineout sums without branches: masked bins add nothing, zeros
+        // add nothing to the sums (only the count skips them), r.a and
+        // r.af are 0 where the in-bounds area is not above 1e-9
+        const float on = isnan(o) ? 0.f : o;
+        lo[g][0] += on;
+        lo[g][1] += on != 0.f ? 1.f : 0.f;
+        lo[g][2] = fmaf(on, r.a, lo[g][2]);
+        lo[g][3] = fmaf(on, r.af, lo[g][3]);
       }
     }
   }
@@ -839,8 +845,9 @@ __global__ void k_tile_recs(const int4 *__restrict__ tbins, const float2 *__rest
   BinRec r;
   r.tb = tbins[i];
   r.info = binfo[r.tb.x];
-  r.a = a_inb[r.tb.x];
-  r.pad = 0.f;
+  const float a = a_inb[r.tb.x];
+  r.a = a > 1e-9f ? a : 0.f;
+  r.af = a > 1e-9f ? 1.f : 0.f;
   out[i] = r;
 }
 
