@@ -376,6 +376,34 @@ __global__ __launch_bounds__(256) void k_transpose(const TIn *__restrict__ frame
       tile[f][q0 + 2 * q] = (TL)(w[q] & 0xffffu);
       tile[f][q0 + 2 * q + 1] = (TL)(w[q] >> 16);
     }
+  } else if (vec_ok && sizeof(TL) == 4) {
+    // Vector loads: thread -> (frames t/16 + 16k, 4 consecutive pixels), so
+    // 16 lanes read 64 pixels of one frame in one piece; the pixels' dark
+    // is read once for the four passes.
+    typedef TIn V4 __attribute__((ext_vector_type(4)));
+    const int fs = t >> 4, q = (t & 15) * 4;
+    float dk[4] = {0.f, 0.f, 0.f, 0.f};
+    if (dint) {
+      const u2v d2 = *reinterpret_cast<const u2v *>(dint + p0 + q);
+      dk[0] = (float)(d2.x & 0xffffu);
+      dk[1] = (float)(d2.x >> 16);
+      dk[2] = (float)(d2.y & 0xffffu);
+      dk[3] = (float)(d2.y >> 16);
+    }
+#pragma unroll
+    for (int k = 0; k < kXtFrames / 16; ++k) {
+      const int f = fs + 16 * k;
+      float x[4] = {0.f, 0.f, 0.f, 0.f};  // frames past the batch stay zero
+      if (f < n_frames) {
+        const V4 v = *reinterpret_cast<const V4 *>(frames + (int64_t)f * frame_stride + p0 + q);
+        x[0] = static_cast<float>(v.x) - dk[0];
+        x[1] = static_cast<float>(v.y) - dk[1];
+        x[2] = static_cast<float>(v.z) - dk[2];
+        x[3] = static_cast<float>(v.w) - dk[3];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) tile[f][q + j] = (TL)x[j];
+    }
   } else {
     const int f = t >> 2, q0 = (t & 3) * 16;
     const TIn *src = frames + (int64_t)f * frame_stride + p0 + q0;
