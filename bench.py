@@ -240,7 +240,10 @@ def main():
     g = ma.build_map(P)
     t_map = time.perf_counter() - t0
     info = g.info()
-    log(f"map: {info.nnz} entries, {P.n_r_bins}x{P.n_eta_bins} bins, {t_map:.2f} s")
+    verify = g.verify_info()
+    log(f"map: {info.nnz} entries, {P.n_r_bins}x{P.n_eta_bins} bins, {t_map:.2f} s; host "
+        f"verification of {verify['pixels']} pixels: {verify['membership_differs']} membership "
+        f"and {verify['values_differ']} value differences")
     frames, dark = synth.torch_frames(P, args.frames, seed=0, n_rings=12, device=dev)
     g.set_dark(dark)
     plan = None
@@ -392,6 +395,7 @@ def main():
         "serving": serving,
         "qmode_2880": qmode,
         "map_build_s": t_map,
+        "map_verify": verify,
     }
     print(json.dumps(out))
 

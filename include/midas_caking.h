@@ -170,6 +170,19 @@ int mc_map_from_host(mc_ctx *ctx, const void *records, int64_t nnz,
                      const int32_t *pairs, const int32_t *flags, int32_t ny,
                      int32_t nz, int32_t n_r, int32_t n_eta, mc_map **out);
 int mc_map_info_get(mc_map *m, mc_map_info *info);
+/* Host verification of GPU map builds (the GPU mapper's membership is
+ * pinned to the host's libm, MapperCore.c:45-424 run on the CPU): each
+ * later mc_map_build on this context recomputes n_pixels pixels, spread over
+ * the detector, with the same per-pixel code built for the host, and
+ * compares them with the GPU's records (default 4096; 0 turns it off;
+ * n_pixels >= NY*NZ checks every pixel).  mc_map_verify_info: out[4] =
+ * pixels checked, records compared, pixels whose bin membership differs,
+ * records whose values differ beyond 1e-6 px^2 (areas) or 1e-5 px
+ * (deltaR).  Differences are reported, not refused: where a vertex lies
+ * exactly on a bin boundary the two libms' last bits can flip a case.
+ * Maps not built by mc_map_build report zeros. */
+int mc_ctx_set_map_verify(mc_ctx *ctx, int64_t n_pixels);
+int mc_map_verify_info(mc_map *m, int64_t *out);
 /* Copies the map back: records (nnz*24 B), pairs (2*n_bins), flags. */
 int mc_map_download(mc_map *m, void *records, int32_t *pairs,
                     int32_t *flags);

@@ -137,6 +137,8 @@ SIGNATURES = {
     "mc_map_set_tuning": (_I, [_V, _I32, _I32, _I32]),
     "mc_map_get_tuning": (_I, [_V, _P(_I32), _P(_I32), _P(_I32)]),
     "mc_map_autotune": (_I, [_V, _I32, _PD]),
+    "mc_ctx_set_map_verify": (_I, [_V, _I64]),
+    "mc_map_verify_info": (_I, [_V, _P(_I64)]),
     "mc_map_set_plain_lanes": (_I, [_V, _I32]),
     "mc_map_get_plain_lanes": (_I, [_V, C.POINTER(C.c_int32)]),
     "mc_map_plan_key": (_I, [_V, _I32, C.c_char_p, C.c_char_p, _I32]),

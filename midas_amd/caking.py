@@ -67,6 +67,11 @@ class Context:
     def sync(self):
         check(lib().mc_ctx_sync(self.handle))
 
+    def set_map_verify(self, n_pixels: int):
+        """Pixels of each later GPU map build recomputed on the host and
+        compared (mc_ctx_set_map_verify; default 4096, 0 = off)."""
+        check(lib().mc_ctx_set_map_verify(self.handle, int(n_pixels)))
+
     def close(self):
         if self.handle:
             lib().mc_ctx_destroy(self.handle)
@@ -246,6 +251,13 @@ class CSRGeometry:
         check(lib().mc_map_get_tuning(self.handle, C.byref(b), C.byref(r), C.byref(l)))
         return b.value, r.value, l.value
 
+    def verify_info(self) -> dict:
+        """Host verification of the GPU build (mc_map_verify_info)."""
+        out = (C.c_int64 * 4)()
+        check(lib().mc_map_verify_info(self.handle, out))
+        return {"pixels": out[0], "records": out[1], "membership_differs": out[2],
+                "values_differ": out[3]}
+
     def autotune(self, dtype=np.uint16) -> float:
         """Times the candidate plans (batched kernel: batch, rows per band,
         LDS budget, tile shape; one-frame kernel: lanes per bin) on frames
@@ -314,10 +326,16 @@ class CSRGeometry:
 
 
 def build_map(params: IntegrationParams, *, mask=None, dist_y=None, dist_z=None,
-              residual=None, panel_shifts=None, ctx: Context | None = None) -> CSRGeometry:
+              residual=None, panel_shifts=None, ctx: Context | None = None,
+              verify_pixels: int | None = None) -> CSRGeometry:
     """GPU FP64 DetectorMapper (MapperCore.c:45-424) for ``params``.
-    ``mask``: 1.0 = masked pixel, raw detector orientation (NZ x NY)."""
+    ``mask``: 1.0 = masked pixel, raw detector orientation (NZ x NY).
+    ``verify_pixels``: pixels recomputed on the host after the build (the
+    context's setting when None, else this sets it; see
+    ``CSRGeometry.verify_info``)."""
     ctx = ctx or default_context()
+    if verify_pixels is not None:
+        ctx.set_map_verify(verify_pixels)
     g = _geometry(params)
     panels = generate_panels(params, panel_shifts)
     h = C.c_void_p()

@@ -989,6 +989,19 @@ int mc_map_write(mc_map *m, const char *dir, const uint8_t *header64) {
   return MC_OK;
 }
 
+int mc_ctx_set_map_verify(mc_ctx *ctx, int64_t n_pixels) {
+  MC_CHECK(ctx && n_pixels >= 0, MC_ERR_ARG, "bad arguments");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  ctx->map_verify = n_pixels;
+  return MC_OK;
+}
+
+int mc_map_verify_info(mc_map *m, int64_t *out) {
+  MC_CHECK(m && out, MC_ERR_ARG, "null argument");
+  for (int k = 0; k < 4; k++) out[k] = m->verify[k];
+  return MC_OK;
+}
+
 int mc_map_info_get(mc_map *m, mc_map_info *info) {
   MC_CHECK(m && info, MC_ERR_ARG, "null argument");
   memset(info, 0, sizeof(*info));

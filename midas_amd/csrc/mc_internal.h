@@ -76,6 +76,7 @@ struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::recursive_mutex mu;  // one submitter on the stream at a time
+  int64_t map_verify = 4096;  // pixels of each GPU map build recomputed on the host
 };
 
 // Bin-major CSR of one integration mode: the one-frame kernel's layout.
@@ -144,6 +145,9 @@ struct Map {
   int64_t nnz = 0;
   int q_mode = 0;
   int64_t skipped[3] = {0, 0, 0};
+  // host verification of the GPU build: pixels checked, records compared,
+  // pixels whose bin membership differs, records whose values differ
+  int64_t verify[4] = {0, 0, 0, 0};
   std::vector<Rec> h_rec;           // canonical records (bin-major)
   std::vector<int32_t> h_pairs;     // nMap pairs
   std::vector<int32_t> h_flags;     // maskMap flags

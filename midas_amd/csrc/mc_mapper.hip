@@ -28,6 +28,9 @@
 #include <numeric>
 #include <vector>
 
+#include <array>
+#include <thread>
+
 #include "mc_internal.h"
 
 namespace mc {
@@ -49,17 +52,17 @@ struct GeoDev {
   int n_panels;
 };
 
-__device__ __forceinline__ double eta_of(double y, double z) {
+__host__ __device__ __forceinline__ double eta_of(double y, double z) {
   return kR2D * atan2(-y, z);
 }
-__device__ __forceinline__ double sgn(double x) {
+__host__ __device__ __forceinline__ double sgn(double x) {
   return x > 0 ? 1.0 : (x < 0 ? -1.0 : 0.0);
 }
-__device__ __forceinline__ bool between(double v, double lo, double hi) {
+__host__ __device__ __forceinline__ bool between(double v, double lo, double hi) {
   return v - kEps <= hi && v + kEps >= lo;
 }
 
-__device__ double residual_at(const GeoDev &G, double Y, double Z) {
+__host__ __device__ double residual_at(const GeoDev &G, double Y, double Z) {
   if (!G.resid) return 0.0;
   const int NY = G.g.NrPixelsY, NZ = G.g.NrPixelsZ;
   double y = Y, z = Z;
@@ -75,7 +78,7 @@ __device__ double residual_at(const GeoDev &G, double Y, double Z) {
 }
 
 // Forward geometry; operation order as DetectorGeometry.c:95-149.
-__device__ void to_reta(const GeoDev &G, double Y, double Z, double dLsd, double dP2,
+__host__ __device__ void to_reta(const GeoDev &G, double Y, double Z, double dLsd, double dP2,
                         double &Rout, double &Eout) {
   const mc_geometry &g = G.g;
   const double px = g.pxY;
@@ -113,12 +116,12 @@ __device__ void to_reta(const GeoDev &G, double Y, double Z, double dLsd, double
   Eout = et;
 }
 
-__device__ __forceinline__ void to_yz(double R, double eta, double &y, double &z) {
+__host__ __device__ __forceinline__ void to_yz(double R, double eta, double &y, double &z) {
   y = -R * sin(eta * kD2R);
   z = R * cos(eta * kD2R);
 }
 
-__device__ int circle_hits(double y1, double z1, double y2, double z2, double R,
+__host__ __device__ int circle_hits(double y1, double z1, double y2, double z2, double R,
                            double h[2][2]) {
   double dy = y2 - y1, dz = z2 - z1;
   double a = dy * dy + dz * dz;
@@ -146,7 +149,7 @@ __device__ int circle_hits(double y1, double z1, double y2, double z2, double R,
   return n;
 }
 
-__device__ bool ray_hit(double y1, double z1, double y2, double z2, double etaDeg,
+__host__ __device__ bool ray_hit(double y1, double z1, double y2, double z2, double etaDeg,
                         double &hy, double &hz) {
   double er = etaDeg * kD2R;
   double ce = cos(er), se = sin(er);
@@ -162,7 +165,7 @@ __device__ bool ray_hit(double y1, double z1, double y2, double z2, double etaDe
   return true;
 }
 
-__device__ bool in_quad(double py, double pz, const double q[4][2]) {
+__host__ __device__ bool in_quad(double py, double pz, const double q[4][2]) {
   int pos = 0, neg = 0;
   for (int e = 0; e < 4; e++) {
     int a = kQ[e], b = kQ[(e + 1) % 4];
@@ -175,7 +178,7 @@ __device__ bool in_quad(double py, double pz, const double q[4][2]) {
 }
 
 // Angular comparator of the reference's polygon sort (positive: a after b).
-__device__ __forceinline__ int cmp_angle(double ax_, double ay_, double bx_, double by_,
+__host__ __device__ __forceinline__ int cmp_angle(double ax_, double ay_, double bx_, double by_,
                                          double cx, double cy) {
   double ax = ax_ - cx, ay = ay_ - cy, bx = bx_ - cx, by = by_ - cy;
   if (ax >= 0 && bx < 0) return 1;
@@ -192,7 +195,7 @@ __device__ __forceinline__ int cmp_angle(double ax_, double ay_, double bx_, dou
 }
 
 // Exact area of the clipped polygon (arc edges integrated exactly).
-__device__ double poly_area(double (*v)[2], int n, double Rlo, double Rhi) {
+__host__ __device__ double poly_area(double (*v)[2], int n, double Rlo, double Rhi) {
   double cx = 0, cy = 0;
   for (int i = 0; i < n; i++) {
     cx += v[i][0];
@@ -236,7 +239,7 @@ __device__ double poly_area(double (*v)[2], int n, double Rlo, double Rhi) {
 
 // Overlap area of pixel quad cyz with bin [Rlo,Rhi]x[Elo,Ehi]; negative
 // return = rejected (-1: <3 vertices, -2: <3 unique, -3: area < 1e-5).
-__device__ double clip_area(const double cyz[4][2], double Rlo, double Rhi, double Elo,
+__host__ __device__ double clip_area(const double cyz[4][2], double Rlo, double Rhi, double Elo,
                             double Ehi) {
   double box[4][2];
   to_yz(Rlo, Elo, box[0][0], box[0][1]);
@@ -325,7 +328,7 @@ __device__ double clip_area(const double cyz[4][2], double Rlo, double Rhi, doub
 }
 
 // First index in [0,n) with a[k] >= x (n if none).
-__device__ int lower_ge(const double *a, int n, double x) {
+__host__ __device__ int lower_ge(const double *a, int n, double x) {
   int lo = 0, hi = n;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
@@ -335,7 +338,7 @@ __device__ int lower_ge(const double *a, int n, double x) {
   return lo;
 }
 // Last index with a[k] <= x (-1 if none).
-__device__ int last_le(const double *a, int n, double x) {
+__host__ __device__ int last_le(const double *a, int n, double x) {
   int lo = 0, hi = n;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
@@ -345,17 +348,17 @@ __device__ int last_le(const double *a, int n, double x) {
   return lo - 1;
 }
 
-template <bool WRITE>
-__global__ __launch_bounds__(128) void k_map(GeoDev G, int32_t *__restrict__ counts,
-                                             const int64_t *__restrict__ offsets,
-                                             int32_t *__restrict__ out_bin,
-                                             Rec *__restrict__ out_rec,
-                                             int32_t *__restrict__ flags,
-                                             unsigned long long *__restrict__ skipped) {
+// One pixel of the map (the reference's per-pixel loop body,
+// MapperCore.c:45-424): sub-pixel corners, candidate bins, exact clipped
+// areas and the record of each overlap.  Out receives flag(bin) for the
+// bins a masked pixel contaminates (count pass), skip(kind) for areas the
+// clipper rejects, and rec(n, bin, record) for the records (write pass,
+// while room(n)).  Returns the pixel's record count.  Compiled for the GPU
+// (k_map) and for the host (the verification pass of map_build_gpu).
+template <bool WRITE, typename Out>
+__host__ __device__ int map_pixel(const GeoDev &G, int64_t p, Out &out) {
   const mc_geometry &g = G.g;
   const int NY = g.NrPixelsY, NZ = g.NrPixelsZ;
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (int64_t)NY * NZ) return;
   // Reference loop order: Y outer (i), Z inner (j).
   const int i = (int)(p / NZ), j = (int)(p % NZ);
   const int64_t pos = (int64_t)j * NY + i;
@@ -410,8 +413,6 @@ __global__ __launch_bounds__(128) void k_map(GeoDev G, int32_t *__restrict__ cou
     }
   }
   int n_out = 0;
-  int64_t w_at = WRITE ? offsets[p] : 0;
-  unsigned long long sk0 = 0, sk1 = 0, sk2 = 0;
   for (int si = 0; si < sp; si++) {
     for (int sj = 0; sj < sp; sj++) {
       double oy[2] = {(double)si / sp - 0.5, (double)(si + 1) / sp - 0.5};
@@ -461,7 +462,7 @@ __global__ __launch_bounds__(128) void k_map(GeoDev G, int32_t *__restrict__ cou
               bool sel = (G.EHi[e] >= lo && G.ELo[e] <= hi) ||
                          (G.EHi[e] >= lo + 360 && G.ELo[e] <= hi + 360) ||
                          (G.EHi[e] >= lo - 360 && G.ELo[e] <= hi - 360);
-              if (sel) flags[(int64_t)r * G.nE + e] = 1;
+              if (sel) out.flag((int64_t)r * G.nE + e);
             }
           }
         continue;
@@ -476,13 +477,10 @@ __global__ __launch_bounds__(128) void k_map(GeoDev G, int32_t *__restrict__ cou
           if (!sel) continue;
           double area = clip_area(cyz, Rlo, Rhi, G.ELo[e], G.EHi[e]);
           if (area < 0) {
-            if (area == -1) sk0++;
-            else if (area == -2) sk1++;
-            else sk2++;
+            out.skip(area == -1 ? 0 : area == -2 ? 1 : 2);
             continue;
           }
-          // The write pass never exceeds the slots the count pass reserved.
-          if (WRITE && n_out < counts[p]) {
+          if (WRITE && out.room(n_out)) {
             double corr = area;
             if (g.SolidAngleCorrection || g.PolarizationCorrection) {
               double tth = atan(Rt * g.pxY / g.Lsd);
@@ -503,23 +501,69 @@ __global__ __launch_bounds__(128) void k_map(GeoDev G, int32_t *__restrict__ cou
             rec.frac = corr;
             rec.deltaR = (float)(Rt - (Rlo + Rhi) * 0.5);
             rec.areaWeight = (float)area;
-            out_bin[w_at + n_out] = (int32_t)(r * G.nE + e);
-            out_rec[w_at + n_out] = rec;
+            out.rec(n_out, (int32_t)(r * G.nE + e), rec);
           }
           n_out++;
         }
       }
     }
   }
+  return n_out;
+}
+
+// Device side of map_pixel's output.
+struct DevOut {
+  int32_t *flags;
+  int32_t *bin;
+  Rec *rec_out;
+  int cap;
+  unsigned long long sk[3] = {0, 0, 0};
+  __device__ void flag(int64_t b) { flags[b] = 1; }
+  __device__ void skip(int k) { sk[k]++; }
+  __device__ bool room(int n) const { return n < cap; }
+  __device__ void rec(int n, int32_t b, const Rec &r) {
+    bin[n] = b;
+    rec_out[n] = r;
+  }
+};
+
+template <bool WRITE>
+__global__ __launch_bounds__(128) void k_map(GeoDev G, int32_t *__restrict__ counts,
+                                             const int64_t *__restrict__ offsets,
+                                             int32_t *__restrict__ out_bin,
+                                             Rec *__restrict__ out_rec,
+                                             int32_t *__restrict__ flags,
+                                             unsigned long long *__restrict__ skipped) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (int64_t)G.g.NrPixelsY * G.g.NrPixelsZ) return;
+  // The write pass never exceeds the slots the count pass reserved.
+  const int64_t w_at = WRITE ? offsets[p] : 0;
+  DevOut out{flags, WRITE ? out_bin + w_at : nullptr, WRITE ? out_rec + w_at : nullptr,
+             WRITE ? counts[p] : 0};
+  const int n_out = map_pixel<WRITE>(G, p, out);
   if (!WRITE) {
     counts[p] = n_out;
-    if (sk0) atomicAdd(&skipped[0], sk0);
-    if (sk1) atomicAdd(&skipped[1], sk1);
-    if (sk2) atomicAdd(&skipped[2], sk2);
+    if (out.sk[0]) atomicAdd(&skipped[0], out.sk[0]);
+    if (out.sk[1]) atomicAdd(&skipped[1], out.sk[1]);
+    if (out.sk[2]) atomicAdd(&skipped[2], out.sk[2]);
   } else if (n_out != counts[p]) {
     atomicAdd(&skipped[3], 1ull);  // count/write passes disagree: reported
   }
 }
+
+// Host side of map_pixel's output (the verification pass).
+struct HostOut {
+  std::vector<int64_t> flagged;
+  std::vector<int32_t> bins;
+  std::vector<Rec> recs;
+  void flag(int64_t b) { flagged.push_back(b); }
+  void skip(int) {}
+  bool room(int) const { return true; }
+  void rec(int, int32_t b, const Rec &r) {
+    bins.push_back(b);
+    recs.push_back(r);
+  }
+};
 
 void apply_transform(const mc_geometry *g, std::vector<double> &img) {
   const int NY = g->NrPixelsY, NZ = g->NrPixelsZ;
@@ -623,21 +667,24 @@ int map_build_gpu(Ctx *ctx, const mc_geometry *gin, const double *mask,
   G.EHi = dEHi.as<double>();
   G.nR = nR;
   G.nE = nE;
-  auto upload_img = [&](const double *src, bool transform, DBuf &dst, const double *&ptr) -> int {
+  // Host copies (transformed like the device ones) serve the verification.
+  std::vector<double> hMask, hDY, hDZ, hRes;
+  auto upload_img = [&](const double *src, bool transform, DBuf &dst, const double *&ptr,
+                        std::vector<double> &host) -> int {
     ptr = nullptr;
     if (!src) return MC_OK;
-    std::vector<double> tmp(src, src + npx);
-    if (transform) apply_transform(&g, tmp);
+    host.assign(src, src + npx);
+    if (transform) apply_transform(&g, host);
     int r2 = dst.alloc(npx * 8);
     if (r2) return r2;
-    MC_HIP(hipMemcpy(dst.p, tmp.data(), npx * 8, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(dst.p, host.data(), npx * 8, hipMemcpyHostToDevice));
     ptr = dst.as<double>();
     return MC_OK;
   };
-  if ((rc = upload_img(mask, true, dMask, G.mask))) return rc;
-  if ((rc = upload_img(distY, true, dDY, G.dY))) return rc;
-  if ((rc = upload_img(distZ, true, dDZ, G.dZ))) return rc;
-  if ((rc = upload_img(residual, false, dRes, G.resid))) return rc;
+  if ((rc = upload_img(mask, true, dMask, G.mask, hMask))) return rc;
+  if ((rc = upload_img(distY, true, dDY, G.dY, hDY))) return rc;
+  if ((rc = upload_img(distZ, true, dDZ, G.dZ, hDZ))) return rc;
+  if ((rc = upload_img(residual, false, dRes, G.resid, hRes))) return rc;
   G.panels = nullptr;
   G.n_panels = 0;
   if (panels && n_panels > 0) {
@@ -686,6 +733,85 @@ int map_build_gpu(Ctx *ctx, const mc_geometry *gin, const double *mask,
   MC_HIP(hipMemcpyAsync(&mismatch, dSkip.as<unsigned long long>() + 3, 8, hipMemcpyDeviceToHost, s));
   MC_HIP(hipStreamSynchronize(s));
   MC_CHECK(mismatch == 0, MC_ERR_STATE, "mapper passes disagree on %llu pixels", mismatch);
+  // Host verification (the same per-pixel code built for the host, with
+  // the host's libm): a spread sample of pixels is recomputed and compared
+  // with the GPU's pixel-major output.  Bin membership must agree; values
+  // within 1e-6 px^2 (areas, corrected areas) and 1e-5 px (deltaR).  Where a
+  // vertex lies exactly on a bin boundary, the two libms' last-bit
+  // differences can flip a case: those pixels are counted, not refused.
+  {
+    const int64_t nver = std::min<int64_t>(npx, ctx->map_verify);
+    int64_t checked = 0, compared = 0, member_bad = 0, value_bad = 0;
+    GeoDev H = G;
+    H.RLo = RLo.data();
+    H.RHi = RHi.data();
+    H.ELo = ELo.data();
+    H.EHi = EHi.data();
+    H.mask = hMask.empty() ? nullptr : hMask.data();
+    H.dY = hDY.empty() ? nullptr : hDY.data();
+    H.dZ = hDZ.empty() ? nullptr : hDZ.data();
+    H.resid = hRes.empty() ? nullptr : hRes.data();
+    H.panels = (panels && n_panels > 0) ? panels : nullptr;
+    H.n_panels = (panels && n_panels > 0) ? n_panels : 0;
+    // Spread over host threads; each keeps its own counts.
+    const int nt = nver < 4096 ? 1
+                               : (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::array<int64_t, 4>> part(nt, std::array<int64_t, 4>{0, 0, 0, 0});
+    auto run = [&](int t) {
+      int64_t &chk = part[t][0], &cmp = part[t][1], &mbad = part[t][2], &vbad = part[t][3];
+      for (int64_t k = nver * t / nt; k < nver * (t + 1) / nt; k++) {
+        const int64_t p =
+            nver == npx ? k : (int64_t)(((uint64_t)k * 2654435761ull + 12345ull) % (uint64_t)npx);
+        const int pi = (int)(p / NZ), pj = (int)(p % NZ);
+        const bool masked = H.mask && H.mask[(int64_t)pj * NY + pi] == 1.0;
+        chk++;
+        if (masked) {
+          HostOut o;
+          map_pixel<false>(H, p, o);
+          for (int64_t bq : o.flagged)
+            if (m->h_flags[bq] != 1) {
+              mbad++;
+              break;
+            }
+          continue;
+        }
+        HostOut o;
+        const int n = map_pixel<true>(H, p, o);
+        if (n != counts[p]) {
+          mbad++;
+          continue;
+        }
+        bool same = true;
+        for (int q = 0; q < n; q++) same = same && o.bins[q] == bins[offs[p] + q];
+        if (!same) {
+          mbad++;
+          continue;
+        }
+        for (int q = 0; q < n; q++) {
+          const Rec &x = o.recs[q], &y = recs[offs[p] + q];
+          cmp++;
+          if (x.y != y.y || x.z != y.z || fabs(x.frac - y.frac) > 1e-6 ||
+              fabs((double)x.areaWeight - (double)y.areaWeight) > 1e-6 ||
+              fabs((double)x.deltaR - (double)y.deltaR) > 1e-5)
+            vbad++;
+        }
+      }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(run, t);
+    run(0);
+    for (auto &x : th) x.join();
+    for (const auto &q : part) {
+      checked += q[0];
+      compared += q[1];
+      member_bad += q[2];
+      value_bad += q[3];
+    }
+    m->verify[0] = checked;
+    m->verify[1] = compared;
+    m->verify[2] = member_bad;
+    m->verify[3] = value_bad;
+  }
   // Stable counting sort pixel-major -> bin-major.
   m->h_pairs.assign(2 * nb, 0);
   for (int64_t q = 0; q < nnz; q++) m->h_pairs[2 * (int64_t)bins[q]]++;

@@ -51,6 +51,11 @@ def c2(oracle_lib):
 
 def test_c2_map_membership(c2):
     P, frames, dark, g, pm, om = c2
+    # the build's host verification pass (4096 pixels spread over the detector)
+    v = g.verify_info()
+    print(f"[c2] host verification of the GPU build: {v}", flush=True)
+    assert v["pixels"] >= 4096 and v["records"] > 0
+    assert v["membership_differs"] == 0 and v["values_differ"] == 0, v
     assert pm.n_entries == REC["nnz"]
     assert _digest(om.pairs) == REC["sha256_pairs"]
     assert _digest(om.flags) == REC["sha256_flags"]

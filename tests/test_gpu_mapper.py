@@ -20,7 +20,13 @@ def test_mapper_matches_oracle(name, oracle_lib):
     print(f"[mapper] case={name} {P.NrPixelsY}x{P.NrPixelsZ}", flush=True)
     mask = _mask(P, 7)
     sh = panel_shifts_for(name)
-    g = ma.build_map(P, mask=mask, panel_shifts=sh)
+    npx = P.NrPixelsY * P.NrPixelsZ
+    g = ma.build_map(P, mask=mask, panel_shifts=sh, verify_pixels=npx)
+    # the build's own host pass recomputed every pixel and agreed
+    v = g.verify_info()
+    assert v["pixels"] == npx and v["records"] > 0, v
+    assert v["membership_differs"] == 0 and v["values_differ"] == 0, v
+    g.ctx.set_map_verify(4096)  # back to the default for later tests
     pm = g.download()
     om = oracle_lib.build_map(to_oracle(P), mask=mask, panel_shifts=sh)
     assert pm.n_entries == om.entries.size > 0
@@ -50,7 +56,12 @@ def test_mapper_plain_1024_membership(oracle_lib):
     P = synth.plain_1024()
     _, _, mask = synth.frames(P, 1, seed=0)
     print("[mapper] plain_1024", flush=True)
-    g = ma.build_map(P, mask=mask)
+    g = ma.build_map(P, mask=mask, verify_pixels=P.NrPixelsY * P.NrPixelsZ)
+    v = g.verify_info()
+    print(f"[mapper] host verification: {v}", flush=True)
+    assert v["pixels"] == P.NrPixelsY * P.NrPixelsZ
+    assert v["membership_differs"] == 0 and v["values_differ"] == 0, v
+    g.ctx.set_map_verify(4096)  # back to the default for later tests
     pm = g.download()
     om = oracle_lib.build_map(to_oracle(P), mask=mask)
     assert_maps_equal(pm, om)
