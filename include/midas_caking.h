@@ -224,9 +224,11 @@ int mc_map_set_dark(mc_map *m, const void *dark, int32_t dtype,
                     int32_t n_frames, double scale, int32_t on_device);
 int mc_map_set_beam_center(mc_map *m, double bc_y, double bc_z);
 
-/* Tuning of the batched kernel: frames per pass (8, 16, 32, 64 or 128),
- * R rows per band (16 or 32), LDS budget per workgroup in bytes
- * (8192..163840).  0 keeps the current value. */
+/* Tuning of the batched kernel: batch (8, 16, 32, 64 or 128) sets the
+ * pass width, the narrowest of 16, 32 or 64 frames holding it (a pass of
+ * 16 or 32 frames runs a kernel with one or two frames per lane; 128 runs
+ * passes of 64), R rows per band (16 or 32), LDS budget per workgroup in
+ * bytes (8192..163840).  0 keeps the current value. */
 int mc_map_set_tuning(mc_map *m, int32_t batch, int32_t rows_per_group,
                       int32_t lds_bytes);
 int mc_map_get_tuning(mc_map *m, int32_t *batch, int32_t *rows_per_group,
@@ -248,7 +250,7 @@ int mc_map_layout_info(mc_map *m, int32_t mode, int32_t dtype, int64_t *out);
 int mc_map_set_plain_lanes(mc_map *m, int32_t lanes);
 int mc_map_get_plain_lanes(mc_map *m, int32_t *lanes);
 /* Times candidate plans on synthetic frames of the map's shape and keeps
- * the fastest: batch 32/64, rows per band 16/32, LDS budget 80/160 KiB and
+ * the fastest: batch 16/32/64, rows per band 16/32, LDS budget 80/160 KiB and
  * three tile shapes for the batched kernel (128 frames a timing), then 4/8/
  * 16 lanes per bin for the one-frame kernel; writes the winner's batched
  * frames/s.  No counterpart in the reference (it has no tuner). */

@@ -229,7 +229,7 @@ int ensure_csr(Map *m, int mode) {
 // LDS of the batched kernel: bin records, staged entries, the staged pixel
 // rows of the pixel-major batch (64 frames per pixel), a zero pixel row, and
 // two pixel-list buffers (the current tile's and the next one's).
-constexpr int kBinRecBytes = 32, kWaves = 16, kTileInts = 16, kXtFrames = 64;
+constexpr int kBinRecBytes = 32, kWaves = 16, kTileInts = 16;
 
 // Tiles of the batched kernel.  Bands of H cake rows; inside a band, tiles
 // are rectangles of rows x eta columns grown column by column while the
@@ -240,13 +240,15 @@ constexpr int kBinRecBytes = 32, kWaves = 16, kTileInts = 16, kXtFrames = 64;
 // Chunks are runs of consecutive tiles of one band of about `target`
 // entries: the work of one workgroup.  in_size is the element size of the
 // pixel-major batch (2 or 4 bytes).
-int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
+int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
   int rc = ensure_csr(m, mode);
   if (rc) return rc;
   const int B = m->batch, H = m->rows_per_group;
+  const int W = width ? width : width_of(B);
+  MC_CHECK(W == 16 || W == 32 || W == 64, MC_ERR_ARG, "pass width %d not in {16, 32, 64}", W);
   const int kTileBins = m->tile_bins, kTileEnts = m->tile_ents;
   for (auto &t : m->tiles[mode])
-    if (t->batch == B && t->rows == H && t->lds_bytes == m->lds_bytes && t->in_size == in_size &&
+    if (t->width == W && t->rows == H && t->lds_bytes == m->lds_bytes && t->in_size == in_size &&
         t->bin_cap == kTileBins && t->ent_cap == kTileEnts) {
       *out = t.get();
       return MC_OK;
@@ -257,7 +259,7 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   const Csr &c = m->csr[mode];
   const int NY = m->ny, nE = m->n_eta, nR = m->n_r;
   const int64_t nb = m->nb();
-  const int RB = kXtFrames * in_size;  // bytes per pixel row
+  const int RB = W * in_size;  // bytes per pixel row
   const int64_t fixed = (int64_t)kTileBins * kBinRecBytes + (int64_t)kTileEnts * 8;
   // Pixels a tile can stage: what the LDS budget holds after the records,
   // the entries, the zero row and the two pixel lists (4 bytes a pixel,
@@ -497,6 +499,7 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out) {
   for (int x = 0; x < 8; x++) per_xcd = std::max(per_xcd, xcd[x + 1] - xcd[x]);
   auto t = std::make_unique<Tiles>();
   t->batch = B;
+  t->width = W;
   t->bin_cap = kTileBins;
   t->ent_cap = kTileEnts;
   t->grid = 8 * per_xcd;
@@ -614,8 +617,8 @@ static int ensure_recs(Map *m, hipStream_t s, Tiles *t, int mode, uint32_t flags
 // Element size of the pixel-major batch for a frame dtype.
 static int batch_elem(int dtype) { return dtype_size(dtype) <= 2 ? 2 : 4; }
 
-// Frames per batched pass (the pixel-major batch holds 64 frames).
-static int pass_frames(const Tiles &t) { return std::min(t.batch, 64); }
+// Frames per batched pass: the layout's pass width.
+static int pass_frames(const Tiles &t) { return t.width; }
 
 static int ensure_xt(DBuf &xt, const Map &m, int dtype) {
   const size_t need = batch_xt_bytes(m, dtype);
@@ -643,28 +646,28 @@ static int ensure_part(DBuf &part, const Tiles &t) {
   return part.bytes >= need ? MC_OK : part.alloc(need);
 }
 
-int ensure_work(Map *m, int mode, int dtype, Work *work) {
+int ensure_work(Map *m, int mode, int dtype, Work *work, int width) {
   Tiles *t = nullptr;
   int rc;
-  if ((rc = ensure_tiles(m, mode, batch_elem(dtype), &t)) ||
+  if ((rc = ensure_tiles(m, mode, batch_elem(dtype), &t, width)) ||
       (rc = ensure_xt(work->xt, *m, dtype)) || (rc = ensure_part(work->part, *t)))
     return rc;
   return MC_OK;
 }
 
-int prepare_batched(Map *m, int mode, uint32_t flags, int dtype) {
+int prepare_batched(Map *m, int mode, uint32_t flags, int dtype, int width) {
   const float2 *binfo = nullptr;
   Tiles *t = nullptr;
   int rc;
   if ((rc = ensure_binfo(m, mode, flags, &binfo)) ||
-      (rc = ensure_tiles(m, mode, batch_elem(dtype), &t)))
+      (rc = ensure_tiles(m, mode, batch_elem(dtype), &t, width)))
     return rc;
   return ensure_recs(m, m->ctx->stream, t, mode, flags, binfo);
 }
 
 int run_integrate(Map *m, hipStream_t s, const void *frames_dev, int dtype, int n_frames,
                   int mode, uint32_t flags, float *cakes_dev, float *lineouts_dev,
-                  bool force_plain, bool force_batched, Work *work) {
+                  bool force_plain, bool force_batched, Work *work, int width) {
   int rc;
   const float2 *binfo = nullptr;
   if ((rc = ensure_binfo(m, mode, flags, &binfo))) return rc;
@@ -679,7 +682,7 @@ int run_integrate(Map *m, hipStream_t s, const void *frames_dev, int dtype, int 
                         lineouts_dev, n_frames, m->npx(), ws.xt.as<float>(), sub_frames);
   }
   Tiles *t = nullptr;
-  if ((rc = ensure_tiles(m, mode, batch_elem(dtype), &t))) return rc;
+  if ((rc = ensure_tiles(m, mode, batch_elem(dtype), &t, width))) return rc;
   if ((rc = ensure_recs(m, s, t, mode, flags, binfo))) return rc;
   if ((rc = ensure_xt(ws.xt, *m, dtype)) || (rc = ensure_part(ws.part, *t))) return rc;
   const int step = pass_frames(*t);
@@ -1795,7 +1798,7 @@ int autotune_run(mc_map *m, int32_t dtype, Plan *best_out, std::string *table) {
     return rc;
   MC_HIP(hipMemset(frames.p, 0x11, frames.bytes));
   MC_HIP(hipDeviceSynchronize());
-  const int Bs[] = {32, 64}, Hs[] = {16, 32}, Ls[] = {81920, 163840};
+  const int Bs[] = {16, 32, 64}, Hs[] = {16, 32}, Ls[] = {81920, 163840};
   const int Shapes[][2] = {{256, 2048}, {128, 1024}, {512, 4096}};
   Plan best{m->batch, m->rows_per_group, m->lds_bytes, m->tile_bins, m->tile_ents,
             m->plain_lanes, -1.0, -1.0};
@@ -1841,7 +1844,7 @@ int autotune_run(mc_map *m, int32_t dtype, Plan *best_out, std::string *table) {
     for (auto &v : m->tiles) {
       std::vector<std::unique_ptr<Tiles>> keep;
       for (auto &t : v)
-        if (t->batch == best.batch && t->rows == best.rows && t->lds_bytes == best.lds &&
+        if (t->width == width_of(best.batch) && t->rows == best.rows && t->lds_bytes == best.lds &&
             t->bin_cap == best.bins && t->ent_cap == best.ents)
           keep.push_back(std::move(t));
       v.swap(keep);

@@ -100,7 +100,8 @@ struct Csr {
 struct Tiles {
   bool built = false;
   uint64_t id = 0;   // unique per layout built (captured graphs check it)
-  int batch = 0, rows = 0, lds_bytes = 0;  // frames per pass, band rows (16 * G), LDS budget
+  int batch = 0, rows = 0, lds_bytes = 0;  // tuned batch, band rows (16 * G), LDS budget
+  int width = 64;    // frames per pass: 16, 32 or 64 (pixel row of the pixel-major batch)
   int bin_cap = 0, ent_cap = 0;  // bin records and staged entries a tile holds in LDS
   int in_size = 0;   // bytes per element of the pixel-major batch (2 or 4)
   int n_chunks = 0, n_bands = 0;
@@ -123,7 +124,7 @@ struct Tiles {
   int64_t n_tiles = 0, n_tbins = 0, n_ents = 0, n_direct = 0, staged_px = 0;
   // Bytes of lineout partials a pass over this layout writes:
   // float [n_chunks][rows][4][64].
-  size_t part_bytes() const { return (size_t)n_chunks * rows * 4 * 64 * sizeof(float); }
+  size_t part_bytes() const { return (size_t)n_chunks * rows * 4 * width * sizeof(float); }
 };
 
 // Scratch of the batched passes of one caller: the pixel-major batch and
@@ -241,17 +242,22 @@ int map_build_gpu(Ctx *ctx, const mc_geometry *g, const double *mask,
 // ---- host helpers (mc_core.cpp) ----
 int map_finish_host(Map *m);  // derived arrays + upload
 int ensure_csr(Map *m, int mode);
-int ensure_tiles(Map *m, int mode, int esize, Tiles **out);
+// Pass width of a batch size: the narrowest of 16, 32, 64 frames holding it.
+inline int width_of(int batch) { return batch <= 16 ? 16 : batch <= 32 ? 32 : 64; }
+// Tile layout of (mode, element size, pass width; 0: the width of the
+// map's tuned batch) under the map's tuning.
+int ensure_tiles(Map *m, int mode, int esize, Tiles **out, int width = 0);
 // Launches the integration of n_frames device frames on stream s (no sync).
 // work: the caller's scratch for batched passes on s (NULL: the map's own,
 // for the map's context stream only).
 int run_integrate(Map *m, hipStream_t s, const void *frames_dev, int dtype, int n_frames,
                   int mode, uint32_t flags, float *cakes_dev, float *lineouts_dev,
-                  bool force_plain, bool force_batched, Work *work = nullptr);
+                  bool force_plain, bool force_batched, Work *work = nullptr,
+                  int width = 0);
 // Sizes work for batched passes of (mode, dtype) over m's current layout.
-int ensure_work(Map *m, int mode, int dtype, Work *work);
+int ensure_work(Map *m, int mode, int dtype, Work *work, int width = 0);
 // Everything the batched kernel reads for (mode, flags, dtype), built ahead.
-int prepare_batched(Map *m, int mode, uint32_t flags, int dtype);
+int prepare_batched(Map *m, int mode, uint32_t flags, int dtype, int width = 0);
 // The per-bin table of (mode, flags); builds it (and the dark term) once.
 int ensure_binfo(Map *m, int mode, uint32_t flags, const float2 **out);
 

@@ -349,7 +349,7 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // integer dark dhi of the pixel is subtracted here, exactly for integer
 // counts below 2^24 (uint16 batches keep the raw count and subtract the
 // dark in k_batched, from the staged entries).
-template <typename TIn, typename TL>
+template <typename TIn, typename TL, int XF>
 __global__ __launch_bounds__(256) void k_transpose(const TIn *__restrict__ frames,
                                                    int64_t frame_stride, int n_frames,
                                                    int64_t npx, int vec_ok,
@@ -358,23 +358,26 @@ __global__ __launch_bounds__(256) void k_transpose(const TIn *__restrict__ frame
                                                    TL *__restrict__ xt) {
   constexpr int PX = 64;                       // pixels per block
   constexpr int EV = 16 / (int)sizeof(TL);     // elements per 16-byte store
-  __shared__ TL tile[kXtFrames][PX + 4 / (int)sizeof(TL)];
+  __shared__ TL tile[XF][PX + 4 / (int)sizeof(TL)];
   const int64_t p0 = (int64_t)(blocks ? blocks[blockIdx.x] : (int32_t)blockIdx.x) * PX;
   const int t = threadIdx.x;
   if (vec_ok && sizeof(TIn) == 2 && sizeof(TL) == 2) {
     // 16-byte loads: thread -> (frame t/4, 16 pixels as two vectors)
     const int f = t >> 2, q0 = (t & 3) * 16;
-    u4v a = 0u, b = 0u;
-    if (f < n_frames) {
-      const u4v *src = reinterpret_cast<const u4v *>(frames + (int64_t)f * frame_stride + p0 + q0);
-      a = src[0];
-      b = src[1];
-    }
-    uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    if (f < XF) {
+      u4v a = 0u, b = 0u;
+      if (f < n_frames) {
+        const u4v *src =
+            reinterpret_cast<const u4v *>(frames + (int64_t)f * frame_stride + p0 + q0);
+        a = src[0];
+        b = src[1];
+      }
+      uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      tile[f][q0 + 2 * q] = (TL)(w[q] & 0xffffu);
-      tile[f][q0 + 2 * q + 1] = (TL)(w[q] >> 16);
+      for (int q = 0; q < 8; ++q) {
+        tile[f][q0 + 2 * q] = (TL)(w[q] & 0xffffu);
+        tile[f][q0 + 2 * q + 1] = (TL)(w[q] >> 16);
+      }
     }
   } else if (vec_ok && sizeof(TL) == 4) {
     // Vector loads: thread -> (frames t/16 + 16k, 4 consecutive pixels), so
@@ -391,7 +394,7 @@ __global__ __launch_bounds__(256) void k_transpose(const TIn *__restrict__ frame
       dk[3] = (float)(d2.y >> 16);
     }
 #pragma unroll
-    for (int k = 0; k < kXtFrames / 16; ++k) {
+    for (int k = 0; k < XF / 16; ++k) {
       const int f = fs + 16 * k;
       float x[4] = {0.f, 0.f, 0.f, 0.f};  // frames past the batch stay zero
       if (f < n_frames) {
@@ -409,6 +412,7 @@ __global__ __launch_bounds__(256) void k_transpose(const TIn *__restrict__ frame
     const TIn *src = frames + (int64_t)f * frame_stride + p0 + q0;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
+      if (f >= XF) break;
       TL v = static_cast<TL>(0);
       if (f < n_frames && p0 + q0 + q < npx) {
         if constexpr (sizeof(TL) == 4) {
@@ -423,11 +427,9 @@ __global__ __launch_bounds__(256) void k_transpose(const TIn *__restrict__ frame
     }
   }
   __syncthreads();
-  // 16-byte stores: thread -> (pixel, EV frames); 64 / EV stores per pixel.
-  constexpr int SPP = kXtFrames / EV;
-#pragma unroll
-  for (int it = 0; it < PX * SPP / 256; ++it) {
-    const int job = t + 256 * it;
+  // 16-byte stores: thread -> (pixel, EV frames); XF / EV stores per pixel.
+  constexpr int SPP = XF / EV;
+  for (int job = t; job < PX * SPP; job += 256) {
     const int p = job / SPP, c = job - p * SPP;
     if (p0 + p >= npx) continue;
     u4v out;
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(256) void k_transpose(const TIn *__restrict__ frame
 #pragma unroll
       for (int k = 0; k < 4; ++k) out[k] = __float_as_uint((float)tile[c * EV + k][p]);
     }
-    *reinterpret_cast<u4v *>(xt + (p0 + p) * kXtFrames + c * EV) = out;
+    *reinterpret_cast<u4v *>(xt + (p0 + p) * XF + c * EV) = out;
   }
 }
 
@@ -449,15 +451,16 @@ __global__ __launch_bounds__(256) void k_transpose(const TIn *__restrict__ frame
 // [frame][16 chunks of 8 pixels]; chunk c of frame f sits at slot
 // c ^ (f / 8), so the 2-byte gathers of the store phase (8 frame groups x
 // 8 pixels per wave) fall on 32 different banks.
+template <int XF>
 __global__ __launch_bounds__(256) void k_transpose16(const uint16_t *__restrict__ frames,
                                                      int64_t frame_stride, int n_frames,
                                                      int64_t npx, const int32_t *__restrict__ blocks,
                                                      uint16_t *__restrict__ xt) {
-  __shared__ __attribute__((aligned(16))) uint16_t tile[kXtFrames * 128];
+  __shared__ __attribute__((aligned(16))) uint16_t tile[XF * 128];
   const int64_t p0 = (int64_t)(blocks ? blocks[blockIdx.x] : (int32_t)blockIdx.x) * 128;
   const int t = threadIdx.x;
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
+  for (int u = 0; u < XF / 16; ++u) {
     const int j = t + 256 * u;
     const int f = j >> 4, c = j & 15;  // 16 consecutive threads read 256 bytes of a frame
     u4v v = 0u;
@@ -466,10 +469,11 @@ __global__ __launch_bounds__(256) void k_transpose16(const uint16_t *__restrict_
     *reinterpret_cast<u4v *>(tile + f * 128 + (c ^ (f >> 3)) * 8) = v;
   }
   __syncthreads();
+  constexpr int GPP = XF / 8;  // 16-byte stores (8 frames) per pixel
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
+  for (int u = 0; u < XF / 16; ++u) {
     const int k = t + 256 * u;
-    const int p = k >> 3, g = k & 7;  // pixel, group of 8 frames
+    const int p = k / GPP, g = k % GPP;  // pixel, group of 8 frames
     if (p0 + p >= npx) continue;
     const int ch = p >> 3, w = p & 7;
     u4v out;
@@ -480,7 +484,7 @@ __global__ __launch_bounds__(256) void k_transpose16(const uint16_t *__restrict_
       const uint32_t b = tile[fb * 128 + (ch ^ g) * 8 + w];
       out[q] = a | (b << 16);
     }
-    *reinterpret_cast<u4v *>(xt + (p0 + p) * kXtFrames + 8 * g) = out;
+    *reinterpret_cast<u4v *>(xt + (p0 + p) * XF + 8 * g) = out;
   }
 }
 
@@ -503,6 +507,38 @@ template <> struct Quad<float> {
     o[2] = v.z;
     o[3] = v.w;
   }
+};
+
+// FPL frames of one pixel row for one lane (FPL = 4, 2 or 1: passes of 64,
+// 32 or 16 frames).
+template <typename TL, int FPL> struct Lanes;
+template <typename TL> struct Lanes<TL, 4> {
+  typedef typename Quad<TL>::type type;
+  __device__ __forceinline__ static void unpack(const type v, float o[4]) {
+    Quad<TL>::unpack(v, o);
+  }
+};
+template <> struct Lanes<uint16_t, 2> {
+  typedef uint32_t type;
+  __device__ __forceinline__ static void unpack(const uint32_t v, float o[2]) {
+    o[0] = (float)(v & 0xffffu);
+    o[1] = (float)(v >> 16);
+  }
+};
+template <> struct Lanes<uint16_t, 1> {
+  typedef uint16_t type;
+  __device__ __forceinline__ static void unpack(const uint16_t v, float o[1]) { o[0] = (float)v; }
+};
+template <> struct Lanes<float, 2> {
+  typedef float type __attribute__((ext_vector_type(2)));
+  __device__ __forceinline__ static void unpack(const type v, float o[2]) {
+    o[0] = v.x;
+    o[1] = v.y;
+  }
+};
+template <> struct Lanes<float, 1> {
+  typedef float type;
+  __device__ __forceinline__ static void unpack(const float v, float o[1]) { o[0] = v; }
 };
 
 template <int LG> struct GranVec;
@@ -572,21 +608,21 @@ __device__ __forceinline__ void stage_pixels(const int32_t *__restrict__ tpix, i
 // start at a multiple of 8), and the zero pixel row after the staged rows
 // (plain stores; the padding entries point at it).  The caller waits for
 // the copies (s_waitcnt) and then for every wave (barrier).
-template <typename TL>
+template <typename TL, int XF>
 __device__ __forceinline__ void tile_stage(const TileHdr &h, int wave, int lane, char *lds,
                                            char *lent, char *plane,
                                            const BinRec *__restrict__ recs,
                                            const uint2 *__restrict__ ents,
                                            const TL *__restrict__ xt,
                                            const int32_t *__restrict__ ltp) {
-  constexpr int RB = kXtFrames * (int)sizeof(TL);  // bytes per pixel row
+  constexpr int RB = XF * (int)sizeof(TL);  // bytes per pixel row
   constexpr int VPP = RB / 16;                     // 16-byte vectors per pixel row
   const int jobs = h.npx * VPP;
   for (int ch = wave; ch * 64 < jobs; ch += kWaves) {
     const int j = ch * 64 + lane;
     if (j < jobs) {
       const int p = ltp[j / VPP];
-      const char *src = reinterpret_cast<const char *>(xt + (int64_t)p * kXtFrames) +
+      const char *src = reinterpret_cast<const char *>(xt + (int64_t)p * XF) +
                         (j % VPP) * 16;
       __builtin_amdgcn_global_load_lds(src, (lds_void *)(plane + ch * 1024), 16, 0, 0);
     }
@@ -626,6 +662,18 @@ __device__ __forceinline__ float group_scatter(const float t[4]) {
   return __uint_as_float(c[0]) + __uint_as_float(c[1]);
 }
 
+// Sum over the four 16-lane groups, every group receiving the total (lane l
+// of each group holds the same frames): the two lane-half swaps return each
+// value's partner, and the sums are the same bits in every group.
+__device__ __forceinline__ float group_allreduce(float v) {
+  const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false,
+                                                  false);
+  v = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+  const auto b = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false,
+                                                  false);
+  return __uint_as_float(b[0]) + __uint_as_float(b[1]);
+}
+
 // Compensated sum: products are added with plain FMAs over blocks of up to
 // 32 entries; a bin with more entries adds its blocks through a Kahan
 // accumulator (relative error about 32 ulp of a block plus 2 ulp,
@@ -650,7 +698,7 @@ __device__ __forceinline__ void kahan_add(Kahan &k, float x) {
 // direct flag) are read once into scalar registers: the granule bookkeeping
 // and the entry loop bounds are then scalar work, and the granule slot is a
 // uniform index.
-template <typename TL, int G, bool DARK>
+template <typename TL, int G, bool DARK, int FPL>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batched(
     const int32_t *__restrict__ chunks, const int32_t *__restrict__ xcd,
     const int32_t *__restrict__ tiles, const BinRec *__restrict__ recs,
@@ -658,8 +706,9 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
     const uint16_t *__restrict__ dint, int n_frames, int ent_off, int plane_off, int tp_off,
     int tp_len, int64_t nb, float *__restrict__ cakes, float *__restrict__ part) {
   extern __shared__ __attribute__((aligned(16))) char lds_raw[];
-  typedef typename Quad<TL>::type QT;
-  constexpr int QB = 4 * (int)sizeof(TL);  // bytes of one lane's 4 frames
+  typedef typename Lanes<TL, FPL>::type QT;
+  constexpr int XF = 16 * FPL;                // frames per pass (pixel row)
+  constexpr int QB = FPL * (int)sizeof(TL);   // bytes of one lane's frames
   constexpr int RBAND = kWaves * G;
   constexpr int LG = 8;  // cake bins per granule: 32-byte stores
   typedef typename GranVec<LG>::type VT;
@@ -672,7 +721,11 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int grp = lane >> 4, l16 = lane & 15;
-  const int frame = 4 * l16 + grp;  // the frame this lane keeps
+  // The frame this lane keeps after the groups' sums meet: 64-frame passes
+  // spread 64 frames over the 64 lanes; narrower passes give each frame to
+  // one lane of the first 4 / FPL groups (the other groups hold copies).
+  const int frame = FPL == 4 ? 4 * l16 + grp : FPL == 2 ? 2 * l16 + (grp & 1) : l16;
+  const bool keeper = FPL == 4 || grp < 4 / FPL;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int x = blockIdx.x & 7;
   const int c = xcd[x] + (int)(blockIdx.x >> 3);
@@ -692,7 +745,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
     for (int k = 0; k < 4; ++k) lo[g][k] = 0.f;
   }
   const bool vec_ok = ((nb & 3) == 0) && (((uintptr_t)cakes & 15) == 0);
-  const bool live = frame < n_frames;
+  const bool live = keeper && frame < n_frames;
   const char *lpl = plane + l16 * QB;  // this lane's 4 frames in a pixel row
 
   // Pixel lists of tiles t (buffer (t - t_beg) & 1) and t + 1 (the other):
@@ -710,8 +763,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
     const TileHdr h = hn;
     const int cur = (t - t_beg) & 1;
     if (t > t_beg) __syncthreads();  // every wave is done with the previous tile's LDS
-    tile_stage<TL>(h, wave, lane, lds_raw, reinterpret_cast<char *>(lent), plane, recs, ents,
-                   xt, reinterpret_cast<const int32_t *>(lds_raw + tp_off + cur * tp_len));
+    tile_stage<TL, XF>(h, wave, lane, lds_raw, reinterpret_cast<char *>(lent), plane, recs,
+                       ents, xt, reinterpret_cast<const int32_t *>(lds_raw + tp_off + cur * tp_len));
     if (t + 1 < t_end) {
       hn = tile_hdr(tiles, t + 1);
       stage_pixels(tpix, hn.px_beg, hn.npx, wave, lane, lds_raw + tp_off + (cur ^ 1) * tp_len);
@@ -734,7 +787,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
         const int rbeg = __builtin_amdgcn_readfirstlane(r.tb.y);
         const int rend = __builtin_amdgcn_readfirstlane(r.tb.z);
         const int rdir = __builtin_amdgcn_readfirstlane(r.tb.w);
-        float tot[4];
+        float tot[FPL];
         if (!rdir) {
           int k = rbeg - h.e_beg;
           const int nblk = (rend - rbeg) >> 3;
@@ -742,8 +795,10 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
             tw = *reinterpret_cast<const u4v *>(lent + k + 2 * grp);
             primed = true;
           }
-          float gs[4] = {0.f, 0.f, 0.f, 0.f};
-          Kahan acc[4];  // set at the fourth group of 8, used only past it
+          float gs[FPL];
+#pragma unroll
+          for (int j = 0; j < FPL; ++j) gs[j] = 0.f;
+          Kahan acc[FPL];  // set at the fourth group of 8, used only past it
           for (int blk = 0; blk < nblk; ++blk, k += 8) {
             const u4v two = tw;
             const uint32_t oa = DARK ? (two.x & 0xffffu) : two.x;
@@ -753,23 +808,23 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
             // The run's next group (past the run's end this reads unused
             // LDS words inside the allocation).
             tw = *reinterpret_cast<const u4v *>(lent + k + 8 + 2 * grp);
-            float fa[4], fb[4];
-            Quad<TL>::unpack(da, fa);
-            Quad<TL>::unpack(db, fb);
+            float fa[FPL], fb[FPL];
+            Lanes<TL, FPL>::unpack(da, fa);
+            Lanes<TL, FPL>::unpack(db, fb);
             if constexpr (DARK) {
               const float dka = (float)(two.x >> 16), dkb = (float)(two.z >> 16);
 #pragma unroll
-              for (int j = 0; j < 4; ++j) {
+              for (int j = 0; j < FPL; ++j) {
                 fa[j] -= dka;
                 fb[j] -= dkb;
               }
             }
             const float wa = __uint_as_float(two.y), wb = __uint_as_float(two.w);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) gs[j] = fmaf(fb[j], wb, fmaf(fa[j], wa, gs[j]));
+            for (int j = 0; j < FPL; ++j) gs[j] = fmaf(fb[j], wb, fmaf(fa[j], wa, gs[j]));
             if ((blk & 3) == 3 && blk + 1 < nblk) {  // more than 32 entries: compensate
 #pragma unroll
-              for (int j = 0; j < 4; ++j) {
+              for (int j = 0; j < FPL; ++j) {
                 if (blk == 3) {
                   acc[j].s = gs[j];
                   acc[j].c = 0.f;
@@ -782,36 +837,44 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
           }
           if (nblk > 4) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < FPL; ++j) {
               kahan_add(acc[j], gs[j]);
               tot[j] = acc[j].value();
             }
           } else {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) tot[j] = gs[j];
+            for (int j = 0; j < FPL; ++j) tot[j] = gs[j];
           }
         } else {
-          Kahan acc[4];
+          Kahan acc[FPL];
           // Entries stay in global memory: group g takes entries g, g+4, ...
           for (int k = rbeg + grp; k < rend; k += 4) {
             const uint2 en = ents[k];
             const QT d = *reinterpret_cast<const QT *>(
-                reinterpret_cast<const char *>(xt + (int64_t)en.x * kXtFrames) + l16 * QB);
-            float fv[4];
-            Quad<TL>::unpack(d, fv);
+                reinterpret_cast<const char *>(xt + (int64_t)en.x * XF) + l16 * QB);
+            float fv[FPL];
+            Lanes<TL, FPL>::unpack(d, fv);
             if constexpr (DARK) {
               const float dk = (float)dint[en.x];
 #pragma unroll
-              for (int j = 0; j < 4; ++j) fv[j] -= dk;
+              for (int j = 0; j < FPL; ++j) fv[j] -= dk;
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) kahan_add(acc[j], fv[j] * __uint_as_float(en.y));
+            for (int j = 0; j < FPL; ++j) kahan_add(acc[j], fv[j] * __uint_as_float(en.y));
           }
 #pragma unroll
-          for (int j = 0; j < 4; ++j) tot[j] = acc[j].value();
+          for (int j = 0; j < FPL; ++j) tot[j] = acc[j].value();
         }
         // The four groups hold partial sums of the same frames.
-        const float mine = group_scatter(tot);
+        float mine;
+        if constexpr (FPL == 4) {
+          mine = group_scatter(tot);
+        } else if constexpr (FPL == 2) {
+          const float s0 = group_allreduce(tot[0]), s1 = group_allreduce(tot[1]);
+          mine = (grp & 1) ? s1 : s0;
+        } else {
+          mine = group_allreduce(tot[0]);
+        }
         const int gr = rbin / LG, gk = rbin - gr * LG;
         if (gr != og[g]) {
           if (om[g] && live) flush_granule<LG>(ob[g], og[g], om[g], frame, cakes, nb, vec_ok);
@@ -835,11 +898,11 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     if (om[g] && live) flush_granule<LG>(ob[g], og[g], om[g], frame, cakes, nb, vec_ok);
-    if (want_lo) {
-      // part[c][row][k][64] for the band row this slot holds.
-      float *Pp = part + ((int64_t)c * RBAND + kWaves * g + wave) * 4 * kXtFrames;
+    if (want_lo && keeper) {
+      // part[c][row][k][XF] for the band row this slot holds.
+      float *Pp = part + ((int64_t)c * RBAND + kWaves * g + wave) * 4 * XF;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) Pp[k * kXtFrames + frame] = lo[g][k];
+      for (int k = 0; k < 4; ++k) Pp[k * XF + frame] = lo[g][k];
     }
   }
 }
@@ -852,12 +915,12 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
 __global__ __launch_bounds__(256) void k_ents_dark(const int32_t *__restrict__ tiles,
                                                    const int32_t *__restrict__ tpix,
                                                    const uint16_t *__restrict__ dint,
-                                                   uint2 *__restrict__ ents) {
+                                                   int row_bytes, uint2 *__restrict__ ents) {
   const int32_t *T = tiles + kTileInts * blockIdx.x;
   const int px_beg = T[0], npx = T[1], e_beg = T[6], e_end = T[7];
   for (int i = e_beg + (int)threadIdx.x; i < e_end; i += 256) {
     const uint32_t off = ents[i].x & 0xffffu;
-    const int slot = (int)(off / (kXtFrames * 2));
+    const int slot = (int)(off / row_bytes);
     uint32_t d = 0u;
     if (dint && slot < npx) d = dint[tpix[px_beg + slot]];
     ents[i].x = off | (d << 16);
@@ -1200,7 +1263,8 @@ int launch_ents_dark(hipStream_t s, const Map &m, Tiles &t) {
   if (t.in_size != 2 || t.n_tiles == 0) return MC_OK;
   hipLaunchKernelGGL(k_ents_dark, dim3((unsigned)t.n_tiles), dim3(256), 0, s,
                      t.tiles.as<int32_t>(), t.tpix.as<int32_t>(),
-                     m.has_dark ? m.d_dint.as<uint16_t>() : nullptr, t.ents.as<uint2>());
+                     m.has_dark ? m.d_dint.as<uint16_t>() : nullptr, t.width * 2,
+                     t.ents.as<uint2>());
   MC_HIP(hipGetLastError());
   return MC_OK;
 }
@@ -1209,10 +1273,11 @@ size_t batch_xt_bytes(const Map &m, int dtype) {
   return (size_t)m.npx() * kXtFrames * (dtype_size(dtype) <= 2 ? 2 : 4);
 }
 
-template <typename TIn, typename TL, int G>
+template <typename TIn, typename TL, int G, int FPL>
 static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mask_nan,
                             const void *frames, int n_frames, float *cakes, float *lineouts,
                             void *xt, float *part) {
+  constexpr int XF = 16 * FPL;
   const uint16_t *dhi = m.has_dark ? m.d_dint.as<uint16_t>() : nullptr;
   // 1. pixel-major batch
   const int64_t npx = m.npx();
@@ -1220,11 +1285,11 @@ static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mas
   // Only the blocks holding pixels the map reads are rewritten.
   if (sizeof(TIn) == 2 && sizeof(TL) == 2 && vec_ok && npx % 128 == 0) {
     if (t.n_blocks128)
-      hipLaunchKernelGGL(k_transpose16, dim3((unsigned)t.n_blocks128), dim3(256), 0, s,
+      hipLaunchKernelGGL(k_transpose16<XF>, dim3((unsigned)t.n_blocks128), dim3(256), 0, s,
                          (const uint16_t *)frames, npx, n_frames, npx,
                          t.blocks128.as<int32_t>(), (uint16_t *)xt);
   } else if (t.n_blocks64) {
-    hipLaunchKernelGGL((k_transpose<TIn, TL>), dim3((unsigned)t.n_blocks64), dim3(256), 0, s,
+    hipLaunchKernelGGL((k_transpose<TIn, TL, XF>), dim3((unsigned)t.n_blocks64), dim3(256), 0, s,
                        (const TIn *)frames, npx, n_frames, npx, vec_ok, t.blocks64.as<int32_t>(),
                        dhi, (TL *)xt);
   }
@@ -1232,7 +1297,7 @@ static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mas
   // 2. tiles (uint16 batches subtract the dark carried by the entries;
   // FP32 batches were dark-subtracted by the transpose)
   constexpr bool kU16 = sizeof(TL) == 2;
-  auto kern = (kU16 && dhi) ? k_batched<TL, G, kU16> : k_batched<TL, G, false>;
+  auto kern = (kU16 && dhi) ? k_batched<TL, G, kU16, FPL> : k_batched<TL, G, false, FPL>;
   const size_t lds = t.lds_needed;
   if (lds > 65536)
     MC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1250,7 +1315,7 @@ static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mas
     const int64_t n = (int64_t)n_frames * m.n_r;
     hipLaunchKernelGGL(k_lineout_finalize, dim3(blocks_for(n, 256)), dim3(256), 0, s, part,
                        t.band_chunk.as<int32_t>(), m.d_row_static.as<float4>(), n_frames,
-                       m.n_r, t.rows, kXtFrames, mask_nan, lineouts);
+                       m.n_r, t.rows, XF, mask_nan, lineouts);
     MC_HIP(hipGetLastError());
   }
   return MC_OK;
@@ -1260,8 +1325,8 @@ int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *bi
                    int mask_nan, const void *frames, int dtype, int n_frames, float *cakes,
                    float *lineouts, void *xt, float *part) {
   (void)binfo;  // folded into the tile records (launch_tile_recs)
-  if (n_frames < 1 || n_frames > kXtFrames) {
-    set_error("batched pass of %d frames outside 1..%d", n_frames, kXtFrames);
+  if (n_frames < 1 || n_frames > t.width) {
+    set_error("batched pass of %d frames outside 1..%d", n_frames, t.width);
     return MC_ERR_ARG;
   }
   if ((dtype_size(dtype) <= 2 ? 2 : 4) != t.in_size) {
@@ -1275,16 +1340,25 @@ int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *bi
   return dispatch_dtype(dtype, [&](auto tag) -> int {
     using T = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
     using TL = typename std::conditional<sizeof(T) <= 2, uint16_t, float>::type;
-    switch (t.rows / kWaves) {
-      case 1:
-        return launch_batched_g<T, TL, 1>(s, m, t, mask_nan, frames, n_frames, cakes, lineouts,
-                                          xt, part);
-      case 2:
-        return launch_batched_g<T, TL, 2>(s, m, t, mask_nan, frames, n_frames, cakes, lineouts,
-                                          xt, part);
-      default:
-        set_error("rows per band %d not in {16, 32}", t.rows);
-        return MC_ERR_ARG;
+    auto go = [&](auto fpl) -> int {
+      constexpr int FPL = decltype(fpl)::value;
+      switch (t.rows / kWaves) {
+        case 1:
+          return launch_batched_g<T, TL, 1, FPL>(s, m, t, mask_nan, frames, n_frames, cakes,
+                                                 lineouts, xt, part);
+        case 2:
+          return launch_batched_g<T, TL, 2, FPL>(s, m, t, mask_nan, frames, n_frames, cakes,
+                                                 lineouts, xt, part);
+        default:
+          set_error("rows per band %d not in {16, 32}", t.rows);
+          return MC_ERR_ARG;
+      }
+    };
+    switch (t.width) {
+      case 16: return go(std::integral_constant<int, 1>());
+      case 32: return go(std::integral_constant<int, 2>());
+      case 64: return go(std::integral_constant<int, 4>());
+      default: set_error("pass width %d not in {16, 32, 64}", t.width); return MC_ERR_ARG;
     }
   });
 }

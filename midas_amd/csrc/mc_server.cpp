@@ -37,10 +37,12 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-// Batches of at most this many frames run the one-frame kernel per frame
-// (measured on 2048^2 uint16: 113 us per frame vs 702 us per batched
-// launch of up to 64 frames).
-constexpr int kPlainMaxFrames = 6;
+// Batches of at most this many frames run the one-frame kernel per frame;
+// larger ones run batched passes of the narrowest width (16, 32 or 64
+// frames) that holds them.  Measured on 2048^2 uint16 with a dark: 113 us
+// per frame one at a time; one pass of 16 frames 296 us whatever its frame
+// count, of 32 frames 360-410 us, of 64 frames 500-700 us.
+constexpr int kPlainMaxFrames = 2;
 
 double ms_since(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
@@ -225,11 +227,11 @@ void scheduler(mc_server *sv) {
         rc = MC_ERR_HIP;
     if (!rc) {
       std::lock_guard<std::mutex> mlk(G.m->mu);
-      // A batched launch costs about as much as 6 one-frame launches (it
-      // computes all 64 frame lanes): small batches take the one-frame
-      // kernel per frame.
+      // Small batches take the one-frame kernel per frame; the others one
+      // pass of the narrowest width that holds them.
       rc = run_integrate(G.m, S.s, S.frames.p, G.dtype, nf, G.mode, G.flags, S.cakes.as<float>(),
-                         S.lo.as<float>(), nf <= kPlainMaxFrames, false, &S.work);
+                         S.lo.as<float>(), nf <= kPlainMaxFrames, false, &S.work,
+                         width_of(nf));
     }
     if (!rc && hipMemcpyAsync(S.h_cakes, S.cakes.p, (size_t)nf * G.cake_floats * 4,
                               hipMemcpyDeviceToHost, S.s) != hipSuccess)
@@ -331,12 +333,17 @@ int mc_server_add_geometry(mc_server *sv, mc_map *m, int32_t dtype, int32_t mode
   int rc;
   size_t pb = 0;  // lineout partials of this geometry's batched passes
   {
+    // The layouts of every pass width a batch of up to max_batch frames
+    // may use are built now, not on a request's path.
     std::lock_guard<std::recursive_mutex> clk(sv->ctx->mu);
     std::lock_guard<std::mutex> mlk(m->mu);
-    if ((rc = prepare_batched(m, mode, flags, dtype))) return rc;
-    Tiles *t = nullptr;
-    if ((rc = ensure_tiles(m, mode, dtype_size(dtype) <= 2 ? 2 : 4, &t))) return rc;
-    pb = t->part_bytes();
+    for (int w : {16, 32, 64}) {
+      if (w > width_of(sv->max_batch)) break;
+      if ((rc = prepare_batched(m, mode, flags, dtype, w))) return rc;
+      Tiles *t = nullptr;
+      if ((rc = ensure_tiles(m, mode, dtype_size(dtype) <= 2 ? 2 : 4, &t, w))) return rc;
+      pb = std::max(pb, t->part_bytes());
+    }
   }
   Geo G;
   G.m = m;

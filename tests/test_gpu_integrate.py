@@ -94,21 +94,24 @@ def test_batched_tilings_agree(batch, rows):
         np.testing.assert_allclose(lb[i], l1[0], rtol=3e-6, atol=1e-4)
 
 
-def test_small_lds_budget_uses_direct_bins():
+@pytest.mark.parametrize("batch,tight_lds", [(8, 24576 + 40 * 48), (64, 24576 + 128 * 48)])
+def test_small_lds_budget_uses_direct_bins(batch, tight_lds):
     """Coarse bins under a small LDS budget: bins whose own pixel box does
     not fit become direct (global gathers) and columns split into row
-    ranges; results still agree with the roomy layout."""
+    ranges; results still agree with the roomy layout.  Passes of 16 frames
+    stage 32-byte pixel rows, 64-frame passes 128-byte rows: each gets a
+    budget that leaves bins direct."""
     P = IntegrationParams(NrPixelsY=160, NrPixelsZ=144, pxY=200.0, pxZ=200.0, Lsd=1e6,
                           BC_y=80.5, BC_z=70.25, RMin=2.0, RMax=100.0, RBinSize=1.5,
                           EtaBinSize=30.0)
     rng = np.random.default_rng(11)
     frames = rng.poisson(200.0, (9, P.NrPixelsZ, P.NrPixelsY)).astype(np.uint16)
     g = ma.build_map(P)
-    g.set_tuning(batch=8, rows_per_group=16, lds_bytes=24576 + 128 * 48)
+    g.set_tuning(batch=batch, rows_per_group=16, lds_bytes=tight_lds)
     tight = g.layout_info()
     assert tight["direct_bins"] > 0
     cb, lb = ma.integrate_batch(frames, g, mode="bilinear")
-    g.set_tuning(batch=8, rows_per_group=32, lds_bytes=163840)
+    g.set_tuning(batch=batch, rows_per_group=32, lds_bytes=163840)
     assert g.layout_info()["direct_bins"] < tight["direct_bins"]
     cr, lr = ma.integrate_batch(frames, g, mode="bilinear")
     np.testing.assert_allclose(cb, cr, rtol=3e-6, atol=1e-4, equal_nan=True)
@@ -263,3 +266,22 @@ def test_tile_shapes_are_bit_identical(bins, ents):
     cs, ls = ma.integrate_batch(frames, g, mode="bilinear", mask_nan=True)
     np.testing.assert_array_equal(cs, cd)
     np.testing.assert_allclose(ls, ld, rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.float32, np.uint8])
+@pytest.mark.parametrize("batch", [16, 32])
+def test_narrow_passes_match_oracle(dtype, batch, oracle_lib):
+    """Passes of 16 and 32 frames (the narrow kernels: one or two frames
+    per lane) for the 32-bit and 8-bit batch paths, against the FP64
+    oracle at 1e-5."""
+    P, mask, frames, dark = _setup("panels", n_frames=batch, seed=9, dtype=dtype)
+    sh = panel_shifts_for("panels")
+    g = ma.build_map(P, mask=mask, panel_shifts=sh)
+    g.set_dark(dark)
+    g.set_tuning(batch=batch)
+    om = oracle_lib.build_map(to_oracle(P), mask=mask, panel_shifts=sh)
+    dmean = dark.astype(np.float64).mean(axis=0)
+    cakes, lo = ma.integrate_batch(frames, g, mode="bilinear", mask_nan=True)
+    for i in (0, batch // 2, batch - 1):
+        ref, scale_b = _oracle(oracle_lib, om, P, frames[i], dmean, "bilinear")
+        assert rel_err(cakes[i], ref, scale_b).max() <= TOL
