@@ -280,14 +280,15 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
       std::sort(v.begin(), v.end());
       v.erase(std::unique(v.begin(), v.end()), v.end());
       big[q] = (int64_t)v.size() > cap_px ||
-               ((c.h_row[q + 1] - c.h_row[q] + 7) & ~7) > kTileEnts;
+               ((c.h_row[q + 1] - c.h_row[q] + 3) & ~3) > kTileEnts;
       if (big[q]) std::vector<int32_t>().swap(v);
     }
   });
   auto cnt = [&](int64_t b) { return (int64_t)(c.h_row[b + 1] - c.h_row[b]); };
-  // Staged entries of a bin are padded to whole groups of 8 (zero-weight
-  // entries on the zero pixel row).
-  auto cntp = [&](int64_t b) { return (cnt(b) + 7) & ~(int64_t)7; };
+  // Staged entries of a bin are padded to a multiple of 4 (zero-weight
+  // entries on the zero pixel row): the kernel reads them 8 at a time, the
+  // last group of a bin possibly half.
+  auto cntp = [&](int64_t b) { return (cnt(b) + 3) & ~(int64_t)3; };
   // Size of the union of two sorted pixel sets.
   auto union_size = [](const std::vector<int32_t> &x, const std::vector<int32_t> &y) {
     size_t i = 0, j = 0, n = 0;
@@ -442,7 +443,7 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
           real_ents++;
         }
         if (!dir)
-          while ((ents.size() - (size_t)e_beg) % 8) ents.push_back(make_uint2(zero_row, 0u));
+          while ((ents.size() - (size_t)e_beg) % 4) ents.push_back(make_uint2(zero_row, 0u));
       };
       // Records row-major; staged entries first (contiguous), then direct.
       std::vector<int4> recs((size_t)T.nrows * T.nc);

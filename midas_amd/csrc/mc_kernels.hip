@@ -709,6 +709,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
   typedef typename Lanes<TL, FPL>::type QT;
   constexpr int XF = 16 * FPL;                // frames per pass (pixel row)
   constexpr int QB = FPL * (int)sizeof(TL);   // bytes of one lane's frames
+  constexpr int RB = XF * (int)sizeof(TL);    // bytes of a staged pixel row
   constexpr int RBAND = kWaves * G;
   constexpr int LG = 8;  // cake bins per granule: 32-byte stores
   typedef typename GranVec<LG>::type VT;
@@ -790,7 +791,11 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
         float tot[FPL];
         if (!rdir) {
           int k = rbeg - h.e_beg;
-          const int nblk = (rend - rbeg) >> 3;
+          // A bin's staged entries are padded to a multiple of 4: blocks of
+          // 8 (two per group), the last one possibly half (groups 0 and 1).
+          const int nent = rend - rbeg;
+          const int nblk = (nent + 7) >> 3;
+          const bool tail = (nent & 4) != 0;
           if (!primed) {
             tw = *reinterpret_cast<const u4v *>(lent + k + 2 * grp);
             primed = true;
@@ -799,15 +804,22 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
 #pragma unroll
           for (int j = 0; j < FPL; ++j) gs[j] = 0.f;
           Kahan acc[FPL];  // set at the fourth group of 8, used only past it
-          for (int blk = 0; blk < nblk; ++blk, k += 8) {
-            const u4v two = tw;
+          for (int blk = 0; blk < nblk; ++blk) {
+            u4v two = tw;
+            const bool half = tail && blk == nblk - 1;
+            if (half && grp >= 2) {
+              // the other half belongs to the next bin: the zero row, weight 0
+              two.x = two.z = (uint32_t)(h.npx * RB);
+              two.y = two.w = 0u;
+            }
             const uint32_t oa = DARK ? (two.x & 0xffffu) : two.x;
             const uint32_t obb = DARK ? (two.z & 0xffffu) : two.z;
             const QT da = *reinterpret_cast<const QT *>(lpl + oa);
             const QT db = *reinterpret_cast<const QT *>(lpl + obb);
+            k += half ? 4 : 8;
             // The run's next group (past the run's end this reads unused
             // LDS words inside the allocation).
-            tw = *reinterpret_cast<const u4v *>(lent + k + 8 + 2 * grp);
+            tw = *reinterpret_cast<const u4v *>(lent + k + 2 * grp);
             float fa[FPL], fb[FPL];
             Lanes<TL, FPL>::unpack(da, fa);
             Lanes<TL, FPL>::unpack(db, fb);
