@@ -129,7 +129,7 @@ def fit_bench(g, P, frames, log, n_frames=32):
     cakes, _ = ma.integrate_batch(frames[:n_frames], g, mode="bilinear", mask_nan=False)
     RLo, RHi, _, _ = ma.bin_edges(P)
     rc = (RLo + RHi) / 2
-    rings = synth.ring_radii(P, 10)
+    rings = synth.ring_radii(P, 12)[:10]  # the first 10 of the frames' 12 rings
     res = {}
     for model in ("pv", "tch"):
         peakfit.fit_cake_rings(cakes, g, rc, rings, pad=20, model=model)  # warm

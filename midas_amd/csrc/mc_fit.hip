@@ -62,6 +62,14 @@ template <int MODEL> constexpr int n_rows() {
 }
 constexpr float kLn2 = 0.69314718055994530942f;
 
+// The normal equations are symmetric: their lower triangle is kept packed,
+// row after row (element (a, b), b <= a, at a (a + 1) / 2 + b).
+template <int NP> constexpr int n_tri() { return NP * (NP + 1) / 2; }
+__device__ __forceinline__ constexpr int tri(int a, int b) { return a * (a + 1) / 2 + b; }
+// Loops over parameters are unrolled (compile-time indices keep the arrays
+// in registers) up to this many parameters.
+template <int NP> constexpr int unroll_np() { return NP <= 10 ? NP : 1; }
+
 struct Win {
   const float *xr;  // sample abscissae relative to x0 (computed in FP64)
   const float *y;   // samples
@@ -186,10 +194,12 @@ __device__ __forceinline__ float tch_peak(const double *p, const Tch &t, float x
   return A * shape;
 }
 
-// Sum of squared residuals and (optionally) the FP64 normal equations.
+// Sum of squared residuals and (optionally) the FP64 normal equations
+// (JtJ packed lower triangle).
 template <int MODEL>
 __device__ double chi2_normal(const Win &w, const double *p, double *JtJ, double *Jtr) {
   constexpr int NP = n_params<MODEL>();
+  constexpr int UF = unroll_np<NP>();
   constexpr int NPK = tch_peaks<MODEL>();
   const float cr = (float)(p[0] - w.x0);
   const float cr2 = MODEL == MC_FIT_PV2 ? (float)(p[1] - w.x0) : 0.0f;
@@ -202,7 +212,9 @@ __device__ double chi2_normal(const Win &w, const double *p, double *JtJ, double
     tc[k] = (float)(p[4 * k + 1] - w.x0);
   }
   if (JtJ) {
-    for (int a = 0; a < NP * NP; a++) JtJ[a] = 0;
+#pragma unroll UF
+    for (int a = 0; a < n_tri<NP>(); a++) JtJ[a] = 0;
+#pragma unroll UF
     for (int a = 0; a < NP; a++) Jtr[a] = 0;
   }
   double chi = 0;
@@ -233,35 +245,46 @@ __device__ double chi2_normal(const Win &w, const double *p, double *JtJ, double
       for (int a = 0; a < NP; a++) {
         Jtr[a] += (double)J[a] * r;
 #pragma unroll
-        for (int b = 0; b <= a; b++) JtJ[a * NP + b] += (double)J[a] * J[b];
+        for (int b = 0; b <= a; b++) JtJ[tri(a, b)] += (double)J[a] * J[b];
       }
     }
   }
   return chi;
 }
 
+// Cholesky factorisation and solve of the packed matrix A (overwritten by
+// its factor) with right-hand side b (overwritten by the solution).
 template <int NP> __device__ bool cholesky_solve(double *A, double *b) {
+  constexpr int UF = unroll_np<NP>();
+#pragma unroll UF
   for (int j = 0; j < NP; j++) {
-    double s = A[j * NP + j];
-    for (int k = 0; k < j; k++) s -= A[j * NP + k] * A[j * NP + k];
+    double s = A[tri(j, j)];
+#pragma unroll UF
+    for (int k = 0; k < j; k++) s -= A[tri(j, k)] * A[tri(j, k)];
     if (!(s > 0)) return false;
-    const double d = sqrt(s);
-    A[j * NP + j] = d;
+    const double dd = sqrt(s);
+    A[tri(j, j)] = dd;
+#pragma unroll UF
     for (int i = j + 1; i < NP; i++) {
-      double t = A[i * NP + j];
-      for (int k = 0; k < j; k++) t -= A[i * NP + k] * A[j * NP + k];
-      A[i * NP + j] = t / d;
+      double t = A[tri(i, j)];
+#pragma unroll UF
+      for (int k = 0; k < j; k++) t -= A[tri(i, k)] * A[tri(j, k)];
+      A[tri(i, j)] = t / dd;
     }
   }
+#pragma unroll UF
   for (int i = 0; i < NP; i++) {
     double t = b[i];
-    for (int k = 0; k < i; k++) t -= A[i * NP + k] * b[k];
-    b[i] = t / A[i * NP + i];
+#pragma unroll UF
+    for (int k = 0; k < i; k++) t -= A[tri(i, k)] * b[k];
+    b[i] = t / A[tri(i, i)];
   }
+#pragma unroll UF
   for (int i = NP - 1; i >= 0; i--) {
     double t = b[i];
-    for (int k = i + 1; k < NP; k++) t -= A[k * NP + i] * b[k];
-    b[i] = t / A[i * NP + i];
+#pragma unroll UF
+    for (int k = i + 1; k < NP; k++) t -= A[tri(k, i)] * b[k];
+    b[i] = t / A[tri(i, i)];
   }
   return true;
 }
@@ -380,25 +403,29 @@ __device__ void fit_one(const Win &w, double rstep, const double *guess, int max
     lo[4 * k + 2] = sqrt(lo[4 * k + 2]);
     hi[4 * k + 2] = sqrt(hi[4 * k + 2]);
   }
-  double JtJ[NP * NP], Jtr[NP], M[NP * NP], d[NP], q[NP];
+  constexpr int UF = unroll_np<NP>();
+  double JtJ[n_tri<NP>()], Jtr[NP], M[n_tri<NP>()], d[NP], q[NP];
   double chi = chi2_normal<MODEL>(w, p, JtJ, Jtr);
   double lam = 1e-3;
   int it = 0, stall = 0;
   for (; it < max_iter; it++) {
     bool accepted = false;
     for (int tries = 0; tries < 12 && !accepted; tries++) {
+#pragma unroll UF
+      for (int a = 0; a < n_tri<NP>(); a++) M[a] = JtJ[a];
+#pragma unroll UF
       for (int a = 0; a < NP; a++) {
-        for (int b = 0; b <= a; b++) {
-          M[a * NP + b] = JtJ[a * NP + b];
-          M[b * NP + a] = JtJ[a * NP + b];
-        }
-        M[a * NP + a] += lam * fmax(JtJ[a * NP + a], 1e-30);
+        M[tri(a, a)] += lam * fmax(JtJ[tri(a, a)], 1e-30);
         d[a] = Jtr[a];
       }
+#pragma unroll UF
       for (int a = 0; a < NP; a++) {  // active set: hold a bound that pushes outwards
         if ((p[a] <= lo[a] && Jtr[a] < 0) || (p[a] >= hi[a] && Jtr[a] > 0)) {
-          for (int b = 0; b < NP; b++) M[a * NP + b] = M[b * NP + a] = 0;
-          M[a * NP + a] = 1;
+#pragma unroll UF
+          for (int b = 0; b < a; b++) M[tri(a, b)] = 0;
+#pragma unroll UF
+          for (int b = a + 1; b < NP; b++) M[tri(b, a)] = 0;
+          M[tri(a, a)] = 1;
           d[a] = 0;
         }
       }
@@ -407,6 +434,7 @@ __device__ void fit_one(const Win &w, double rstep, const double *guess, int max
         continue;
       }
       double step = 0, mag = 0;
+#pragma unroll UF
       for (int a = 0; a < NP; a++) {
         q[a] = fmin(fmax(p[a] + d[a], lo[a]), hi[a]);
         step += (q[a] - p[a]) * (q[a] - p[a]);
@@ -415,6 +443,7 @@ __device__ void fit_one(const Win &w, double rstep, const double *guess, int max
       const double c2 = chi2_normal<MODEL>(w, q, nullptr, nullptr);
       if (c2 <= chi) {
         const double dchi = chi - c2;
+#pragma unroll UF
         for (int a = 0; a < NP; a++) p[a] = q[a];
         chi = chi2_normal<MODEL>(w, p, JtJ, Jtr);
         lam = fmax(lam / 10, 1e-12);
