@@ -11,6 +11,7 @@ import pytest
 import midas_amd as ma
 from midas_amd.params import IntegrationParams
 from midas_amd.server import FrameServer, send_frame
+from helpers import rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -53,8 +54,8 @@ def test_mix_equals_alone():
             assert r.status == 0 and r.geometry == gids[k]
             alone, lo = ma.integrate_batch(frames[k][i], maps[k][0], mode="bilinear",
                                            mask_nan=True)
-            np.testing.assert_allclose(r.cake, alone[0], rtol=1e-5, atol=1e-3, equal_nan=True)
-            np.testing.assert_allclose(r.lineouts, lo[0], rtol=1e-5, atol=1e-3)
+            assert rel_err(r.cake, alone[0], alone[0]).max() <= 1e-5
+            assert rel_err(r.lineouts, lo[0], lo[0]).max() <= 1e-5
 
 
 def test_budget_admission():
@@ -87,8 +88,7 @@ def test_tcp_wire_protocol():
         ids = sorted(srv.results)
         for i, rid in enumerate(ids):
             alone, _ = ma.integrate_batch(frames[i], g, mode="bilinear", mask_nan=True)
-            np.testing.assert_allclose(srv.results[rid].cake, alone[0], rtol=1e-5, atol=1e-3,
-                                       equal_nan=True)
+            assert rel_err(srv.results[rid].cake, alone[0], alone[0]).max() <= 1e-5
 
 
 def test_zero_copy_buffers_and_large_frames():
@@ -116,8 +116,7 @@ def test_zero_copy_buffers_and_large_frames():
         srv.drain()
         for i in range(6):
             alone, lo = ma.integrate_batch(frames[i], g, mode="bilinear", mask_nan=True)
-            np.testing.assert_allclose(srv.results[i].cake, alone[0], rtol=1e-5, atol=1e-3,
-                                       equal_nan=True)
+            assert rel_err(srv.results[i].cake, alone[0], alone[0]).max() <= 1e-5
 
 
 def test_concurrent_batches_of_one_geometry():
@@ -153,7 +152,7 @@ def test_concurrent_batches_of_one_geometry():
             r = srv.results[rid]
             assert r.status == 0
             alone, lo = ma.integrate_batch(frames[i], g, mode="bilinear", mask_nan=True)
-            np.testing.assert_allclose(r.cake, alone[0], rtol=1e-5, atol=1e-3, equal_nan=True)
+            assert rel_err(r.cake, alone[0], alone[0]).max() <= 1e-5
             np.testing.assert_allclose(r.lineouts, lo[0], rtol=1e-5, atol=0)
 
 
