@@ -226,20 +226,22 @@ int ensure_csr(Map *m, int mode) {
 // ------------------------------------------------------------------
 // Tile planner of the batched kernel.
 // ------------------------------------------------------------------
-// LDS of the batched kernel: bin records, staged entries, the staged pixel
-// rows of the pixel-major batch (64 frames per pixel), a zero pixel row, and
-// two pixel-list buffers (the current tile's and the next one's).
-constexpr int kBinRecBytes = 32, kWaves = 16, kTileInts = 16;
+// LDS of the batched kernel: staged entries, the staged pixel rows of the
+// pixel-major batch, a zero pixel row, and two pixel-list buffers (the
+// current tile's and the next one's).  Quad records are read with scalar
+// loads and take no LDS.
+constexpr int kWaves = 16, kTileInts = 16;
 
 // Tiles of the batched kernel.  Bands of H cake rows; inside a band, tiles
-// are rectangles of rows x eta columns grown column by column while the
-// footprint of their entries (the distinct pixels) fits the LDS budget; a
-// tile stages exactly its footprint, one pixel row of the batch per pixel.
-// A column that does not fit whole is split into row ranges, and a bin
-// that does not fit alone is "direct" (gathered from global memory).
-// Chunks are runs of consecutive tiles of one band of about `target`
-// entries: the work of one workgroup.  in_size is the element size of the
-// pixel-major batch (2 or 4 bytes).
+// are rectangles of rows x eta columns grown while the footprint of their
+// entries (the distinct pixels) fits the LDS budget -- four columns at a
+// time where the rows hold a multiple of 4 bins, so that tiles cut the rows
+// between quads; a tile stages exactly its footprint, one pixel row of the
+// batch per pixel.  A column that does not fit whole is split into row
+// ranges, and a bin that does not fit alone is "direct" (gathered from
+// global memory).  Chunks are runs of consecutive tiles of one band of
+// about `target` entries: the work of one workgroup.  in_size is the
+// element size of the pixel-major batch (2 or 4 bytes).
 int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
   int rc = ensure_csr(m, mode);
   if (rc) return rc;
@@ -257,18 +259,22 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
   MC_CHECK(B >= 1 && B <= 128, MC_ERR_ARG, "batch %d not in 1..128", B);
   MC_CHECK(in_size == 2 || in_size == 4, MC_ERR_ARG, "batch element size %d", in_size);
   const Csr &c = m->csr[mode];
-  const int NY = m->ny, nE = m->n_eta, nR = m->n_r;
+  const int nE = m->n_eta, nR = m->n_r;
   const int64_t nb = m->nb();
   const int RB = W * in_size;  // bytes per pixel row
-  const int64_t fixed = (int64_t)kTileBins * kBinRecBytes + (int64_t)kTileEnts * 8;
-  // Pixels a tile can stage: what the LDS budget holds after the records,
-  // the entries, the zero row and the two pixel lists (4 bytes a pixel,
-  // rounded up to 16 bytes).
+  const int64_t fixed = (int64_t)kTileEnts * 8;
+  // Pixels a tile can stage: what the LDS budget holds after the entries,
+  // the zero row and the two pixel lists (4 bytes a pixel, rounded up to
+  // 16 bytes).
   int64_t cap_px = (m->lds_bytes - fixed - RB - 24) / (RB + 8);
   // uint16 entries carry the row's byte offset in 16 bits (the high half
   // holds the pixel's dark): the zero row, at npx * RB, must stay below 2^16.
   if (in_size == 2) cap_px = std::min<int64_t>(cap_px, 65535 / RB - 1);
   MC_CHECK(cap_px >= 16, MC_ERR_ARG, "LDS budget %d too small", m->lds_bytes);
+  auto cnt = [&](int64_t b) { return (int64_t)(c.h_row[b + 1] - c.h_row[b]); };
+  // Staged entries of a quad: pairs of entries per slot up to its largest
+  // bin, 8 entries a pair (four slots).
+  auto pairs = [](int64_t n) { return (n + 1) / 2; };
   // Per-bin footprints: the sorted distinct pixels of its entries; a bin
   // whose footprint or entries do not fit a tile alone is direct.
   std::vector<std::vector<int32_t>> fp(nb);
@@ -279,16 +285,11 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
       v.assign(c.h_pix.begin() + c.h_row[q], c.h_pix.begin() + c.h_row[q + 1]);
       std::sort(v.begin(), v.end());
       v.erase(std::unique(v.begin(), v.end()), v.end());
-      big[q] = (int64_t)v.size() > cap_px ||
-               ((c.h_row[q + 1] - c.h_row[q] + 3) & ~3) > kTileEnts;
+      big[q] = (int64_t)v.size() > cap_px || 8 * pairs(cnt(q)) > kTileEnts;
       if (big[q]) std::vector<int32_t>().swap(v);
     }
   });
-  auto cnt = [&](int64_t b) { return (int64_t)(c.h_row[b + 1] - c.h_row[b]); };
-  // Staged entries of a bin are padded to a multiple of 4 (zero-weight
-  // entries on the zero pixel row): the kernel reads them 8 at a time, the
-  // last group of a bin possibly half.
-  auto cntp = [&](int64_t b) { return (cnt(b) + 3) & ~(int64_t)3; };
+  auto staged_cnt = [&](int64_t b) { return big[b] ? (int64_t)0 : cnt(b); };
   // Size of the union of two sorted pixel sets.
   auto union_size = [](const std::vector<int32_t> &x, const std::vector<int32_t> &y) {
     size_t i = 0, j = 0, n = 0;
@@ -306,9 +307,14 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
     std::set_union(x.begin(), x.end(), y.begin(), y.end(), std::back_inserter(u));
     x.swap(u);
   };
+  // Last quad of each row of a tile being grown and its largest bin.
+  struct RowQ {
+    int64_t q = -1, n = 0;
+  };
   struct TileH {
     int ra = 0, nrows = 0, e0 = 0, nc = 0;
     std::vector<int32_t> px;  // footprint, sorted
+    std::vector<RowQ> rq;     // per row (whole-band tiles)
     int64_t ents = 0;         // staged entries
     bool chunk_start = false; // first tile of a workgroup's run
   };
@@ -323,6 +329,9 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
   const int64_t target =
       std::max<int64_t>(4096, (c.nnz + kBinCost * nb + kTileCost * (nb / kTileBins + 1)) / 2048);
   const int n_bands = (nR + H - 1) / H;
+  // Columns added at a time: 4 where every row starts at a multiple of 4
+  // bins (tiles then hold whole quads), else 1.
+  const int U = nE % 4 == 0 ? 4 : 1;
   std::vector<std::vector<TileH>> bands(n_bands);
   parallel_for(n_bands, [&](int64_t a, int64_t bq) {
     for (int64_t gi = a; gi < bq; gi++) {
@@ -333,6 +342,27 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
         return (int64_t)T.nrows * T.nc + nbins <= kTileBins && T.ents + ents <= kTileEnts &&
                union_size(T.px, add) <= cap_px;
       };
+      // Staged entries that columns e .. e + n - 1 of all rows add to T
+      // (rq empty: a fresh tile); with `commit` T's row quads advance.
+      auto add_cols = [&](TileH &T, int e, int n, bool commit) {
+        std::vector<RowQ> rq = T.rq;
+        rq.resize(nrows);
+        int64_t add = 0;
+        for (int j = 0; j < nrows; j++)
+          for (int u = 0; u < n; u++) {
+            const int64_t b = (int64_t)(row0 + j) * nE + e + u, q = b / 4, k = staged_cnt(b);
+            if (rq[j].q == q) {
+              add += 8 * (pairs(std::max(rq[j].n, k)) - pairs(rq[j].n));
+              rq[j].n = std::max(rq[j].n, k);
+            } else {
+              add += 8 * pairs(k);
+              rq[j].q = q;
+              rq[j].n = k;
+            }
+          }
+        if (commit) T.rq.swap(rq);
+        return add;
+      };
       TileH cur;
       bool next_starts = true;  // the next tile begins a chunk
       int64_t acc = 0;          // cost since the current chunk began
@@ -340,66 +370,81 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
         T.chunk_start = next_starts;
         next_starts = false;
         acc += kTileCost;
+        std::vector<RowQ>().swap(T.rq);
         tv.push_back(std::move(T));
       };
       auto flush = [&]() {
         if (cur.nc > 0) push(std::move(cur));
         cur = TileH();
       };
+      // Adds columns e .. e + n - 1 (all rows) to the current tile or a
+      // fresh one; false when they do not fit a fresh tile.
       std::vector<int32_t> col;
-      for (int e = 0; e < nE; e++) {
+      auto place = [&](int e, int n) {
+        col.clear();
+        for (int j = 0; j < nrows; j++)
+          for (int u = 0; u < n; u++) {
+            const int64_t b = (int64_t)(row0 + j) * nE + e + u;
+            if (!big[b]) merge_into(col, fp[b]);
+          }
+        if (cur.nc > 0 && cur.nrows == nrows) {
+          const int64_t ce = add_cols(cur, e, n, false);
+          if (room(cur, col, ce, nrows * n)) {
+            merge_into(cur.px, col);
+            cur.ents += add_cols(cur, e, n, true);
+            cur.nc += n;
+            return true;
+          }
+        }
+        flush();
+        TileH fresh;
+        fresh.nrows = nrows;
+        const int64_t ce = add_cols(fresh, e, n, false);
+        if (!room(fresh, col, ce, nrows * n)) return false;
+        cur = std::move(fresh);
+        cur.e0 = e;
+        cur.nc = n;
+        cur.px = col;
+        cur.ents = add_cols(cur, e, n, true);
+        return true;
+      };
+      for (int e = 0; e < nE; e += U) {
+        const int n = std::min(U, nE - e);
         if (e % 8 == 0 && acc >= target) {  // end the chunk here
           flush();
           next_starts = true;
           acc = 0;
         }
-        col.clear();
-        int64_t ce = 0;
-        for (int j = 0; j < nrows; j++) {
-          const int64_t b = (int64_t)(row0 + j) * nE + e;
-          acc += cnt(b) + kBinCost;
-          if (big[b]) continue;
-          merge_into(col, fp[b]);
-          ce += cntp(b);
-        }
-        if (cur.nc > 0 && cur.nrows == nrows && room(cur, col, ce, nrows)) {
-          merge_into(cur.px, col);
-          cur.ents += ce;
-          cur.nc++;
-          continue;
-        }
-        flush();
-        TileH fresh;
-        fresh.nrows = nrows;
-        if (room(fresh, col, ce, nrows)) {
-          cur = std::move(fresh);
-          cur.e0 = e;
-          cur.nc = 1;
-          cur.px = col;
-          cur.ents = ce;
-          continue;
-        }
-        // The whole column does not fit: row ranges of one column each.
-        TileH piece;
-        piece.e0 = e;
-        piece.nc = 1;
-        static const std::vector<int32_t> none;
-        for (int j = 0; j < nrows; j++) {
-          const int64_t b = (int64_t)(row0 + j) * nE + e;
-          const std::vector<int32_t> &add = big[b] ? none : fp[b];
-          const int64_t ae = big[b] ? 0 : cntp(b);
-          if (piece.nrows > 0 && !room(piece, add, ae, 1)) {
-            push(std::move(piece));
-            piece = TileH();
-            piece.e0 = e;
-            piece.nc = 1;
-            piece.ra = j;
+        for (int j = 0; j < nrows; j++)
+          for (int u = 0; u < n; u++) acc += cnt((int64_t)(row0 + j) * nE + e + u) + kBinCost;
+        if (place(e, n)) continue;
+        // The columns do not fit a fresh tile together: one at a time.
+        for (int e1 = e; e1 < e + n; e1++) {
+          if (n > 1 && place(e1, 1)) continue;
+          // The whole column does not fit: row ranges of one column each,
+          // a quad per row.
+          flush();
+          TileH piece;
+          piece.e0 = e1;
+          piece.nc = 1;
+          static const std::vector<int32_t> none;
+          for (int j = 0; j < nrows; j++) {
+            const int64_t b = (int64_t)(row0 + j) * nE + e1;
+            const std::vector<int32_t> &add = big[b] ? none : fp[b];
+            const int64_t ae = 8 * pairs(staged_cnt(b));
+            if (piece.nrows > 0 && !room(piece, add, ae, 1)) {
+              push(std::move(piece));
+              piece = TileH();
+              piece.e0 = e1;
+              piece.nc = 1;
+              piece.ra = j;
+            }
+            merge_into(piece.px, add);
+            piece.ents += ae;
+            piece.nrows++;
           }
-          merge_into(piece.px, add);
-          piece.ents += ae;
-          piece.nrows++;
+          push(std::move(piece));
         }
-        push(std::move(piece));
       }
       flush();
     }
@@ -407,12 +452,12 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
   // Flatten, starting a chunk at each tile the band planner marked.
   std::vector<int64_t> chunk_cost;
   std::vector<int32_t> chunks, band_chunk(n_bands + 1, 0), tiles;
-  std::vector<int4> tbins;
+  std::vector<int4> qs;
+  std::vector<int2> qdir;
   std::vector<uint2> ents;
   std::vector<int32_t> tpix;  // staged pixels of every tile, tile after tile
-  tbins.reserve(nb);
   ents.reserve(c.nnz + c.nnz / 4);
-  int64_t n_direct = 0, max_npx = 0, staged = 0, real_ents = 0;
+  int64_t n_direct = 0, max_npx = 0, staged = 0, real_ents = 0, n_slots = 0;
   for (int gi = 0; gi < n_bands; gi++) {
     const int row0 = gi * H, nrows_band = std::min(H, nR - row0);
     for (auto &T : bands[gi]) {
@@ -427,59 +472,89 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
       while (tpix.size() % 4) tpix.push_back(0);  // lists start at 16-byte boundaries
       max_npx = std::max<int64_t>(max_npx, npx);
       staged += npx;
-      const int32_t b_beg = (int32_t)tbins.size(), e_beg = (int32_t)ents.size();
+      const int32_t e_beg = (int32_t)ents.size(), q_beg = (int32_t)qs.size();
       const uint32_t zero_row = (uint32_t)((int64_t)npx * RB);
-      auto push_entries = [&](int64_t b, bool dir) {
-        for (int32_t k = c.h_row[b]; k < c.h_row[b + 1]; k++) {
-          const int p = c.h_pix[k];
-          uint32_t off = (uint32_t)p;
-          if (!dir) {  // byte offset of the pixel's row in the tile's staged list
-            const int64_t slot = std::lower_bound(T.px.begin(), T.px.end(), p) - T.px.begin();
-            off = (uint32_t)(slot * RB);
-          }
-          uint32_t wb;
-          memcpy(&wb, &c.h_w[k], 4);
-          ents.push_back(make_uint2(off, wb));
-          real_ents++;
-        }
-        if (!dir)
-          while ((ents.size() - (size_t)e_beg) % 4) ents.push_back(make_uint2(zero_row, 0u));
-      };
-      // Records row-major; staged entries first (contiguous), then direct.
-      std::vector<int4> recs((size_t)T.nrows * T.nc);
+      // Quads of each row: the aligned groups of 4 bins that cover the
+      // row's columns; rows are padded to the tile's largest count.
+      int nq = 0;
+      for (int j = 0; j < T.nrows; j++) {
+        const int64_t b_lo = (int64_t)(row0 + T.ra + j) * nE + T.e0, b_hi = b_lo + T.nc;
+        nq = std::max<int>(nq, (int)((b_hi - 1) / 4 - b_lo / 4 + 1));
+      }
       int64_t all_ents = 0;
-      for (int j = 0; j < T.nrows; j++)
-        for (int e = 0; e < T.nc; e++) {
-          const int64_t b = (int64_t)(row0 + T.ra + j) * nE + T.e0 + e;
-          all_ents += cnt(b);
-          if (big[b]) continue;
-          const int32_t eb = (int32_t)ents.size();
-          push_entries(b, false);
-          recs[(size_t)j * T.nc + e] = make_int4((int32_t)b, eb, (int32_t)ents.size(), 0);
+      for (int j = 0; j < T.nrows; j++) {
+        const int64_t b_lo = (int64_t)(row0 + T.ra + j) * nE + T.e0, b_hi = b_lo + T.nc;
+        const int64_t q_lo = b_lo / 4, nq_j = (b_hi - 1) / 4 - q_lo + 1;
+        for (int qi = 0; qi < nq; qi++) {
+          const int32_t e_off = (int32_t)(ents.size() - (size_t)e_beg);
+          qdir.insert(qdir.end(), 4, make_int2(0, 0));
+          if (qi >= nq_j) {
+            qs.push_back(make_int4(0, 0, e_off, 0));
+            continue;
+          }
+          const int64_t b0 = 4 * (q_lo + qi);
+          int32_t flags = 0;
+          int64_t np = 0;
+          for (int s = 0; s < 4; s++) {
+            const int64_t b = b0 + s;
+            if (b < b_lo || b >= b_hi) continue;
+            flags |= 1 << s;
+            n_slots++;
+            all_ents += cnt(b);
+            if (big[b]) flags |= 16 << s;
+            np = std::max(np, pairs(staged_cnt(b)));
+          }
+          for (int64_t mi = 0; mi < np; mi++)
+            for (int s = 0; s < 4; s++)
+              for (int u = 0; u < 2; u++) {
+                const int64_t b = b0 + s, idx = 2 * mi + u;
+                if (((flags >> s) & 1) && !big[b] && idx < cnt(b)) {
+                  const int32_t k = c.h_row[b] + (int32_t)idx;
+                  const int p = c.h_pix[k];
+                  // byte offset of the pixel's row in the tile's staged list
+                  const int64_t slot =
+                      std::lower_bound(T.px.begin(), T.px.end(), p) - T.px.begin();
+                  uint32_t wb;
+                  memcpy(&wb, &c.h_w[k], 4);
+                  ents.push_back(make_uint2((uint32_t)(slot * RB), wb));
+                  real_ents++;
+                } else {
+                  ents.push_back(make_uint2(zero_row, 0u));
+                }
+              }
+          qs.push_back(make_int4((int32_t)b0, flags, e_off, (int32_t)np));
         }
+      }
       const int32_t e_end = (int32_t)ents.size();
-      for (int j = 0; j < T.nrows; j++)
-        for (int e = 0; e < T.nc; e++) {
-          const int64_t b = (int64_t)(row0 + T.ra + j) * nE + T.e0 + e;
-          if (!big[b]) continue;
+      MC_CHECK(e_end - e_beg <= kTileEnts, MC_ERR_STATE, "tile stages %d entries over %d",
+               e_end - e_beg, kTileEnts);
+      // Direct slots: their entries (global pixel, weight) after the staged ones.
+      for (int32_t qq = q_beg; qq < (int32_t)qs.size(); qq++)
+        for (int s = 0; s < 4; s++) {
+          if (!((qs[qq].y >> (4 + s)) & 1)) continue;
+          const int64_t b = qs[qq].x + s;
           const int32_t eb = (int32_t)ents.size();
-          push_entries(b, true);
-          recs[(size_t)j * T.nc + e] = make_int4((int32_t)b, eb, (int32_t)ents.size(), 1);
+          for (int32_t k = c.h_row[b]; k < c.h_row[b + 1]; k++) {
+            uint32_t wb;
+            memcpy(&wb, &c.h_w[k], 4);
+            ents.push_back(make_uint2((uint32_t)c.h_pix[k], wb));
+            real_ents++;
+          }
+          qdir[4 * (size_t)qq + s] = make_int2(eb, (int32_t)ents.size());
           n_direct++;
         }
       // Every tile's entries start at a multiple of 8 (16-byte staging).
       while (ents.size() % 8) ents.push_back(make_uint2(0u, 0u));
-      tbins.insert(tbins.end(), recs.begin(), recs.end());
-      tiles.insert(tiles.end(), {px_beg, npx, 0, 0, b_beg, (int32_t)tbins.size(), e_beg, e_end,
-                                 T.ra, T.nc, 0, 0, T.nrows, 0, 0, 0});
+      tiles.insert(tiles.end(), {px_beg, npx, 0, 0, q_beg, nq, e_beg, e_end, T.ra, T.nc, 0, 0,
+                                 T.nrows, 0, 0, 0});
       chunks[chunks.size() - 3] = (int32_t)(tiles.size() / kTileInts);
       const int64_t tcost = all_ents + kTileCost + kBinCost * T.nrows * T.nc;
       chunk_cost.back() += tcost;
     }
     band_chunk[gi + 1] = (int32_t)(chunks.size() / 4);
   }
-  MC_CHECK((int64_t)tbins.size() == nb, MC_ERR_STATE, "tile planner covered %lld of %lld bins",
-           (long long)tbins.size(), (long long)nb);
+  MC_CHECK(n_slots == nb, MC_ERR_STATE, "tile planner covered %lld of %lld bins",
+           (long long)n_slots, (long long)nb);
   MC_CHECK(real_ents == c.nnz, MC_ERR_STATE, "tile planner placed %lld of %lld entries",
            (long long)real_ents, (long long)c.nnz);
   // The 8 XCDs take contiguous chunk ranges of equal cost (workgroups are
@@ -509,18 +584,21 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
   t->in_size = in_size;
   t->n_chunks = (int)(chunks.size() / 4);
   t->n_bands = n_bands;
+  t->plane_off = (int)fixed;
   t->tp_off = (int)(fixed + (max_npx + 1) * RB);
   t->tp_len = (int)(((max_npx + 3) & ~(int64_t)3) * 4);
   t->lds_needed = (size_t)t->tp_off + 2 * (size_t)t->tp_len;
   MC_CHECK((int64_t)t->lds_needed <= m->lds_bytes, MC_ERR_STATE, "tile LDS %zu over budget %d",
            t->lds_needed, m->lds_bytes);
   t->n_tiles = (int64_t)tiles.size() / kTileInts;
-  t->n_tbins = (int64_t)tbins.size();
+  t->n_quads = (int64_t)qs.size();
   t->n_ents = (int64_t)ents.size();
   t->n_direct = n_direct;
   t->staged_px = staged;
   if ((rc = t->chunks.alloc(chunks.size() * 4)) || (rc = t->tiles.alloc(tiles.size() * 4)) ||
-      (rc = t->tbins.alloc(tbins.size() * 16)) || (rc = t->ents.alloc(ents.size() * 8)) ||
+      (rc = t->qs.alloc(std::max<size_t>(qs.size(), 1) * 16)) ||
+      (rc = t->qdir.alloc(std::max<size_t>(qdir.size(), 1) * 8)) ||
+      (rc = t->ents.alloc(ents.size() * 8)) ||
       (rc = t->band_chunk.alloc(band_chunk.size() * 4)) || (rc = t->xcd.alloc(9 * 4)) ||
       (rc = t->tpix.alloc(std::max<size_t>(tpix.size(), 1) * 4)))
     return rc;
@@ -548,7 +626,10 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
   }
   MC_HIP(hipMemcpy(t->chunks.p, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->tiles.p, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice));
-  MC_HIP(hipMemcpy(t->tbins.p, tbins.data(), tbins.size() * 16, hipMemcpyHostToDevice));
+  if (!qs.empty()) {
+    MC_HIP(hipMemcpy(t->qs.p, qs.data(), qs.size() * 16, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(t->qdir.p, qdir.data(), qdir.size() * 8, hipMemcpyHostToDevice));
+  }
   MC_HIP(hipMemcpy(t->ents.p, ents.data(), ents.size() * 8, hipMemcpyHostToDevice));
   MC_HIP(hipMemcpy(t->band_chunk.p, band_chunk.data(), band_chunk.size() * 4,
                    hipMemcpyHostToDevice));
@@ -606,9 +687,9 @@ static int ensure_recs(Map *m, hipStream_t s, Tiles *t, int mode, uint32_t flags
     MC_HIP(hipStreamSynchronize(s));
     t->ents_dark = m->dark_version;
   }
-  if (t->recs.p && t->recs_binfo == binfo && t->recs_version == m->binfo_version[key])
+  if (t->qrecs.p && t->recs_binfo == binfo && t->recs_version == m->binfo_version[key])
     return MC_OK;
-  if ((rc = launch_tile_recs(s, *m, *t, binfo))) return rc;
+  if ((rc = launch_quad_recs(s, *m, *t, binfo))) return rc;
   MC_HIP(hipStreamSynchronize(s));
   t->recs_binfo = binfo;
   t->recs_version = m->binfo_version[key];
@@ -1026,8 +1107,8 @@ int mc_map_info_get(mc_map *m, mc_map_info *info) {
     const Csr &c = m->csr[q];
     hb += c.row.bytes + c.pix.bytes + c.w.bytes + c.dterm.bytes + c.d2term.bytes;
     for (auto &t : m->tiles[q])
-      hb += t->chunks.bytes + t->tiles.bytes + t->tbins.bytes + t->ents.bytes +
-            t->band_chunk.bytes + t->recs.bytes + t->tpix.bytes + t->blocks64.bytes +
+      hb += t->chunks.bytes + t->tiles.bytes + t->qs.bytes + t->qdir.bytes + t->ents.bytes +
+            t->band_chunk.bytes + t->qrecs.bytes + t->tpix.bytes + t->blocks64.bytes +
             t->blocks128.bytes;
   }
   info->hbm_bytes = hb;

@@ -95,6 +95,9 @@ struct Csr {
   bool d2_valid = false;
 };
 
+// Bytes of a quad record of the batched kernel (4 ints, 16 floats).
+constexpr int kQuadRecBytes = 80;
+
 // Row-block/tile layout of the batched kernel for one (mode, batch, rows,
 // LDS budget).
 struct Tiles {
@@ -102,15 +105,17 @@ struct Tiles {
   uint64_t id = 0;   // unique per layout built (captured graphs check it)
   int batch = 0, rows = 0, lds_bytes = 0;  // tuned batch, band rows (16 * G), LDS budget
   int width = 64;    // frames per pass: 16, 32 or 64 (pixel row of the pixel-major batch)
-  int bin_cap = 0, ent_cap = 0;  // bin records and staged entries a tile holds in LDS
+  int bin_cap = 0, ent_cap = 0;  // bins and staged entries a tile holds (entries in LDS)
   int in_size = 0;   // bytes per element of the pixel-major batch (2 or 4)
   int n_chunks = 0, n_bands = 0;
   size_t lds_needed = 0;
+  int plane_off = 0;  // LDS bytes: the staged pixel rows (after the entries)
   int tp_off = 0, tp_len = 0;  // LDS bytes: the two pixel-list buffers (offset, length of one)
   DBuf chunks;       // int32 x4 per chunk: tile_begin tile_end band_row0 band_rows
   DBuf tiles;        // int32 x16 per tile (see mc_kernels.hip)
-  DBuf tbins;        // int32 x4 per tile bin: cake bin, ent_begin, ent_end, direct flag
-  DBuf recs;         // 32-byte records: tbins + scale, dark term, area of one binfo table
+  DBuf qs;           // int32 x4 per quad: cake bin of slot 0, flags, entry offset, pairs
+  DBuf qrecs;        // QuadRec per quad: qs + scale, dark term, areas of one binfo table
+  DBuf qdir;         // int32 x2 per quad slot: entry range of a direct slot
   const void *recs_binfo = nullptr;
   uint64_t recs_version = 0;
   DBuf ents;         // uint32 x2: byte offset of the staged pixel row (or global pixel), weight bits
@@ -121,7 +126,7 @@ struct Tiles {
   DBuf xcd;          // int32 [9]: chunk range of each XCD (equal cost)
   int grid = 0;      // workgroups of the launch: 8 x the largest XCD range
   uint64_t ents_dark = 0;  // dark version whose pixel darks the staged entries carry
-  int64_t n_tiles = 0, n_tbins = 0, n_ents = 0, n_direct = 0, staged_px = 0;
+  int64_t n_tiles = 0, n_quads = 0, n_ents = 0, n_direct = 0, staged_px = 0;
   // Bytes of lineout partials a pass over this layout writes:
   // float [n_chunks][rows][4][64].
   size_t part_bytes() const { return (size_t)n_chunks * rows * 4 * width * sizeof(float); }
@@ -216,7 +221,7 @@ int launch_plain(hipStream_t s, const Map &m, const Csr &c, const float2 *binfo,
                  float *lineouts, int n_frames, int64_t frame_stride, float *sub,
                  int64_t sub_frames);
 // Tile records of t for one binfo table (scale/dark/area folded in).
-int launch_tile_recs(hipStream_t s, const Map &m, Tiles &t, const float2 *binfo);
+int launch_quad_recs(hipStream_t s, const Map &m, Tiles &t, const float2 *binfo);
 // Writes each staged entry's pixel dark (dint, 0 without a dark) into the
 // high half of its first word (2-byte batch elements).
 int launch_ents_dark(hipStream_t s, const Map &m, Tiles &t);

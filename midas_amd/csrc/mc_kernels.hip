@@ -280,46 +280,53 @@ __global__ __launch_bounds__(256) void k_profile(
 // Batched path: two kernels per pass of up to 64 frames.
 //
 // k_transpose turns the pass's frames [F][npx] into the pixel-major batch
-// Xt[npx][64] (frame fastest, frames past F zero): 128 bytes per pixel for
-// 8/16-bit frames (kept as uint16), 256 bytes for the other types (FP32,
-// exact for integer counts below 2^24).
+// Xt[npx][W] (W = 16, 32 or 64 frames, frame fastest, frames past F zero):
+// 2W bytes per pixel for 8/16-bit frames (kept as uint16), 4W bytes for
+// the other types (FP32, exact for integer counts below 2^24).
 //
 // k_batched cuts the cake rows into bands of 16*G rows and a band into
-// tiles: rectangles of rows x eta columns whose pixel bounding box fits the
+// tiles: rectangles of rows x eta columns whose pixel footprint fits the
 // LDS budget.  A workgroup of 16 waves takes one chunk -- a run of
 // consecutive tiles of one band -- and for each tile stages into LDS
-//   * the tile's bin records (cake bin, entry range, scale/dark, area),
-//   * its map entries (byte offset of the pixel row in the box, weight),
-//     each bin's entries padded to groups of 8 with zero-weight entries on
-//     a zero pixel row,
-//   * the box of the batch: box row after box row, each pixel's 64 frames
-//     contiguous -- whole 128-byte lines of Xt, so staging reads only what
-//     it keeps.
+//   * its map entries (byte offset of the pixel row in the footprint,
+//     weight),
+//   * the footprint of the batch: each pixel's W frames contiguous --
+//     whole lines of Xt, so staging reads only what it keeps.
 // Staging uses global->LDS copies (no registers), so a workgroup needs at
 // most 64 VGPRs per lane and two workgroups share a CU: one stages its
-// next tile while the other reduces.  In the reduction the four 16-lane groups of a wave take the
-// entries of a bin in turn (group g the entries 2g, 2g+1 of each group of
-// 8) and lane l of a group holds frames 4l..4l+3: one 8-byte LDS read
-// serves 4 frames of one entry.  The staged entries of a band row are one
-// contiguous run (records are row-major), so the entries of the next
-// group of 8 are read while the current one is reduced, across bin
-// boundaries.  The groups' partial sums meet by a lane-exchange
-// reduce-scatter at the end of the bin, after which lane (g, l) holds the
-// total of frame 4l+g.
+// next tile while the other reduces.
+//
+// The bins of a band row are reduced in quads: four consecutive cake bins,
+// the first a multiple of 4 (slots outside the tile's columns of the row
+// hold no bin).  The four 16-lane groups of a wave take the four bins, one
+// each, and lane l of a group holds frames l*W/16 .. (l+1)*W/16 - 1: one
+// LDS read serves W/16 frames of one entry, and no lane exchange is needed
+// to finish a bin.  A quad's staged entries are interleaved: pair m of
+// slot s at entries 8m + 2s, 8m + 2s + 1 (pairs past a bin's end are
+// zero-weight entries on a zero pixel row), so each iteration reads one
+// 16-byte pair per group; the staged entries of a row are one run, quad
+// after quad, and the next pair is read an iteration ahead across quads.
+// The quad records (bins, flags, entry run, scale/dark/area of the four
+// bins) are uniform over the wave and come in through scalar loads.
+// After a quad, two lane-half swaps and two 16-lane swaps transpose the
+// groups' 4 x 4 blocks (64-frame passes): lane (g, l) then holds frame
+// 4l + g of the four bins, so every lane applies the bins' scale and dark
+// term from scalar registers, and the four values fill half of an 8-bin
+// granule of its frame in registers, stored whole (32 bytes).  Narrower
+// passes store the values where they are (8 or 4 bytes).
 // Wave w owns the band rows j = w (mod 16), so it keeps the lineout sums
 // of its G rows in registers across the chunk; k_lineout_finalize adds the
-// chunks of each band.  Cake values are stored per frame in whole 8-bin
-// granules (32-byte stores).  Bins whose own box exceeds the budget ("direct" bins) gather
-// from Xt in global memory.
+// chunks of each band.  Bins whose own footprint exceeds the budget
+// ("direct" bins) gather from Xt in global memory.
 //   chunk: tile_begin tile_end band_row0 band_rows
-//   tile : y0 z0 bw bh b_beg b_end e_beg e_end ra nc vpr magic nrows 0 0 0
-//          (box of bw x bh pixels at (y0, z0), band rows ra .. ra + nrows - 1,
-//          nc columns, records row-major; vpr = 16-byte staging vectors per
-//          box row, magic = the multiplier of the division by vpr)
-//   rec  : cake bin, ent_begin, ent_end, direct flag, scale, dark term,
-//          in-bounds area
-//   ent  : (byte offset of the pixel row in the box, or global pixel if
-//          direct), weight
+//   tile : px_beg npx 0 0 q_beg nq e_beg e_end ra nc 0 0 nrows 0 0 0
+//          (footprint of npx pixels listed at px_beg, band rows ra .. ra +
+//          nrows - 1, columns of nc eta bins, nq quads per row: the quads
+//          of row j at q_beg + j * nq, rows padded with empty quads)
+//   quad : QuadRec (below); qdir[4 quad + s] = the direct entry range of
+//          slot s
+//   ent  : (byte offset of the pixel row in the footprint, or global pixel
+//          if direct), weight
 // Workgroup b takes chunk xcd[b % 8] + b / 8: hardware hands consecutive
 // workgroups to the 8 XCDs in turn, so each XCD works through one
 // contiguous range of bands of equal estimated cost, and neighbouring bands
@@ -329,15 +336,20 @@ constexpr int kWaves = 16;
 constexpr int kTileInts = 16;
 constexpr int kXtFrames = 64;     // frames per pixel row of the batch
 
-// A bin's record in a tile: {cake bin, entry begin, entry end, direct},
-// {scale, dark term}, and for the lineouts the bin's in-bounds area where
-// it counts (above 1e-9; 0 otherwise) and 1/0 for the same condition.
-struct BinRec {
-  int4 tb;
-  float2 info;
-  float a;
-  float af;
+// Four consecutive cake bins of one band row, the first a multiple of 4.
+// Slots that hold no bin of the row's columns in the tile get scale, dark
+// term and areas 0 (they add nothing and are not stored).
+struct QuadRec {
+  int32_t b0;     // cake bin of slot 0
+  int32_t flags;  // bits 0-3: slot s holds a bin; bits 4-7: its entries are direct
+  int32_t e_off;  // first staged entry of the quad, from the tile's first
+  int32_t n_it;   // staged entry pairs of each slot
+  float scale[4];
+  float dterm[4];
+  float a[4];   // in-bounds area where above 1e-9, else 0
+  float af[4];  // 1 where the in-bounds area is above 1e-9, else 0
 };
+static_assert(sizeof(QuadRec) == kQuadRecBytes, "quad record size");
 
 typedef int i4v __attribute__((ext_vector_type(4)));
 typedef unsigned u2v __attribute__((ext_vector_type(2)));
@@ -568,7 +580,7 @@ __device__ __forceinline__ void flush_granule(const VT &v, int idx, uint32_t mas
 }
 
 struct TileHdr {
-  int px_beg, npx, b_beg, nbt, e_beg, net, ra, nc, nrows;
+  int px_beg, npx, q_beg, nq, e_beg, net, ra, nrows;
 };
 
 __device__ __forceinline__ TileHdr tile_hdr(const int32_t *__restrict__ tiles, int t) {
@@ -576,12 +588,11 @@ __device__ __forceinline__ TileHdr tile_hdr(const int32_t *__restrict__ tiles, i
   TileHdr h;
   h.px_beg = T[0];
   h.npx = T[1];
-  h.b_beg = T[4];
-  h.nbt = T[5] - T[4];
+  h.q_beg = T[4];
+  h.nq = T[5];
   h.e_beg = T[6];
   h.net = T[7] - T[6];
   h.ra = T[8];
-  h.nc = T[9];
   h.nrows = T[12];
   return h;
 }
@@ -604,15 +615,13 @@ __device__ __forceinline__ void stage_pixels(const int32_t *__restrict__ tpix, i
 // Stages one tile into LDS with gfx950 global->LDS copies (16 bytes per
 // lane; a wave fills 1 KB of consecutive LDS, so the copies need no
 // registers): the rows of the tile's pixels (its footprint, listed in LDS
-// at ltp), the bin records (at lds), the entries (at lent; a tile's entries
-// start at a multiple of 8), and the zero pixel row after the staged rows
-// (plain stores; the padding entries point at it).  The caller waits for
-// the copies (s_waitcnt) and then for every wave (barrier).
+// at ltp), the entries (at lent; a tile's entries start at a multiple of
+// 8), and the zero pixel row after the staged rows (plain stores; the
+// padding entries point at it).  The caller waits for the copies
+// (s_waitcnt) and then for every wave (barrier).
 template <typename TL, int XF>
-__device__ __forceinline__ void tile_stage(const TileHdr &h, int wave, int lane, char *lds,
-                                           char *lent, char *plane,
-                                           const BinRec *__restrict__ recs,
-                                           const uint2 *__restrict__ ents,
+__device__ __forceinline__ void tile_stage(const TileHdr &h, int wave, int lane, char *lent,
+                                           char *plane, const uint2 *__restrict__ ents,
                                            const TL *__restrict__ xt,
                                            const int32_t *__restrict__ ltp) {
   constexpr int RB = XF * (int)sizeof(TL);  // bytes per pixel row
@@ -627,15 +636,8 @@ __device__ __forceinline__ void tile_stage(const TileHdr &h, int wave, int lane,
       __builtin_amdgcn_global_load_lds(src, (lds_void *)(plane + ch * 1024), 16, 0, 0);
     }
   }
-  const int nrv = 2 * h.nbt;  // 16-byte record vectors
-  for (int ch = wave; ch * 64 < nrv; ch += kWaves) {
-    const int j = ch * 64 + lane;
-    if (j < nrv)
-      __builtin_amdgcn_global_load_lds(reinterpret_cast<const u4v *>(recs + h.b_beg) + j,
-                                       (lds_void *)(lds + ch * 1024), 16, 0, 0);
-  }
   const int nev = h.net >> 1;  // 16-byte entry vectors
-  for (int ch = wave; ch * 64 < nev; ch += kWaves) {
+  for (int ch = kWaves - 1 - wave; ch * 64 < nev; ch += kWaves) {
     const int j = ch * 64 + lane;
     if (j < nev)
       __builtin_amdgcn_global_load_lds(reinterpret_cast<const u4v *>(ents + h.e_beg) + j,
@@ -643,23 +645,6 @@ __device__ __forceinline__ void tile_stage(const TileHdr &h, int wave, int lane,
   }
   if (wave == kWaves - 1 && lane < VPP)
     reinterpret_cast<u4v *>(plane + (size_t)h.npx * RB)[lane] = 0u;
-}
-
-// Reduce-scatter over the four 16-lane groups of a wave: t[j] holds a
-// group's partial sum for frame 4l+j (l = lane in the group); the result in
-// group g is the total over the groups of frame 4l+g.  Two gfx950 lane-half
-// swaps pair groups (0,2), (1,3), one 16-lane swap pairs (0,1), (2,3).
-__device__ __forceinline__ float group_scatter(const float t[4]) {
-  const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(t[0]), __float_as_uint(t[2]),
-                                                  false, false);
-  const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(t[1]), __float_as_uint(t[3]),
-                                                  false, false);
-  // groups 0,1: frames +0 / +1; groups 2,3: frames +2 / +3
-  const float s02 = __uint_as_float(a[0]) + __uint_as_float(a[1]);
-  const float s13 = __uint_as_float(b[0]) + __uint_as_float(b[1]);
-  const auto c = __builtin_amdgcn_permlane16_swap(__float_as_uint(s02), __float_as_uint(s13),
-                                                  false, false);
-  return __uint_as_float(c[0]) + __uint_as_float(c[1]);
 }
 
 // Sum over the four 16-lane groups, every group receiving the total (lane l
@@ -674,6 +659,13 @@ __device__ __forceinline__ float group_allreduce(float v) {
   return __uint_as_float(b[0]) + __uint_as_float(b[1]);
 }
 
+// Sum of lanes l and l + 32 (groups g and g ^ 2), both receiving it.
+__device__ __forceinline__ float half_sum(float v) {
+  const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false,
+                                                  false);
+  return __uint_as_float(a[0]) + __uint_as_float(a[1]);
+}
+
 // Compensated sum: products are added with plain FMAs over blocks of up to
 // 32 entries; a bin with more entries adds its blocks through a Kahan
 // accumulator (relative error about 32 ulp of a block plus 2 ulp,
@@ -685,6 +677,17 @@ __device__ __forceinline__ void kahan_add(Kahan &k, float x) {
   k.s = t;
 }
 
+// Row sums of the three lineouts: masked (NaN) values add nothing, zeros
+// add nothing to the sums (only the count skips them); a and af are 0
+// where the in-bounds area is not above 1e-9, and for empty slots.
+__device__ __forceinline__ void lineout_add(float lo[4], float o, float a, float af) {
+  const float on = isnan(o) ? 0.f : o;
+  lo[0] += on;
+  lo[1] += on != 0.f ? 1.f : 0.f;
+  lo[2] = fmaf(on, a, lo[2]);
+  lo[3] = fmaf(on, af, lo[3]);
+}
+
 // Two workgroups share a CU (8 waves per SIMD, at most 64 VGPRs and half
 // the LDS each): while one waits for its tile's copies, the other reduces.
 //
@@ -693,39 +696,34 @@ __device__ __forceinline__ void kahan_add(Kahan &k, float x) {
 // in the high half (k_ents_dark); each value is converted and the dark
 // subtracted before the product, so the sums are of dark-subtracted counts.
 // Direct bins read the dark from dint.
-//
-// A bin's record fields that steer control flow (cake bin, entry range,
-// direct flag) are read once into scalar registers: the granule bookkeeping
-// and the entry loop bounds are then scalar work, and the granule slot is a
-// uniform index.
 template <typename TL, int G, bool DARK, int FPL>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batched(
     const int32_t *__restrict__ chunks, const int32_t *__restrict__ xcd,
-    const int32_t *__restrict__ tiles, const BinRec *__restrict__ recs,
-    const uint2 *__restrict__ ents, const int32_t *__restrict__ tpix, const TL *__restrict__ xt,
-    const uint16_t *__restrict__ dint, int n_frames, int ent_off, int plane_off, int tp_off,
-    int tp_len, int64_t nb, float *__restrict__ cakes, float *__restrict__ part) {
+    const int32_t *__restrict__ tiles, const QuadRec *__restrict__ quads,
+    const int2 *__restrict__ qdir, const uint2 *__restrict__ ents,
+    const int32_t *__restrict__ tpix, const TL *__restrict__ xt,
+    const uint16_t *__restrict__ dint, int n_frames, int plane_off, int tp_off, int tp_len,
+    int64_t nb, float *__restrict__ cakes, float *__restrict__ part) {
   extern __shared__ __attribute__((aligned(16))) char lds_raw[];
   typedef typename Lanes<TL, FPL>::type QT;
   constexpr int XF = 16 * FPL;                // frames per pass (pixel row)
   constexpr int QB = FPL * (int)sizeof(TL);   // bytes of one lane's frames
-  constexpr int RB = XF * (int)sizeof(TL);    // bytes of a staged pixel row
   constexpr int RBAND = kWaves * G;
   constexpr int LG = 8;  // cake bins per granule: 32-byte stores
   typedef typename GranVec<LG>::type VT;
-  BinRec *lbin = reinterpret_cast<BinRec *>(lds_raw);
-  // LDS: the tile's bin records, its entries at ent_off, its pixel rows at
-  // plane_off (offsets of the layout's tile shape), the two pixel lists at
-  // tp_off.
-  uint2 *lent = reinterpret_cast<uint2 *>(lds_raw + ent_off);
+  // LDS: the tile's entries at 0, its pixel rows at plane_off, the two
+  // pixel lists at tp_off.
+  const uint2 *lent = reinterpret_cast<const uint2 *>(lds_raw);
   char *plane = lds_raw + plane_off;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int grp = lane >> 4, l16 = lane & 15;
-  // The frame this lane keeps after the groups' sums meet: 64-frame passes
-  // spread 64 frames over the 64 lanes; narrower passes give each frame to
-  // one lane of the first 4 / FPL groups (the other groups hold copies).
+  // The frame of this lane's values after a quad: 64-frame passes spread
+  // the 64 frames over the 64 lanes; 32-frame passes give lanes (g, l)
+  // frame 2l + (g & 1), 16-frame passes frame l.
   const int frame = FPL == 4 ? 4 * l16 + grp : FPL == 2 ? 2 * l16 + (grp & 1) : l16;
+  // Lanes whose lineout sums are written (the others hold partial sums of
+  // the same frames, added in first).
   const bool keeper = FPL == 4 || grp < 4 / FPL;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int x = blockIdx.x & 7;
@@ -746,8 +744,9 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
     for (int k = 0; k < 4; ++k) lo[g][k] = 0.f;
   }
   const bool vec_ok = ((nb & 3) == 0) && (((uintptr_t)cakes & 15) == 0);
-  const bool live = keeper && frame < n_frames;
-  const char *lpl = plane + l16 * QB;  // this lane's 4 frames in a pixel row
+  const bool vec2_ok = ((nb & 1) == 0) && (((uintptr_t)cakes & 7) == 0);
+  const bool live = frame < n_frames;
+  const char *lpl = plane + l16 * QB;  // this lane's frames in a pixel row
 
   // Pixel lists of tiles t (buffer (t - t_beg) & 1) and t + 1 (the other):
   // a tile's list is in LDS before its staging starts.
@@ -764,90 +763,71 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
     const TileHdr h = hn;
     const int cur = (t - t_beg) & 1;
     if (t > t_beg) __syncthreads();  // every wave is done with the previous tile's LDS
-    tile_stage<TL, XF>(h, wave, lane, lds_raw, reinterpret_cast<char *>(lent), plane, recs,
-                       ents, xt, reinterpret_cast<const int32_t *>(lds_raw + tp_off + cur * tp_len));
+    tile_stage<TL, XF>(h, wave, lane, lds_raw, plane, ents, xt,
+                       reinterpret_cast<const int32_t *>(lds_raw + tp_off + cur * tp_len));
     if (t + 1 < t_end) {
       hn = tile_hdr(tiles, t + 1);
       stage_pixels(tpix, hn.px_beg, hn.npx, wave, lane, lds_raw + tp_off + (cur ^ 1) * tp_len);
     }
     __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
     __syncthreads();
-    const int nrows = h.nrows;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      const int jr = kWaves * g + wave - h.ra;  // record row inside the tile
-      if (jr < 0 || jr >= nrows) continue;
-      const BinRec *rrow = lbin + jr * h.nc;
-      BinRec rn = rrow[0];
-      u4v tw = 0u;  // entries of the row's next group of 8 (read ahead)
-      bool primed = false;
-      for (int e = 0; e < h.nc; ++e) {
-        const BinRec r = rn;
-        if (e + 1 < h.nc) rn = rrow[e + 1];  // next record early: hides its LDS latency
-        const int rbin = __builtin_amdgcn_readfirstlane(r.tb.x);
-        const int rbeg = __builtin_amdgcn_readfirstlane(r.tb.y);
-        const int rend = __builtin_amdgcn_readfirstlane(r.tb.z);
-        const int rdir = __builtin_amdgcn_readfirstlane(r.tb.w);
-        float tot[FPL];
-        if (!rdir) {
-          int k = rbeg - h.e_beg;
-          // A bin's staged entries are padded to a multiple of 4: blocks of
-          // 8 (two per group), the last one possibly half (groups 0 and 1).
-          const int nent = rend - rbeg;
-          const int nblk = (nent + 7) >> 3;
-          const bool tail = (nent & 4) != 0;
-          if (!primed) {
-            tw = *reinterpret_cast<const u4v *>(lent + k + 2 * grp);
-            primed = true;
-          }
+      const int jr = kWaves * g + wave - h.ra;  // row inside the tile
+      if (jr < 0 || jr >= h.nrows) continue;
+      const int q_row = h.q_beg + jr * h.nq;
+      // The pair this group reads next (the row's entries are one run).
+      u4v tw = *reinterpret_cast<const u4v *>(lent + quads[q_row].e_off + 2 * grp);
+      for (int qi = 0; qi < h.nq; ++qi) {
+        const QuadRec &Q = quads[q_row + qi];
+        const int flags = Q.flags;
+        if ((flags & 15) == 0) continue;  // row padding
+        const int nit = Q.n_it;
+        int k = Q.e_off;
+        float tot[FPL];  // this group's bin, this lane's frames
+        {
           float gs[FPL];
 #pragma unroll
           for (int j = 0; j < FPL; ++j) gs[j] = 0.f;
-          Kahan acc[FPL];  // set at the fourth group of 8, used only past it
-          for (int blk = 0; blk < nblk; ++blk) {
-            u4v two = tw;
-            const bool half = tail && blk == nblk - 1;
-            if (half && grp >= 2) {
-              // the other half belongs to the next bin: the zero row, weight 0
-              two.x = two.z = (uint32_t)(h.npx * RB);
-              two.y = two.w = 0u;
-            }
-            const uint32_t oa = DARK ? (two.x & 0xffffu) : two.x;
-            const uint32_t obb = DARK ? (two.z & 0xffffu) : two.z;
-            const QT da = *reinterpret_cast<const QT *>(lpl + oa);
-            const QT db = *reinterpret_cast<const QT *>(lpl + obb);
-            k += half ? 4 : 8;
-            // The run's next group (past the run's end this reads unused
-            // LDS words inside the allocation).
-            tw = *reinterpret_cast<const u4v *>(lent + k + 2 * grp);
-            float fa[FPL], fb[FPL];
-            Lanes<TL, FPL>::unpack(da, fa);
-            Lanes<TL, FPL>::unpack(db, fb);
-            if constexpr (DARK) {
-              const float dka = (float)(two.x >> 16), dkb = (float)(two.z >> 16);
+          // Blocks of 16 pairs (32 entries) of plain FMAs; a bin with more
+          // adds its blocks through a Kahan accumulator.
+          Kahan acc[FPL];
+          int m = 0;
+          for (;;) {
+            const int m_end = min(nit, m + 16);
+            for (; m < m_end; ++m) {
+              const u4v two = tw;
+              k += 8;
+              // the next pair (past the run's end this reads LDS words
+              // inside the allocation that are not used)
+              tw = *reinterpret_cast<const u4v *>(lent + k + 2 * grp);
+              const uint32_t oa = DARK ? (two.x & 0xffffu) : two.x;
+              const uint32_t obb = DARK ? (two.z & 0xffffu) : two.z;
+              const QT da = *reinterpret_cast<const QT *>(lpl + oa);
+              const QT db = *reinterpret_cast<const QT *>(lpl + obb);
+              float fa[FPL], fb[FPL];
+              Lanes<TL, FPL>::unpack(da, fa);
+              Lanes<TL, FPL>::unpack(db, fb);
+              if constexpr (DARK) {
+                const float dka = (float)(two.x >> 16), dkb = (float)(two.z >> 16);
 #pragma unroll
-              for (int j = 0; j < FPL; ++j) {
-                fa[j] -= dka;
-                fb[j] -= dkb;
-              }
-            }
-            const float wa = __uint_as_float(two.y), wb = __uint_as_float(two.w);
-#pragma unroll
-            for (int j = 0; j < FPL; ++j) gs[j] = fmaf(fb[j], wb, fmaf(fa[j], wa, gs[j]));
-            if ((blk & 3) == 3 && blk + 1 < nblk) {  // more than 32 entries: compensate
-#pragma unroll
-              for (int j = 0; j < FPL; ++j) {
-                if (blk == 3) {
-                  acc[j].s = gs[j];
-                  acc[j].c = 0.f;
-                } else {
-                  kahan_add(acc[j], gs[j]);
+                for (int j = 0; j < FPL; ++j) {
+                  fa[j] -= dka;
+                  fb[j] -= dkb;
                 }
-                gs[j] = 0.f;
               }
+              const float wa = __uint_as_float(two.y), wb = __uint_as_float(two.w);
+#pragma unroll
+              for (int j = 0; j < FPL; ++j) gs[j] = fmaf(fb[j], wb, fmaf(fa[j], wa, gs[j]));
+            }
+            if (m >= nit) break;
+#pragma unroll
+            for (int j = 0; j < FPL; ++j) {
+              kahan_add(acc[j], gs[j]);
+              gs[j] = 0.f;
             }
           }
-          if (nblk > 4) {
+          if (nit > 16) {
 #pragma unroll
             for (int j = 0; j < FPL; ++j) {
               kahan_add(acc[j], gs[j]);
@@ -857,64 +837,135 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
 #pragma unroll
             for (int j = 0; j < FPL; ++j) tot[j] = gs[j];
           }
-        } else {
-          Kahan acc[FPL];
-          // Entries stay in global memory: group g takes entries g, g+4, ...
-          for (int k = rbeg + grp; k < rend; k += 4) {
-            const uint2 en = ents[k];
-            const QT d = *reinterpret_cast<const QT *>(
-                reinterpret_cast<const char *>(xt + (int64_t)en.x * XF) + l16 * QB);
-            float fv[FPL];
-            Lanes<TL, FPL>::unpack(d, fv);
-            if constexpr (DARK) {
-              const float dk = (float)dint[en.x];
+        }
+        if (flags & 0xf0) {
+          // Direct slots: the slot's group gathers its entries from the
+          // batch in global memory.
+          if ((flags >> (4 + grp)) & 1) {
+            const int2 dr = qdir[4 * (q_row + qi) + grp];
+            Kahan acc[FPL];
 #pragma unroll
-              for (int j = 0; j < FPL; ++j) fv[j] -= dk;
+            for (int j = 0; j < FPL; ++j) acc[j].s = tot[j];
+            for (int e = dr.x; e < dr.y; ++e) {
+              const uint2 en = ents[e];
+              const QT d = *reinterpret_cast<const QT *>(
+                  reinterpret_cast<const char *>(xt + (int64_t)en.x * XF) + l16 * QB);
+              float fv[FPL];
+              Lanes<TL, FPL>::unpack(d, fv);
+              if constexpr (DARK) {
+                const float dk = (float)dint[en.x];
+#pragma unroll
+                for (int j = 0; j < FPL; ++j) fv[j] -= dk;
+              }
+#pragma unroll
+              for (int j = 0; j < FPL; ++j) kahan_add(acc[j], fv[j] * __uint_as_float(en.y));
             }
 #pragma unroll
-            for (int j = 0; j < FPL; ++j) kahan_add(acc[j], fv[j] * __uint_as_float(en.y));
+            for (int j = 0; j < FPL; ++j) tot[j] = acc[j].value();
+          }
+        }
+        const uint32_t pm = (uint32_t)flags & 15u;
+        if constexpr (FPL == 4) {
+          // 4 x 4 transpose over the groups: lane (g, l) gets frame 4l + g
+          // of slots 0..3.
+          const auto p0 = __builtin_amdgcn_permlane32_swap(__float_as_uint(tot[0]),
+                                                           __float_as_uint(tot[2]), false, false);
+          const auto p1 = __builtin_amdgcn_permlane32_swap(__float_as_uint(tot[1]),
+                                                           __float_as_uint(tot[3]), false, false);
+          const auto c0 = __builtin_amdgcn_permlane16_swap(p0[0], p1[0], false, false);
+          const auto c1 = __builtin_amdgcn_permlane16_swap(p0[1], p1[1], false, false);
+          float o[4] = {__uint_as_float(c0[0]), __uint_as_float(c0[1]), __uint_as_float(c1[0]),
+                        __uint_as_float(c1[1])};
+#pragma unroll
+          for (int s = 0; s < 4; ++s) o[s] = (o[s] - Q.dterm[s]) * Q.scale[s];  // NaN: masked
+          const int gr = Q.b0 / LG;
+          const bool hi = (Q.b0 & 4) != 0;
+          if (gr != og[g]) {
+            if (om[g] && live) flush_granule<LG>(ob[g], og[g], om[g], frame, cakes, nb, vec_ok);
+            og[g] = gr;
+            om[g] = 0u;
+          }
+          om[g] |= hi ? pm << 4 : pm;
+          if (pm == 15u) {
+            if (hi) {
+              ob[g][4] = o[0];
+              ob[g][5] = o[1];
+              ob[g][6] = o[2];
+              ob[g][7] = o[3];
+            } else {
+              ob[g][0] = o[0];
+              ob[g][1] = o[1];
+              ob[g][2] = o[2];
+              ob[g][3] = o[3];
+            }
+          } else {
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+              if ((pm >> s) & 1u) {
+                if (hi) ob[g][4 + s] = o[s];
+                else ob[g][s] = o[s];
+              }
           }
 #pragma unroll
-          for (int j = 0; j < FPL; ++j) tot[j] = acc[j].value();
-        }
-        // The four groups hold partial sums of the same frames.
-        float mine;
-        if constexpr (FPL == 4) {
-          mine = group_scatter(tot);
+          for (int s = 0; s < 4; ++s) lineout_add(lo[g], o[s], Q.a[s], Q.af[s]);
         } else if constexpr (FPL == 2) {
-          const float s0 = group_allreduce(tot[0]), s1 = group_allreduce(tot[1]);
-          mine = (grp & 1) ? s1 : s0;
+          // One 16-lane swap: lane (g, l) gets frame 2l + (g & 1) of slots
+          // 2 (g >> 1) and 2 (g >> 1) + 1.
+          const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(tot[0]),
+                                                          __float_as_uint(tot[1]), false, false);
+          const bool up = grp >= 2;
+          const int s0 = up ? 2 : 0;
+          const float o0 = (__uint_as_float(p[0]) - (up ? Q.dterm[2] : Q.dterm[0])) *
+                           (up ? Q.scale[2] : Q.scale[0]);
+          const float o1 = (__uint_as_float(p[1]) - (up ? Q.dterm[3] : Q.dterm[1])) *
+                           (up ? Q.scale[3] : Q.scale[1]);
+          if (live) {
+            const uint32_t pb = (pm >> s0) & 3u;
+            float *dst = cakes + (int64_t)frame * nb + Q.b0 + s0;
+            if (pb == 3u && vec2_ok) {
+              *reinterpret_cast<float2 *>(dst) = make_float2(o0, o1);
+            } else {
+              if (pb & 1u) dst[0] = o0;
+              if (pb & 2u) dst[1] = o1;
+            }
+          }
+          lineout_add(lo[g], o0, up ? Q.a[2] : Q.a[0], up ? Q.af[2] : Q.af[0]);
+          lineout_add(lo[g], o1, up ? Q.a[3] : Q.a[1], up ? Q.af[3] : Q.af[1]);
         } else {
-          mine = group_allreduce(tot[0]);
+          // Lane (g, l) holds frame l of slot g.
+          const float sc = grp == 0 ? Q.scale[0] : grp == 1 ? Q.scale[1]
+                         : grp == 2 ? Q.scale[2] : Q.scale[3];
+          const float dt = grp == 0 ? Q.dterm[0] : grp == 1 ? Q.dterm[1]
+                         : grp == 2 ? Q.dterm[2] : Q.dterm[3];
+          const float aa = grp == 0 ? Q.a[0] : grp == 1 ? Q.a[1] : grp == 2 ? Q.a[2] : Q.a[3];
+          const float af = grp == 0 ? Q.af[0] : grp == 1 ? Q.af[1]
+                         : grp == 2 ? Q.af[2] : Q.af[3];
+          const float o = (tot[0] - dt) * sc;
+          if (live && ((pm >> grp) & 1u)) cakes[(int64_t)frame * nb + Q.b0 + grp] = o;
+          lineout_add(lo[g], o, aa, af);
         }
-        const int gr = rbin / LG, gk = rbin - gr * LG;
-        if (gr != og[g]) {
-          if (om[g] && live) flush_granule<LG>(ob[g], og[g], om[g], frame, cakes, nb, vec_ok);
-          og[g] = gr;
-          om[g] = 0u;
-        }
-        om[g] |= 1u << gk;
-        const float o = (mine - r.info.y) * r.info.x;  // NaN scale: a masked bin
-        ob[g][gk] = o;
-        // lineout sums without branches: masked bins add nothing, zeros
-        // add nothing to the sums (only the count skips them), r.a and
-        // r.af are 0 where the in-bounds area is not above 1e-9
-        const float on = isnan(o) ? 0.f : o;
-        lo[g][0] += on;
-        lo[g][1] += on != 0.f ? 1.f : 0.f;
-        lo[g][2] = fmaf(on, r.a, lo[g][2]);
-        lo[g][3] = fmaf(on, r.af, lo[g][3]);
       }
     }
   }
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    if (om[g] && live) flush_granule<LG>(ob[g], og[g], om[g], frame, cakes, nb, vec_ok);
-    if (want_lo && keeper) {
-      // part[c][row][k][XF] for the band row this slot holds.
-      float *Pp = part + ((int64_t)c * RBAND + kWaves * g + wave) * 4 * XF;
+    if constexpr (FPL == 4) {
+      if (om[g] && live) flush_granule<LG>(ob[g], og[g], om[g], frame, cakes, nb, vec_ok);
+    }
+    if (want_lo) {
+      float v[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) Pp[k * XF + frame] = lo[g][k];
+      for (int k = 0; k < 4; ++k) {
+        v[k] = lo[g][k];
+        if constexpr (FPL == 2) v[k] = half_sum(v[k]);
+        if constexpr (FPL == 1) v[k] = group_allreduce(v[k]);
+      }
+      if (keeper) {
+        // part[c][row][k][XF] for the band row this slot holds.
+        float *Pp = part + ((int64_t)c * RBAND + kWaves * g + wave) * 4 * XF;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) Pp[k * XF + frame] = v[k];
+      }
     }
   }
 }
@@ -939,18 +990,33 @@ __global__ __launch_bounds__(256) void k_ents_dark(const int32_t *__restrict__ t
   }
 }
 
-// Tile bin records with the per-bin scale/dark and in-bounds area of the
-// current (mode, flags, dark), so the kernel reads each record once.
-__global__ void k_tile_recs(const int4 *__restrict__ tbins, const float2 *__restrict__ binfo,
-                            const float *__restrict__ a_inb, int64_t n, BinRec *__restrict__ out) {
+// Quad records of the current (mode, flags, dark): the scale, dark term and
+// in-bounds area of each slot's bin (0 for empty slots), so the batched
+// kernel reads each quad once.  qs: cake bin of slot 0, flags, entry
+// offset, pairs.
+__global__ void k_quad_recs(const int4 *__restrict__ qs, const float2 *__restrict__ binfo,
+                            const float *__restrict__ a_inb, int64_t n,
+                            QuadRec *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  BinRec r;
-  r.tb = tbins[i];
-  r.info = binfo[r.tb.x];
-  const float a = a_inb[r.tb.x];
-  r.a = a > 1e-9f ? a : 0.f;
-  r.af = a > 1e-9f ? 1.f : 0.f;
+  const int4 q = qs[i];
+  QuadRec r;
+  r.b0 = q.x;
+  r.flags = q.y;
+  r.e_off = q.z;
+  r.n_it = q.w;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    r.scale[s] = r.dterm[s] = r.a[s] = r.af[s] = 0.f;
+    if ((q.y >> s) & 1) {
+      const float2 bi = binfo[q.x + s];
+      const float a = a_inb[q.x + s];
+      r.scale[s] = bi.x;
+      r.dterm[s] = bi.y;
+      r.a[s] = a > 1e-9f ? a : 0.f;
+      r.af[s] = a > 1e-9f ? 1.f : 0.f;
+    }
+  }
   out[i] = r;
 }
 
@@ -1259,14 +1325,15 @@ int launch_hybrid_decode(hipStream_t s, const void *msg, int64_t npx, int64_t co
   return MC_OK;
 }
 
-int launch_tile_recs(hipStream_t s, const Map &m, Tiles &t, const float2 *binfo) {
-  if (!t.recs.p) {
-    int rc = t.recs.alloc((size_t)t.n_tbins * sizeof(BinRec));
+int launch_quad_recs(hipStream_t s, const Map &m, Tiles &t, const float2 *binfo) {
+  if (!t.qrecs.p) {
+    int rc = t.qrecs.alloc((size_t)std::max<int64_t>(t.n_quads, 1) * sizeof(QuadRec));
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_tile_recs, dim3(blocks_for(t.n_tbins, 256)), dim3(256), 0, s,
-                     t.tbins.as<int4>(), binfo, m.d_area_inb.as<float>(), t.n_tbins,
-                     t.recs.as<BinRec>());
+  if (t.n_quads == 0) return MC_OK;
+  hipLaunchKernelGGL(k_quad_recs, dim3(blocks_for(t.n_quads, 256)), dim3(256), 0, s,
+                     t.qs.as<int4>(), binfo, m.d_area_inb.as<float>(), t.n_quads,
+                     t.qrecs.as<QuadRec>());
   MC_HIP(hipGetLastError());
   return MC_OK;
 }
@@ -1317,10 +1384,9 @@ static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mas
   if (!lineouts) part = nullptr;
   hipLaunchKernelGGL(kern, dim3((unsigned)t.grid), dim3(64 * kWaves), lds, s,
                      t.chunks.as<int32_t>(), t.xcd.as<int32_t>(), t.tiles.as<int32_t>(),
-                     t.recs.as<BinRec>(), t.ents.as<uint2>(), t.tpix.as<int32_t>(),
-                     (const TL *)xt, dhi, n_frames, t.bin_cap * (int)sizeof(BinRec),
-                     t.bin_cap * (int)sizeof(BinRec) + t.ent_cap * 8, t.tp_off, t.tp_len, m.nb(),
-                     cakes, part);
+                     t.qrecs.as<QuadRec>(), t.qdir.as<int2>(), t.ents.as<uint2>(),
+                     t.tpix.as<int32_t>(), (const TL *)xt, dhi, n_frames, t.plane_off, t.tp_off,
+                     t.tp_len, m.nb(), cakes, part);
   MC_HIP(hipGetLastError());
   // 3. lineouts
   if (lineouts) {
@@ -1336,7 +1402,7 @@ static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mas
 int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
                    int mask_nan, const void *frames, int dtype, int n_frames, float *cakes,
                    float *lineouts, void *xt, float *part) {
-  (void)binfo;  // folded into the tile records (launch_tile_recs)
+  (void)binfo;  // folded into the quad records (launch_quad_recs)
   if (n_frames < 1 || n_frames > t.width) {
     set_error("batched pass of %d frames outside 1..%d", n_frames, t.width);
     return MC_ERR_ARG;

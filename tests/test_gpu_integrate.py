@@ -94,7 +94,9 @@ def test_batched_tilings_agree(batch, rows):
         np.testing.assert_allclose(lb[i], l1[0], rtol=3e-6, atol=1e-4)
 
 
-@pytest.mark.parametrize("batch,tight_lds", [(8, 24576 + 40 * 48), (64, 24576 + 128 * 48)])
+# The budgets leave room for 2048 staged entries (16 KiB) and about 46
+# pixel rows of the pass width.
+@pytest.mark.parametrize("batch,tight_lds", [(8, 16384 + 40 * 48), (64, 16384 + 128 * 48)])
 def test_small_lds_budget_uses_direct_bins(batch, tight_lds):
     """Coarse bins under a small LDS budget: bins whose own pixel box does
     not fit become direct (global gathers) and columns split into row
