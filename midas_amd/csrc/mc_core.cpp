@@ -272,9 +272,11 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
   if (in_size == 2) cap_px = std::min<int64_t>(cap_px, 65535 / RB - 1);
   MC_CHECK(cap_px >= 16, MC_ERR_ARG, "LDS budget %d too small", m->lds_bytes);
   auto cnt = [&](int64_t b) { return (int64_t)(c.h_row[b + 1] - c.h_row[b]); };
-  // Staged entries of a quad: pairs of entries per slot up to its largest
-  // bin, 8 entries a pair (four slots).
+  // Staged entries of a quad: the slots holding staged entries ("staged
+  // slots") are interleaved a pair at a time, each padded to the pairs of
+  // the quad's largest bin; the other slots read the tile's zero pair.
   auto pairs = [](int64_t n) { return (n + 1) / 2; };
+  auto qents = [&](int64_t n, int64_t p) { return 2 * pairs(n) * p; };
   // Per-bin footprints: the sorted distinct pixels of its entries; a bin
   // whose footprint or entries do not fit a tile alone is direct.
   std::vector<std::vector<int32_t>> fp(nb);
@@ -285,7 +287,7 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
       v.assign(c.h_pix.begin() + c.h_row[q], c.h_pix.begin() + c.h_row[q + 1]);
       std::sort(v.begin(), v.end());
       v.erase(std::unique(v.begin(), v.end()), v.end());
-      big[q] = (int64_t)v.size() > cap_px || 8 * pairs(cnt(q)) > kTileEnts;
+      big[q] = (int64_t)v.size() > cap_px || qents(cnt(q), 1) + 2 > kTileEnts;
       if (big[q]) std::vector<int32_t>().swap(v);
     }
   });
@@ -307,9 +309,10 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
     std::set_union(x.begin(), x.end(), y.begin(), y.end(), std::back_inserter(u));
     x.swap(u);
   };
-  // Last quad of each row of a tile being grown and its largest bin.
+  // Last quad of each row of a tile being grown: its largest staged bin and
+  // its staged slots.
   struct RowQ {
-    int64_t q = -1, n = 0;
+    int64_t q = -1, n = 0, p = 0;
   };
   struct TileH {
     int ra = 0, nrows = 0, e0 = 0, nc = 0;
@@ -337,9 +340,10 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
     for (int64_t gi = a; gi < bq; gi++) {
       auto &tv = bands[gi];
       const int row0 = (int)gi * H, nrows = std::min(H, nR - row0);
+      // (a tile's staged entries end with its zero pair: 2 entries)
       auto room = [&](const TileH &T, const std::vector<int32_t> &add, int64_t ents,
                       int nbins) {
-        return (int64_t)T.nrows * T.nc + nbins <= kTileBins && T.ents + ents <= kTileEnts &&
+        return (int64_t)T.nrows * T.nc + nbins <= kTileBins && T.ents + ents + 2 <= kTileEnts &&
                union_size(T.px, add) <= cap_px;
       };
       // Staged entries that columns e .. e + n - 1 of all rows add to T
@@ -351,13 +355,17 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
         for (int j = 0; j < nrows; j++)
           for (int u = 0; u < n; u++) {
             const int64_t b = (int64_t)(row0 + j) * nE + e + u, q = b / 4, k = staged_cnt(b);
+            const int64_t st = k > 0 ? 1 : 0;
             if (rq[j].q == q) {
-              add += 8 * (pairs(std::max(rq[j].n, k)) - pairs(rq[j].n));
+              const int64_t was = qents(rq[j].n, rq[j].p);
               rq[j].n = std::max(rq[j].n, k);
+              rq[j].p += st;
+              add += qents(rq[j].n, rq[j].p) - was;
             } else {
-              add += 8 * pairs(k);
+              add += qents(k, st);
               rq[j].q = q;
               rq[j].n = k;
+              rq[j].p = st;
             }
           }
         if (commit) T.rq.swap(rq);
@@ -418,6 +426,45 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
         for (int j = 0; j < nrows; j++)
           for (int u = 0; u < n; u++) acc += cnt((int64_t)(row0 + j) * nE + e + u) + kBinCost;
         if (place(e, n)) continue;
+        // Four columns (whole quads) that do not fit a fresh tile: row
+        // ranges of the four columns, a quad per row, while a single row of
+        // them fits.
+        if (n == 4) {
+          std::vector<std::vector<int32_t>> radd(nrows);
+          std::vector<int64_t> rents(nrows);
+          bool fits = true;
+          for (int j = 0; j < nrows && fits; j++) {
+            int64_t kmax = 0, st = 0;
+            for (int u = 0; u < 4; u++) {
+              const int64_t b = (int64_t)(row0 + j) * nE + e + u;
+              if (!big[b]) merge_into(radd[j], fp[b]);
+              kmax = std::max(kmax, staged_cnt(b));
+              st += staged_cnt(b) > 0 ? 1 : 0;
+            }
+            rents[j] = qents(kmax, st);
+            fits = room(TileH(), radd[j], rents[j], 4);
+          }
+          if (fits) {
+            flush();
+            TileH piece;
+            for (int j = 0; j < nrows; j++) {
+              if (piece.nrows > 0 && !room(piece, radd[j], rents[j], 4)) {
+                push(std::move(piece));
+                piece = TileH();
+              }
+              if (piece.nrows == 0) {
+                piece.e0 = e;
+                piece.nc = 4;
+                piece.ra = j;
+              }
+              merge_into(piece.px, radd[j]);
+              piece.ents += rents[j];
+              piece.nrows++;
+            }
+            push(std::move(piece));
+            continue;
+          }
+        }
         // The columns do not fit a fresh tile together: one at a time.
         for (int e1 = e; e1 < e + n; e1++) {
           if (n > 1 && place(e1, 1)) continue;
@@ -431,7 +478,7 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
           for (int j = 0; j < nrows; j++) {
             const int64_t b = (int64_t)(row0 + j) * nE + e1;
             const std::vector<int32_t> &add = big[b] ? none : fp[b];
-            const int64_t ae = 8 * pairs(staged_cnt(b));
+            const int64_t ae = qents(staged_cnt(b), staged_cnt(b) > 0 ? 1 : 0);
             if (piece.nrows > 0 && !room(piece, add, ae, 1)) {
               push(std::move(piece));
               piece = TileH();
@@ -502,13 +549,15 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
             n_slots++;
             all_ents += cnt(b);
             if (big[b]) flags |= 16 << s;
+            if (staged_cnt(b) > 0) flags |= 256 << s;  // a staged slot
             np = std::max(np, pairs(staged_cnt(b)));
           }
           for (int64_t mi = 0; mi < np; mi++)
             for (int s = 0; s < 4; s++)
               for (int u = 0; u < 2; u++) {
+                if (!((flags >> (8 + s)) & 1)) continue;
                 const int64_t b = b0 + s, idx = 2 * mi + u;
-                if (((flags >> s) & 1) && !big[b] && idx < cnt(b)) {
+                if (idx < cnt(b)) {
                   const int32_t k = c.h_row[b] + (int32_t)idx;
                   const int p = c.h_pix[k];
                   // byte offset of the pixel's row in the tile's staged list
@@ -525,6 +574,11 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
           qs.push_back(make_int4((int32_t)b0, flags, e_off, (int32_t)np));
         }
       }
+      // The zero pair: the lanes of a quad's slots without staged entries
+      // read it (the zero row, weight 0).
+      const int32_t zp = (int32_t)(ents.size() - (size_t)e_beg);
+      ents.push_back(make_uint2(zero_row, 0u));
+      ents.push_back(make_uint2(zero_row, 0u));
       const int32_t e_end = (int32_t)ents.size();
       MC_CHECK(e_end - e_beg <= kTileEnts, MC_ERR_STATE, "tile stages %d entries over %d",
                e_end - e_beg, kTileEnts);
@@ -545,7 +599,7 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
         }
       // Every tile's entries start at a multiple of 8 (16-byte staging).
       while (ents.size() % 8) ents.push_back(make_uint2(0u, 0u));
-      tiles.insert(tiles.end(), {px_beg, npx, 0, 0, q_beg, nq, e_beg, e_end, T.ra, T.nc, 0, 0,
+      tiles.insert(tiles.end(), {px_beg, npx, zp, 0, q_beg, nq, e_beg, e_end, T.ra, T.nc, 0, 0,
                                  T.nrows, 0, 0, 0});
       chunks[chunks.size() - 3] = (int32_t)(tiles.size() / kTileInts);
       const int64_t tcost = all_ents + kTileCost + kBinCost * T.nrows * T.nc;

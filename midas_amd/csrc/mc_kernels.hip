@@ -341,9 +341,12 @@ constexpr int kXtFrames = 64;     // frames per pixel row of the batch
 // term and areas 0 (they add nothing and are not stored).
 struct QuadRec {
   int32_t b0;     // cake bin of slot 0
-  int32_t flags;  // bits 0-3: slot s holds a bin; bits 4-7: its entries are direct
+  int32_t flags;  // bits 0-3: slot s holds a bin; bits 4-7: its entries are direct;
+                  // bits 8-11: it has staged entries (a staged slot)
   int32_t e_off;  // first staged entry of the quad, from the tile's first
-  int32_t n_it;   // staged entry pairs of each slot
+  int32_t n_it;   // staged entry pairs of each staged slot (the staged slots'
+                  // pairs are interleaved: pair m of the r-th staged slot is
+                  // entries e_off + 2 (m P + r), P staged slots)
   float scale[4];
   float dterm[4];
   float a[4];   // in-bounds area where above 1e-9, else 0
@@ -580,14 +583,18 @@ __device__ __forceinline__ void flush_granule(const VT &v, int idx, uint32_t mas
 }
 
 struct TileHdr {
-  int px_beg, npx, q_beg, nq, e_beg, net, ra, nrows;
+  int px_beg, npx, zp, q_beg, nq, e_beg, net, ra, nrows;
 };
 
+// Tile record: {first listed pixel, pixels, zero pair (entry offset), 0,
+// first quad, quads per row, entry range, first row, columns, 0, 0, rows,
+// 0, 0, 0}.
 __device__ __forceinline__ TileHdr tile_hdr(const int32_t *__restrict__ tiles, int t) {
   const int32_t *T = tiles + kTileInts * t;
   TileHdr h;
   h.px_beg = T[0];
   h.npx = T[1];
+  h.zp = T[2];
   h.q_beg = T[4];
   h.nq = T[5];
   h.e_beg = T[6];
@@ -776,14 +783,24 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
       const int jr = kWaves * g + wave - h.ra;  // row inside the tile
       if (jr < 0 || jr >= h.nrows) continue;
       const int q_row = h.q_beg + jr * h.nq;
-      // The pair this group reads next (the row's entries are one run).
-      u4v tw = *reinterpret_cast<const u4v *>(lent + quads[q_row].e_off + 2 * grp);
+      // The pair this group reads next and its entry index: a quad whose
+      // slots all hold staged entries reads on where the previous full quad
+      // ended (the row's entries are one run), others read their first pair.
+      u4v tw;
+      int ra = -1;
       for (int qi = 0; qi < h.nq; ++qi) {
         const QuadRec &Q = quads[q_row + qi];
         const int flags = Q.flags;
         if ((flags & 15) == 0) continue;  // row padding
         const int nit = Q.n_it;
-        int k = Q.e_off;
+        // This group's first pair and the step between its pairs; a group
+        // whose slot has no staged entries reads the zero pair throughout.
+        const uint32_t sm = ((uint32_t)flags >> 8) & 15u;
+        const bool mine = (sm >> grp) & 1u;
+        const int rank = __builtin_popcount(sm & ((1u << grp) - 1u));
+        int ka = mine ? Q.e_off + 2 * rank : h.zp;  // in entries
+        const int kstep = mine ? 2 * __builtin_popcount(sm) : 0;
+        if (ka != ra) tw = *reinterpret_cast<const u4v *>(lent + ka);
         float tot[FPL];  // this group's bin, this lane's frames
         {
           float gs[FPL];
@@ -797,10 +814,10 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
             const int m_end = min(nit, m + 16);
             for (; m < m_end; ++m) {
               const u4v two = tw;
-              k += 8;
+              ka += kstep;
               // the next pair (past the run's end this reads LDS words
               // inside the allocation that are not used)
-              tw = *reinterpret_cast<const u4v *>(lent + k + 2 * grp);
+              tw = *reinterpret_cast<const u4v *>(lent + ka);
               const uint32_t oa = DARK ? (two.x & 0xffffu) : two.x;
               const uint32_t obb = DARK ? (two.z & 0xffffu) : two.z;
               const QT da = *reinterpret_cast<const QT *>(lpl + oa);
@@ -838,6 +855,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
             for (int j = 0; j < FPL; ++j) tot[j] = gs[j];
           }
         }
+        ra = ka;  // tw holds the pair at ka
         if (flags & 0xf0) {
           // Direct slots: the slot's group gathers its entries from the
           // batch in global memory.
