@@ -178,7 +178,11 @@ def test_profile_1d_matches_fused(oracle_lib):
     cakes, lo = ma.integrate_batch(frames, g, mode="bilinear", mask_nan=True)
     for mode, row in (("cpu_mean", 0), ("area_weighted", 1), ("simple_mean", 2)):
         p = ma.profile_1d(cakes, g, mode=mode)
-        np.testing.assert_allclose(p, lo[:, row], rtol=2e-5, atol=1e-4)
+        # the same reduction of |cake| bounds the rounding of either order
+        scale = ma.profile_1d(np.abs(cakes), g, mode=mode)
+        err = np.abs(np.asarray(p, np.float64) - lo[:, row]) / np.maximum(scale, 1e-30)
+        print(f"[profile] {mode}: worst {err.max():.2e}", flush=True)
+        assert err.max() <= 1e-5
 
 
 def test_torch_device_tensors(oracle_lib):
