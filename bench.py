@@ -190,10 +190,84 @@ def qmode_bench(log, steps=5, warmup=2):
                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                         "bytes_per_frame": bpf},
            "map_build_s": t_map, "layout_s": t_layout, "layout": layout}
+    out["serving"] = qmode_serving_bench(g, P, frames[:8].cpu().numpy(), log)
     g.close()
     del frames, cakes, lo
     torch.cuda.empty_cache()
     return out
+
+
+def qmode_serving_bench(g, P, frames_host, log, n_frames=2000, rate=400.0, n_rate=1000):
+    """The Q-mode workload streamed through the serving loop with fitting:
+    host frames (float32, 2880^2, 33 MB each, the given frames in turn)
+    submitted to a FrameServer, each frame's area-weighted lineout handed
+    to a StreamPeakFitter that fits the 16 rings (TCH, one launch per
+    queued group of frames).  Two runs: n_frames submitted back to back
+    (frames/s from the first submit to the last fit; the latency is then
+    queueing), and n_rate frames at Poisson arrivals of `rate` per second
+    (submit-to-result latency)."""
+    import tempfile
+
+    import numpy as np
+
+    import midas_amd as ma
+    from midas_amd import synth
+    from midas_amd.server import FrameServer
+    from midas_amd.stream_fits import StreamPeakFitter
+
+    RLo, RHi, _, _ = ma.bin_edges(P)
+    rc = (RLo + RHi) / 2
+    q = synth.ring_q()
+    rings = (P.Lsd / P.pxY) * np.tan(2.0 * np.arcsin(q * P.Wavelength / (4.0 * np.pi)))
+    rng = np.random.default_rng(5)
+
+    def run(n, pace):
+        with tempfile.TemporaryDirectory() as td:
+            fitter = StreamPeakFitter(rc, float(np.median(RHi - RLo)), rings,
+                                      fit_roi_padding=60, out_dir=td, max_batch=128)
+            with FrameServer(hbm_budget_bytes=128 << 30, max_batch=16, queue_depth=48,
+                             keep_results=False,
+                             on_result=lambda r: fitter.submit(r.request_id,
+                                                               r.lineouts[1])) as srv:
+                gid = srv.add_geometry(g, np.float32, mode="gradient")
+                for k in range(4):  # first launches of each kernel
+                    srv.submit(gid, frames_host[k % len(frames_host)], -1 - k)
+                srv.drain()
+                gaps = rng.exponential(1.0 / pace, n) if pace else np.zeros(n)
+                t0 = time.perf_counter()
+                t_next = t0
+                for i in range(n):
+                    t_next += gaps[i]
+                    dt = t_next - time.perf_counter()
+                    if dt > 3e-4:
+                        time.sleep(dt - 2e-4)
+                    while time.perf_counter() < t_next:
+                        pass
+                    srv.submit(gid, frames_host[i % len(frames_host)], i, timeout_ms=60000)
+                srv.drain()
+                t_int = time.perf_counter() - t0
+                st = srv.stats()
+            fitter.close()
+            wall = time.perf_counter() - t0
+            fitted = [v for k, v in fitter.fitted.items() if k >= 0]
+            n_fits = int(sum(int((v[:, 0] != 0).sum()) for v in fitted))
+        return {"frames": n, "offered_fps": pace or None, "fps": n / wall,
+                "fps_integration": n / t_int, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                "batches": st["batches"], "frames_fitted": len(fitted), "fits": n_fits,
+                "fits_per_s": n_fits / wall}
+
+    log(f"qmode_2880 serving: {n_frames} frames back to back, 16 rings fitted per frame")
+    full = run(n_frames, 0.0)
+    log(f"qmode_2880 serving: {full['fps']:,.0f} frames/s with fits "
+        f"({full['fps_integration']:,.0f} integrated), {full['fits']} fits of "
+        f"{full['frames_fitted']} frames")
+    paced = run(n_rate, rate)
+    log(f"qmode_2880 serving at {rate:.0f} frames/s offered: {paced['fps']:,.0f} frames/s, "
+        f"p50 {paced['p50_ms']:.2f} ms, p99 {paced['p99_ms']:.2f} ms")
+    return {"back_to_back": full, "paced": paced,
+            "what": "host frames over PCIe (33 MB up, a 33 MB cake and the lineouts down per "
+                    "frame), submit -> result latency; TCH fits of the 16 rings of each "
+                    "frame's area-weighted lineout, off the frame path"}
 
 
 def main():

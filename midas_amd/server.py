@@ -70,8 +70,11 @@ class FrameServer:
     def _callback(self, user, rid, geo, cake, lo, status):
         shape = self._shapes[geo]
         c = l = None
+        # Cakes are copied only when kept; an on_result consumer always gets
+        # the lineouts (3 rows of n_r), e.g. for the stream's peak fits.
         if status == 0 and self.keep:
             c = np.ctypeslib.as_array(cake, shape=(shape[0] * shape[1],)).reshape(shape).copy()
+        if status == 0 and (self.keep or self.on_result is not None):
             l = np.ctypeslib.as_array(lo, shape=(3 * shape[0],)).reshape(3, shape[0]).copy()
         r = Result(rid, geo, c, l, status)
         if self.on_result is not None:
