@@ -355,6 +355,18 @@ int mc_profile(mc_map *m, const float *cakes, int32_t n_frames,
                int32_t cakes_on_device, float *lineouts,
                int32_t lineouts_on_device);
 
+/* 1-D lineouts and their variances of cakes with propagated variance
+ * (profile_1d_with_variance, packages/midas_integrate/midas_integrate/
+ * kernels.py:604-644): mode MC_LINEOUT_AREA gives sum(A*I)/sum(A) and
+ * sum(A^2*Var)/sum(A)^2 over the bins with A > 1e-9, MC_LINEOUT_MEAN the
+ * mean of I and sum(Var)/N^2 over the same bins; NaN bins count as
+ * missing, a row without bins gives 0.  cakes and variances are
+ * [n_frames][n_r][n_eta], profile and profile_var [n_frames][n_r], float32,
+ * host or device (*_on_device). */
+int mc_profile_variance(mc_map *m, const float *cakes, const float *variances,
+                        int32_t n_frames, int32_t in_on_device, int32_t mode,
+                        float *profile, float *profile_var, int32_t out_on_device);
+
 /* ---- Zarr-zip input: Blosc1 chunks ---- */
 
 /* Decompresses one Blosc1 chunk (the chunk format of the reference's

@@ -16,7 +16,7 @@ from .caking import (AREA_THRESHOLD, CSRGeometry, Context, Event, build_csr, bui
                      IntegrationGraph, default_context, hybrid_message, integrate,
                      integrate_async, integrate_batch, integrate_hybrid,
                      integrate_with_variance,
-                     load_map_gpu, profile_1d, time_integrate)
+                     load_map_gpu, profile_1d, profile_1d_with_variance, time_integrate)
 from .outputs import ResultFiles
 from .postprocess import gauss_smooth_eta, median_filter_eta
 from .fused_csr import build_fused_geometry
@@ -30,6 +30,6 @@ __all__ = [
     "IntegrationGraph", "default_context", "integrate", "integrate_batch",
     "integrate_with_variance", "Event", "integrate_async", "integrate_hybrid",
     "hybrid_message",
-    "load_map_gpu", "profile_1d", "time_integrate", "ResultFiles", "gauss_smooth_eta",
+    "load_map_gpu", "profile_1d", "profile_1d_with_variance", "time_integrate", "ResultFiles", "gauss_smooth_eta",
     "median_filter_eta", "build_fused_geometry", "enrich_zarr",
 ]

@@ -147,6 +147,7 @@ SIGNATURES = {
     "mc_map_layout_info": (_I, [_V, _I32, _I32, _P(_I64)]),
     "mc_integrate": (_I, [_V, _V, _I32, _I32, _I32, _I32, _U32, _V, _I32, _V, _I32]),
     "mc_profile": (_I, [_V, _V, _I32, _I32, _V, _I32]),
+    "mc_profile_variance": (_I, [_V, _V, _V, _I32, _I32, _I32, _V, _V, _I32]),
     "mc_integrate_variance": (_I, [_V, _V, _I32, _I32, _I32, _V, _I32, _I32, _U32, _V, _V,
                                    _I32]),
     "mc_time_integrate": (_I, [_V, _V, _I32, _I32, _I32, _U32, _I32, _I32, _I32, _V, _V,

@@ -615,6 +615,44 @@ def profile_1d(int2d, geom: CSRGeometry, *, mode: str = "area_weighted"):
     return out[0] if squeeze else out
 
 
+def profile_1d_with_variance(int2d, var2d, geom: CSRGeometry, *, mode: str = "area_weighted"):
+    """(I_1d, Var_1d) of a cake and its propagated variance
+    (profile_1d_with_variance, kernels.py:604-644): 'area_weighted' gives
+    sum(A I)/sum(A) and sum(A^2 Var)/sum(A)^2, 'simple_mean' the mean of I
+    and sum(Var)/N^2, over the bins with A > 1e-9 (NaN bins missing).
+    (n_r, n_eta) -> (n_r,) each, (F, n_r, n_eta) -> (F, n_r)."""
+    if mode not in ("area_weighted", "simple_mean"):
+        raise ValueError(f"unknown profile_1d_with_variance mode {mode!r}")
+    code = _PROFILE_ROWS[mode]
+    squeeze = (int2d.ndim == 2)
+    if _is_cuda(int2d):
+        t = _torch()
+        c = int2d.reshape(-1, geom.n_r, geom.n_eta).to(t.float32).contiguous()
+        v = var2d.reshape(-1, geom.n_r, geom.n_eta).to(device=c.device, dtype=t.float32)
+        v = v.contiguous()
+        if v.shape != c.shape:
+            raise ValueError("var2d must match int2d")
+        F = c.shape[0]
+        prof = t.empty((F, geom.n_r), dtype=t.float32, device=c.device)
+        pvar = t.empty_like(prof)
+        t.cuda.current_stream(c.device).synchronize()
+        check(lib().mc_profile_variance(geom.handle, C.c_void_p(c.data_ptr()),
+                                        C.c_void_p(v.data_ptr()), F, 1, code,
+                                        C.c_void_p(prof.data_ptr()),
+                                        C.c_void_p(pvar.data_ptr()), 1))
+    else:
+        c = np.ascontiguousarray(int2d, dtype=np.float32).reshape(-1, geom.n_r, geom.n_eta)
+        v = np.ascontiguousarray(var2d, dtype=np.float32).reshape(-1, geom.n_r, geom.n_eta)
+        if v.shape != c.shape:
+            raise ValueError("var2d must match int2d")
+        F = c.shape[0]
+        prof = np.empty((F, geom.n_r), dtype=np.float32)
+        pvar = np.empty_like(prof)
+        check(lib().mc_profile_variance(geom.handle, c.ctypes.data, v.ctypes.data, F, 0, code,
+                                        prof.ctypes.data, pvar.ctypes.data, 0))
+    return (prof[0], pvar[0]) if squeeze else (prof, pvar)
+
+
 class IntegrationGraph:
     """The batched passes over one device frame buffer, captured once in a
     HIP graph and replayed by :meth:`launch`.  The caller refills

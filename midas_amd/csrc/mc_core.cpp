@@ -1717,6 +1717,45 @@ int mc_profile(mc_map *m, const float *cakes, int32_t n_frames, int32_t cakes_on
   return MC_OK;
 }
 
+int mc_profile_variance(mc_map *m, const float *cakes, const float *variances,
+                        int32_t n_frames, int32_t in_on_device, int32_t mode, float *profile,
+                        float *profile_var, int32_t out_on_device) {
+  MC_CHECK(m && cakes && variances && profile && profile_var && n_frames >= 1, MC_ERR_ARG,
+           "bad arguments");
+  MC_CHECK(mode == MC_LINEOUT_AREA || mode == MC_LINEOUT_MEAN, MC_ERR_ARG,
+           "profile mode %d: area-weighted (%d) or simple mean (%d)", mode,
+           MC_LINEOUT_AREA, MC_LINEOUT_MEAN);
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  MC_HIP(hipSetDevice(m->ctx->device));
+  hipStream_t s = m->ctx->stream;
+  const size_t cbytes = (size_t)n_frames * m->nb() * 4, pbytes = (size_t)n_frames * m->n_r * 4;
+  DBuf dc, dv, dp, dq;
+  int rc;
+  const float *cd = cakes, *vd = variances;
+  if (!in_on_device) {
+    if ((rc = dc.alloc(cbytes)) || (rc = dv.alloc(cbytes))) return rc;
+    MC_HIP(hipMemcpyAsync(dc.p, cakes, cbytes, hipMemcpyHostToDevice, s));
+    MC_HIP(hipMemcpyAsync(dv.p, variances, cbytes, hipMemcpyHostToDevice, s));
+    cd = dc.as<float>();
+    vd = dv.as<float>();
+  }
+  float *pd = profile, *qd = profile_var;
+  if (!out_on_device) {
+    if ((rc = dp.alloc(pbytes)) || (rc = dq.alloc(pbytes))) return rc;
+    pd = dp.as<float>();
+    qd = dq.as<float>();
+  }
+  if ((rc = launch_profile_var(s, *m, cd, vd, n_frames, mode == MC_LINEOUT_MEAN ? 1 : 0,
+                               pd, qd)))
+    return rc;
+  if (!out_on_device) {
+    MC_HIP(hipMemcpyAsync(profile, pd, pbytes, hipMemcpyDeviceToHost, s));
+    MC_HIP(hipMemcpyAsync(profile_var, qd, pbytes, hipMemcpyDeviceToHost, s));
+  }
+  MC_HIP(hipStreamSynchronize(s));
+  return MC_OK;
+}
+
 // ---- the batched passes of one call, captured in a HIP graph ----
 struct mc_graph {
   mc_map *m = nullptr;

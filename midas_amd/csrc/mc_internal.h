@@ -233,6 +233,10 @@ int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *bi
 size_t batch_xt_bytes(const Map &m, int dtype);
 int launch_profile(hipStream_t s, const Map &m, const float *cakes, int n_frames,
                    int mask_nan, float *lineouts);
+// Lineouts and their variances of cakes with propagated variance
+// ([n_frames][n_r] each); simple: simple_mean instead of area_weighted.
+int launch_profile_var(hipStream_t s, const Map &m, const float *cakes, const float *vars,
+                       int n_frames, int simple, float *prof, float *pvar);
 // One hybrid message (u32 count, u16 base[npx], i32 index[count], i64
 // value[count]) in device memory at msg, decoded to int32 out[npx].
 int launch_hybrid_decode(hipStream_t s, const void *msg, int64_t npx, int64_t count,

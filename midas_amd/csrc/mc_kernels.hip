@@ -276,6 +276,48 @@ __global__ __launch_bounds__(256) void k_profile(
   if (threadIdx.x == 0) write_lineouts(lo, row_static[r], mask_nan, lineouts + (int64_t)f * 3 * n_r, n_r, r);
 }
 
+// Lineout of a cake with propagated variance (profile_1d_with_variance,
+// packages/midas_integrate/midas_integrate/kernels.py:604-644): per (row
+// r, frame f), over the bins whose in-bounds area is above 1e-9 and whose
+// value is not NaN, with weight w = A (area_weighted) or 1 (simple_mean):
+// I = sum w I / sum w and V = sum w^2 Var / (sum w)^2, 0 where no bin
+// counts.  Sums in FP64.
+__global__ __launch_bounds__(256) void k_profile_var(
+    const float *__restrict__ cakes, const float *__restrict__ vars,
+    const float *__restrict__ a_inb, int n_eta, int n_r, int64_t nb, int simple,
+    float *__restrict__ prof, float *__restrict__ pvar) {
+  const int r = blockIdx.x, f = blockIdx.y;
+  const float *c = cakes + (int64_t)f * nb + (int64_t)r * n_eta;
+  const float *v = vars + (int64_t)f * nb + (int64_t)r * n_eta;
+  const float *a = a_inb + (int64_t)r * n_eta;
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (int e = threadIdx.x; e < n_eta; e += 256) {
+    const float o = c[e];
+    if (a[e] > 1e-9f && !isnan(o)) {
+      const double w = simple ? 1.0 : (double)a[e];
+      s1 += w * (double)o;
+      s2 += w * w * (double)v[e];
+      s3 += w;
+    }
+  }
+  __shared__ double red[3][256];
+  red[0][threadIdx.x] = s1;
+  red[1][threadIdx.x] = s2;
+  red[2][threadIdx.x] = s3;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double sw = red[2][0];
+    const bool ok = simple ? sw > 0.0 : sw > 1e-9;
+    prof[(int64_t)f * n_r + r] = ok ? (float)(red[0][0] / sw) : 0.f;
+    pvar[(int64_t)f * n_r + r] = ok ? (float)(red[1][0] / (sw * sw)) : 0.f;
+  }
+}
+
 // ---------------------------------------------------------------------
 // Batched path: two kernels per pass of up to 64 frames.
 //
@@ -1324,6 +1366,14 @@ int launch_profile(hipStream_t s, const Map &m, const float *cakes, int n_frames
   hipLaunchKernelGGL(k_profile, dim3(m.n_r, n_frames), dim3(256), 0, s, cakes,
                      m.d_area_inb.as<float>(), m.d_row_static.as<float4>(), m.n_eta, m.n_r,
                      m.nb(), mask_nan, lineouts);
+  MC_HIP(hipGetLastError());
+  return MC_OK;
+}
+
+int launch_profile_var(hipStream_t s, const Map &m, const float *cakes, const float *vars,
+                       int n_frames, int simple, float *prof, float *pvar) {
+  hipLaunchKernelGGL(k_profile_var, dim3(m.n_r, n_frames), dim3(256), 0, s, cakes, vars,
+                     m.d_area_inb.as<float>(), m.n_eta, m.n_r, m.nb(), simple, prof, pvar);
   MC_HIP(hipGetLastError());
   return MC_OK;
 }

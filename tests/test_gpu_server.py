@@ -58,6 +58,28 @@ def test_mix_equals_alone():
             assert rel_err(r.lineouts, lo[0], lo[0]).max() <= 1e-5
 
 
+def test_on_result_gets_lineouts_without_kept_cakes():
+    """keep_results=False with an on_result consumer (the stream's peak
+    fits): each callback gets the frame's lineouts, no cake is copied."""
+    P, dt = _geoms()[1]
+    g = ma.build_map(P)
+    rng = np.random.default_rng(3)
+    frames = rng.poisson(200, (12, P.NrPixelsZ, P.NrPixelsY)).astype(dt)
+    got = {}
+    with FrameServer(hbm_budget_bytes=4 << 30, max_batch=8, keep_results=False,
+                     on_result=lambda r: got.__setitem__(r.request_id, r)) as srv:
+        gid = srv.add_geometry(g, dt)
+        for i, f in enumerate(frames):
+            srv.submit(gid, f, i)
+        srv.drain()
+    assert sorted(got) == list(range(len(frames)))
+    for i, f in enumerate(frames):
+        r = got[i]
+        assert r.status == 0 and r.cake is None
+        _, lo = ma.integrate_batch(f, g, mode="bilinear", mask_nan=True)
+        assert rel_err(r.lineouts, lo[0], lo[0]).max() <= 1e-5
+
+
 def test_budget_admission():
     P, dt = _geoms()[1]
     g = ma.build_map(P)

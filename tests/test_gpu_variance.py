@@ -87,3 +87,42 @@ def test_linear_in_variance_image_with_merged_weights():
     _, q = ma.integrate_with_variance(frames, g, variance_image=frames.astype(np.float32),
                                       mask_nan=True)
     np.testing.assert_allclose(p[ok], q[ok], rtol=1e-6)
+
+
+def test_profile_with_variance_matches_oracle(oracle_lib):
+    """profile_1d_with_variance (mc_profile_variance) on cakes with
+    propagated variance, NaN (masked) bins included, against the FP64
+    restatement of kernels.py:604-644 with the oracle map's in-bounds
+    areas: 1e-5 of the same reduction of |cake| (intensity) and 1e-5
+    relative (variance, positive terms); host arrays and torch tensors."""
+    from oracle.profile import profile_with_variance
+    P = small_cases()["tilt"]
+    rng = np.random.default_rng(8)
+    mask = (rng.random((P.NrPixelsZ, P.NrPixelsY)) < 0.01).astype(np.float64)
+    frames = rng.poisson(300.0, (3, P.NrPixelsZ, P.NrPixelsY)).astype(np.uint16)
+    dark = rng.poisson(30.0, (P.NrPixelsZ, P.NrPixelsY)).astype(np.uint16)
+    print("[variance] tilt, profile with variance", flush=True)
+    g = ma.build_map(P, mask=mask)
+    g.set_dark(dark)
+    om = oracle_lib.build_map(to_oracle(P), mask=mask)
+    area = oracle_lib.area_inbounds(om, P.NrPixelsY, P.NrPixelsZ).reshape(P.n_r_bins,
+                                                                         P.n_eta_bins)
+    cakes, var = ma.integrate_with_variance(frames, g, mode="bilinear", mask_nan=True)
+    assert np.isnan(cakes).any()
+    worst = [0.0, 0.0]
+    for mode in ("area_weighted", "simple_mean"):
+        prof, pvar = ma.profile_1d_with_variance(cakes, var, g, mode=mode)
+        for i in range(len(frames)):
+            want_p, want_v = profile_with_variance(cakes[i], var[i], area, mode)
+            scale, _ = profile_with_variance(np.abs(cakes[i]), var[i], area, mode)
+            ep = np.abs(prof[i] - want_p) / np.maximum(scale, 1e-30)
+            ev = np.abs(pvar[i] - want_v) / np.maximum(np.abs(want_v), 1e-30)
+            worst = [max(worst[0], ep.max()), max(worst[1], ev.max())]
+    print(f"[variance] profile worst {worst[0]:.2e}, variance worst {worst[1]:.2e}", flush=True)
+    assert worst[0] <= 1e-5 and worst[1] <= 1e-5
+    torch = pytest.importorskip("torch")
+    tp, tv = ma.profile_1d_with_variance(torch.as_tensor(cakes[0], device="cuda"),
+                                         torch.as_tensor(var[0], device="cuda"), g)
+    hp, hv = ma.profile_1d_with_variance(cakes[0], var[0], g)
+    np.testing.assert_array_equal(tp.cpu().numpy(), hp)
+    np.testing.assert_array_equal(tv.cpu().numpy(), hv)
