@@ -343,11 +343,13 @@ __global__ __launch_bounds__(256) void k_profile_var(
 // hold no bin).  The four 16-lane groups of a wave take the four bins, one
 // each, and lane l of a group holds frames l*W/16 .. (l+1)*W/16 - 1: one
 // LDS read serves W/16 frames of one entry, and no lane exchange is needed
-// to finish a bin.  A quad's staged entries are interleaved: pair m of
-// slot s at entries 8m + 2s, 8m + 2s + 1 (pairs past a bin's end are
+// to finish a bin.  A quad's staged slots (those with staged entries, P of
+// them) are interleaved: pair m of the r-th staged slot at entries
+// e_off + 2 (m P + r) and the next (pairs past a bin's end are
 // zero-weight entries on a zero pixel row), so each iteration reads one
-// 16-byte pair per group; the staged entries of a row are one run, quad
-// after quad, and the next pair is read an iteration ahead across quads.
+// 16-byte pair per group; the groups of the other slots read the tile's
+// zero pair.  The staged entries of a row are one run, quad after quad,
+// and the next pair is read an iteration ahead across full quads.
 // The quad records (bins, flags, entry run, scale/dark/area of the four
 // bins) are uniform over the wave and come in through scalar loads.
 // After a quad, two lane-half swaps and two 16-lane swaps transpose the
@@ -361,10 +363,11 @@ __global__ __launch_bounds__(256) void k_profile_var(
 // chunks of each band.  Bins whose own footprint exceeds the budget
 // ("direct" bins) gather from Xt in global memory.
 //   chunk: tile_begin tile_end band_row0 band_rows
-//   tile : px_beg npx 0 0 q_beg nq e_beg e_end ra nc 0 0 nrows 0 0 0
+//   tile : px_beg npx zp 0 q_beg nq e_beg e_end ra nc 0 0 nrows 0 0 0
 //          (footprint of npx pixels listed at px_beg, band rows ra .. ra +
 //          nrows - 1, columns of nc eta bins, nq quads per row: the quads
-//          of row j at q_beg + j * nq, rows padded with empty quads)
+//          of row j at q_beg + j * nq, rows padded with empty quads; the
+//          zero pair at entry zp of the tile)
 //   quad : QuadRec (below); qdir[4 quad + s] = the direct entry range of
 //          slot s
 //   ent  : (byte offset of the pixel row in the footprint, or global pixel
