@@ -1626,6 +1626,10 @@ int mc_integrate_async(mc_map *m, const void *frames_dev, int32_t dtype, int32_t
                        lineouts_dev, false, false);
 }
 
+// Frames from which mc_integrate_variance sums Poisson variances in
+// batched passes (fewer: the one-frame kernel per frame).
+constexpr int kVarBatchMin = 3;
+
 int mc_integrate_variance(mc_map *m, const void *frames, int32_t dtype, int32_t n_frames,
                           int32_t frames_on_device, const float *variance,
                           int32_t variance_on_device, int32_t mode, uint32_t flags,
@@ -1676,11 +1680,30 @@ int mc_integrate_variance(mc_map *m, const void *frames, int32_t dtype, int32_t 
     cd = dc.as<float>();
     vard = dvar.as<float>();
   }
-  int64_t sub_frames = 0;
-  if ((rc = ensure_sub(m->work.xt, *m, n_frames, &sub_frames))) return rc;
-  if ((rc = launch_plain_var(s, *m, c, binfo, d2, fd, dtype, vd, cd, vard, n_frames,
-                             m->work.xt.as<float>(), sub_frames)))
-    return rc;
+  if (!variance && n_frames >= kVarBatchMin && m->rows_per_group == 16) {
+    // Poisson statistics over 3 frames or more: the batched kernel sums
+    // w x and w^2 x from the same staged pixel rows, 32 frames a pass.
+    Tiles *t = nullptr;
+    if ((rc = ensure_tiles(m, mode, batch_elem(dtype), &t, 32)) ||
+        (rc = ensure_recs(m, s, t, mode, flags, binfo)) ||
+        (rc = ensure_xt(m->work.xt, *m, dtype)) || (rc = ensure_part(m->work.part, *t)))
+      return rc;
+    const int mask_nan = (flags & MC_F_MASK_NAN) ? 1 : 0;
+    for (int f0 = 0; f0 < n_frames; f0 += 32) {
+      const int nf = std::min(32, n_frames - f0);
+      if ((rc = launch_batched(s, *m, *t, binfo, mask_nan,
+                               (const char *)fd + (size_t)f0 * m->npx() * es, dtype, nf,
+                               cd + (size_t)f0 * m->nb(), nullptr, m->work.xt.p,
+                               m->work.part.as<float>(), vard + (size_t)f0 * m->nb(), d2)))
+        return rc;
+    }
+  } else {
+    int64_t sub_frames = 0;
+    if ((rc = ensure_sub(m->work.xt, *m, n_frames, &sub_frames))) return rc;
+    if ((rc = launch_plain_var(s, *m, c, binfo, d2, fd, dtype, vd, cd, vard, n_frames,
+                               m->work.xt.as<float>(), sub_frames)))
+      return rc;
+  }
   if (!out_on_device) {
     MC_HIP(hipMemcpyAsync(cakes, cd, cbytes, hipMemcpyDeviceToHost, s));
     MC_HIP(hipMemcpyAsync(variances, vard, cbytes, hipMemcpyDeviceToHost, s));

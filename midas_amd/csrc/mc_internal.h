@@ -226,10 +226,14 @@ int launch_quad_recs(hipStream_t s, const Map &m, Tiles &t, const float2 *binfo)
 // high half of its first word (2-byte batch elements).
 int launch_ents_dark(hipStream_t s, const Map &m, Tiles &t);
 // One batched pass of up to 64 frames; xt is scratch of batch_xt_bytes(),
-// part of t.part_bytes() (needed when lineouts is not NULL).
+// part of t.part_bytes() (needed when lineouts is not NULL).  With vcakes
+// (32-frame layouts of 16-row bands) the pass also writes the Poisson
+// variance of each bin, sum w^2 (x - dint) less d2term (sum w^2 (dark -
+// dint), NULL for 0), times the value's scale squared.
 int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
                    int mask_nan, const void *frames, int dtype, int n_frames, float *cakes,
-                   float *lineouts, void *xt, float *part);
+                   float *lineouts, void *xt, float *part, float *vcakes = nullptr,
+                   const float *d2term = nullptr);
 size_t batch_xt_bytes(const Map &m, int dtype);
 int launch_profile(hipStream_t s, const Map &m, const float *cakes, int n_frames,
                    int mask_nan, float *lineouts);

@@ -748,14 +748,16 @@ __device__ __forceinline__ void lineout_add(float lo[4], float o, float a, float
 // in the high half (k_ents_dark); each value is converted and the dark
 // subtracted before the product, so the sums are of dark-subtracted counts.
 // Direct bins read the dark from dint.
-template <typename TL, int G, bool DARK, int FPL>
+template <typename TL, int G, bool DARK, int FPL, bool VAR>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batched(
     const int32_t *__restrict__ chunks, const int32_t *__restrict__ xcd,
     const int32_t *__restrict__ tiles, const QuadRec *__restrict__ quads,
     const int2 *__restrict__ qdir, const uint2 *__restrict__ ents,
     const int32_t *__restrict__ tpix, const TL *__restrict__ xt,
     const uint16_t *__restrict__ dint, int n_frames, int plane_off, int tp_off, int tp_len,
-    int64_t nb, float *__restrict__ cakes, float *__restrict__ part) {
+    int64_t nb, float *__restrict__ cakes, float *__restrict__ part,
+    float *__restrict__ vcakes, const float *__restrict__ d2term) {
+  static_assert(!VAR || FPL == 2, "variance sums run in 32-frame passes");
   extern __shared__ __attribute__((aligned(16))) char lds_raw[];
   typedef typename Lanes<TL, FPL>::type QT;
   constexpr int XF = 16 * FPL;                // frames per pass (pixel row)
@@ -847,13 +849,17 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
         const int kstep = mine ? 2 * __builtin_popcount(sm) : 0;
         if (ka != ra) tw = *reinterpret_cast<const u4v *>(lent + ka);
         float tot[FPL];  // this group's bin, this lane's frames
+        constexpr int NV = VAR ? FPL : 1;
+        float vtot[NV];  // VAR: the same bins' sums of w^2 x
         {
-          float gs[FPL];
+          float gs[FPL], vs[NV];
 #pragma unroll
           for (int j = 0; j < FPL; ++j) gs[j] = 0.f;
+#pragma unroll
+          for (int j = 0; j < NV; ++j) vs[j] = 0.f;
           // Blocks of 16 pairs (32 entries) of plain FMAs; a bin with more
           // adds its blocks through a Kahan accumulator.
-          Kahan acc[FPL];
+          Kahan acc[FPL], vacc[NV];
           int m = 0;
           for (;;) {
             const int m_end = min(nit, m + 16);
@@ -881,12 +887,25 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
               const float wa = __uint_as_float(two.y), wb = __uint_as_float(two.w);
 #pragma unroll
               for (int j = 0; j < FPL; ++j) gs[j] = fmaf(fb[j], wb, fmaf(fa[j], wa, gs[j]));
+              if constexpr (VAR) {
+                // Poisson: the dark-subtracted count is its own variance
+                const float wa2 = wa * wa, wb2 = wb * wb;
+#pragma unroll
+                for (int j = 0; j < NV; ++j) vs[j] = fmaf(fb[j], wb2, fmaf(fa[j], wa2, vs[j]));
+              }
             }
             if (m >= nit) break;
 #pragma unroll
             for (int j = 0; j < FPL; ++j) {
               kahan_add(acc[j], gs[j]);
               gs[j] = 0.f;
+            }
+            if constexpr (VAR) {
+#pragma unroll
+              for (int j = 0; j < NV; ++j) {
+                kahan_add(vacc[j], vs[j]);
+                vs[j] = 0.f;
+              }
             }
           }
           if (nit > 16) {
@@ -895,9 +914,18 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
               kahan_add(acc[j], gs[j]);
               tot[j] = acc[j].value();
             }
+            if constexpr (VAR) {
+#pragma unroll
+              for (int j = 0; j < NV; ++j) {
+                kahan_add(vacc[j], vs[j]);
+                vtot[j] = vacc[j].value();
+              }
+            }
           } else {
 #pragma unroll
             for (int j = 0; j < FPL; ++j) tot[j] = gs[j];
+#pragma unroll
+            for (int j = 0; j < NV; ++j) vtot[j] = vs[j];
           }
         }
         ra = ka;  // tw holds the pair at ka
@@ -906,9 +934,11 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
           // batch in global memory.
           if ((flags >> (4 + grp)) & 1) {
             const int2 dr = qdir[4 * (q_row + qi) + grp];
-            Kahan acc[FPL];
+            Kahan acc[FPL], vacc[NV];
 #pragma unroll
             for (int j = 0; j < FPL; ++j) acc[j].s = tot[j];
+#pragma unroll
+            for (int j = 0; j < NV; ++j) vacc[j].s = vtot[j];
             for (int e = dr.x; e < dr.y; ++e) {
               const uint2 en = ents[e];
               const QT d = *reinterpret_cast<const QT *>(
@@ -920,11 +950,18 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
 #pragma unroll
                 for (int j = 0; j < FPL; ++j) fv[j] -= dk;
               }
+              const float w = __uint_as_float(en.y);
 #pragma unroll
-              for (int j = 0; j < FPL; ++j) kahan_add(acc[j], fv[j] * __uint_as_float(en.y));
+              for (int j = 0; j < FPL; ++j) kahan_add(acc[j], fv[j] * w);
+              if constexpr (VAR) {
+#pragma unroll
+                for (int j = 0; j < NV; ++j) kahan_add(vacc[j], fv[j] * (w * w));
+              }
             }
 #pragma unroll
             for (int j = 0; j < FPL; ++j) tot[j] = acc[j].value();
+#pragma unroll
+            for (int j = 0; j < NV; ++j) vtot[j] = vacc[j].value();
           }
         }
         const uint32_t pm = (uint32_t)flags & 15u;
@@ -990,6 +1027,31 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
             } else {
               if (pb & 1u) dst[0] = o0;
               if (pb & 2u) dst[1] = o1;
+            }
+          }
+          if constexpr (VAR) {
+            // The variance: the same swap of the w^2 sums, less the dark
+            // remainder's sum w^2 (dark - dint), times the value's scale
+            // squared (NaN where the value is).
+            const auto pv = __builtin_amdgcn_permlane16_swap(
+                __float_as_uint(vtot[0]), __float_as_uint(vtot[1]), false, false);
+            const uint32_t pb = (pm >> s0) & 3u;
+            float r0 = 0.f, r1 = 0.f;
+            if (d2term) {
+              if (pb & 1u) r0 = d2term[Q.b0 + s0];
+              if (pb & 2u) r1 = d2term[Q.b0 + s0 + 1];
+            }
+            const float sc0 = up ? Q.scale[2] : Q.scale[0], sc1 = up ? Q.scale[3] : Q.scale[1];
+            const float v0 = (__uint_as_float(pv[0]) - r0) * sc0 * sc0;
+            const float v1 = (__uint_as_float(pv[1]) - r1) * sc1 * sc1;
+            if (live) {
+              float *dst = vcakes + (int64_t)frame * nb + Q.b0 + s0;
+              if (pb == 3u && vec2_ok) {
+                *reinterpret_cast<float2 *>(dst) = make_float2(v0, v1);
+              } else {
+                if (pb & 1u) dst[0] = v0;
+                if (pb & 2u) dst[1] = v1;
+              }
             }
           }
           lineout_add(lo[g], o0, up ? Q.a[2] : Q.a[0], up ? Q.af[2] : Q.af[0]);
@@ -1426,7 +1488,7 @@ size_t batch_xt_bytes(const Map &m, int dtype) {
 template <typename TIn, typename TL, int G, int FPL>
 static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mask_nan,
                             const void *frames, int n_frames, float *cakes, float *lineouts,
-                            void *xt, float *part) {
+                            void *xt, float *part, float *vcakes, const float *d2term) {
   constexpr int XF = 16 * FPL;
   const uint16_t *dhi = m.has_dark ? m.d_dint.as<uint16_t>() : nullptr;
   // 1. pixel-major batch
@@ -1447,7 +1509,16 @@ static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mas
   // 2. tiles (uint16 batches subtract the dark carried by the entries;
   // FP32 batches were dark-subtracted by the transpose)
   constexpr bool kU16 = sizeof(TL) == 2;
-  auto kern = (kU16 && dhi) ? k_batched<TL, G, kU16, FPL> : k_batched<TL, G, false, FPL>;
+  auto kern = (kU16 && dhi) ? k_batched<TL, G, kU16, FPL, false>
+                            : k_batched<TL, G, false, FPL, false>;
+  if constexpr (FPL == 2 && G == 1) {
+    if (vcakes)
+      kern = (kU16 && dhi) ? k_batched<TL, G, kU16, FPL, true> : k_batched<TL, G, false, FPL, true>;
+  }
+  if (vcakes && !(FPL == 2 && G == 1)) {
+    set_error("variance sums need 32-frame passes of 16-row bands");
+    return MC_ERR_STATE;
+  }
   const size_t lds = t.lds_needed;
   if (lds > 65536)
     MC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1457,7 +1528,7 @@ static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mas
                      t.chunks.as<int32_t>(), t.xcd.as<int32_t>(), t.tiles.as<int32_t>(),
                      t.qrecs.as<QuadRec>(), t.qdir.as<int2>(), t.ents.as<uint2>(),
                      t.tpix.as<int32_t>(), (const TL *)xt, dhi, n_frames, t.plane_off, t.tp_off,
-                     t.tp_len, m.nb(), cakes, part);
+                     t.tp_len, m.nb(), cakes, part, vcakes, d2term);
   MC_HIP(hipGetLastError());
   // 3. lineouts
   if (lineouts) {
@@ -1472,7 +1543,8 @@ static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mas
 
 int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *binfo,
                    int mask_nan, const void *frames, int dtype, int n_frames, float *cakes,
-                   float *lineouts, void *xt, float *part) {
+                   float *lineouts, void *xt, float *part, float *vcakes,
+                   const float *d2term) {
   (void)binfo;  // folded into the quad records (launch_quad_recs)
   if (n_frames < 1 || n_frames > t.width) {
     set_error("batched pass of %d frames outside 1..%d", n_frames, t.width);
@@ -1494,10 +1566,10 @@ int launch_batched(hipStream_t s, const Map &m, const Tiles &t, const float2 *bi
       switch (t.rows / kWaves) {
         case 1:
           return launch_batched_g<T, TL, 1, FPL>(s, m, t, mask_nan, frames, n_frames, cakes,
-                                                 lineouts, xt, part);
+                                                 lineouts, xt, part, vcakes, d2term);
         case 2:
           return launch_batched_g<T, TL, 2, FPL>(s, m, t, mask_nan, frames, n_frames, cakes,
-                                                 lineouts, xt, part);
+                                                 lineouts, xt, part, vcakes, d2term);
         default:
           set_error("rows per band %d not in {16, 32}", t.rows);
           return MC_ERR_ARG;

@@ -126,3 +126,52 @@ def test_profile_with_variance_matches_oracle(oracle_lib):
     hp, hv = ma.profile_1d_with_variance(cakes[0], var[0], g)
     np.testing.assert_array_equal(tp.cpu().numpy(), hp)
     np.testing.assert_array_equal(tv.cpu().numpy(), hv)
+
+
+@pytest.mark.parametrize("dtype", [np.uint16, np.float32])
+def test_batched_poisson_variance(oracle_lib, dtype):
+    """Three frames or more with Poisson statistics take the batched kernel
+    (32-frame passes summing w x and w^2 x from the same staged pixel
+    rows): values and variances against the FP64 oracle on a whole-pixel
+    map, and against the one-frame kernel frame by frame on a tilted
+    bilinear map with masked bins, over two passes (40 frames); 1e-5
+    relative (of the |frame - dark| cake for the values)."""
+    P = small_cases()["flat"]
+    rng = np.random.default_rng(4)
+    frames = rng.poisson(200.0, (5, P.NrPixelsZ, P.NrPixelsY)).astype(dtype)
+    dark = rng.poisson(20.0, (P.NrPixelsZ, P.NrPixelsY)).astype(dtype)
+    print(f"[variance] batched Poisson, flat/floor, {np.dtype(dtype).name}", flush=True)
+    g = ma.build_map(P)
+    g.set_dark(dark)
+    om = oracle_lib.build_map(to_oracle(P))
+    cakes, var = ma.integrate_with_variance(frames, g, mode="floor")
+    for i in range(len(frames)):
+        img = frames[i].astype(np.float64) - dark
+        want = _oracle_var(om, img, P.NrPixelsY)
+        np.testing.assert_allclose(var[i], want, rtol=1e-5, atol=1e-6)
+        # values: the one-frame kernel (checked against the oracle elsewhere)
+        # (frames - dark > 0 everywhere here, so |cake| is the cake of |frame - dark|)
+        one, _ = ma.integrate_batch(frames[i], g, mode="floor", mask_nan=False, lineouts=False)
+        assert (img > 0).all()
+        assert (np.abs(cakes[i] - one[0]) / np.maximum(np.abs(one[0]), 1e-30)).max() <= 1e-5
+    P = small_cases()["tilt"]
+    mask = (rng.random((P.NrPixelsZ, P.NrPixelsY)) < 0.01).astype(np.float64)
+    frames = rng.poisson(300.0, (40, P.NrPixelsZ, P.NrPixelsY)).astype(dtype)
+    dark = rng.poisson(30.0, (P.NrPixelsZ, P.NrPixelsY)).astype(dtype)
+    print(f"[variance] batched Poisson, tilt/bilinear, masked, 40 frames", flush=True)
+    g = ma.build_map(P, mask=mask)
+    g.set_dark(dark)
+    cakes, var = ma.integrate_with_variance(frames, g, mode="bilinear", mask_nan=True)
+    worst = [0.0, 0.0]
+    for i in range(0, 40, 3):
+        c1, v1 = ma.integrate_with_variance(frames[i], g, mode="bilinear", mask_nan=True)
+        assert np.array_equal(np.isnan(cakes[i]), np.isnan(c1))
+        assert np.array_equal(np.isnan(var[i]), np.isnan(v1))
+        assert (frames[i].astype(np.float64) - dark > 0).all()
+        ok = ~np.isnan(c1)
+        ec = np.abs(cakes[i][ok] - c1[ok]) / np.maximum(np.abs(c1[ok]), 1e-30)
+        ev = np.abs(var[i][ok] - v1[ok]) / np.maximum(np.abs(v1[ok]), 1e-30)
+        worst = [max(worst[0], ec.max()), max(worst[1], ev.max())]
+    print(f"[variance] batched vs one-frame: values {worst[0]:.2e}, variances {worst[1]:.2e}",
+          flush=True)
+    assert worst[0] <= 1e-5 and worst[1] <= 1e-5
