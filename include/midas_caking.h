@@ -340,7 +340,9 @@ int mc_graph_destroy(mc_graph *g);
  * images, or NULL for Poisson statistics: Var = the frame minus the dark
  * (the dark's sum w^2 * dark is subtracted).  cakes and variances are
  * [n_frames][n_r][n_eta] float32, host or device (out_on_device).  The
- * hybrid dtype is not accepted here. */
+ * hybrid dtype is not accepted here.  Poisson statistics over 3 frames or
+ * more run batched passes of 32 frames (values and variances from one
+ * gather); given variance images run the one-frame kernel per frame. */
 int mc_integrate_variance(mc_map *m, const void *frames, int32_t dtype, int32_t n_frames,
                           int32_t frames_on_device, const float *variance,
                           int32_t variance_on_device, int32_t mode, uint32_t flags,
