@@ -239,8 +239,9 @@ constexpr int kWaves = 16, kTileInts = 16;
 // between quads; a tile stages exactly its footprint, one pixel row of the
 // batch per pixel.  A column that does not fit whole is split into row
 // ranges, and a bin that does not fit alone is "direct" (gathered from
-// global memory).  Chunks are runs of consecutive tiles of one band of
-// about `target` entries: the work of one workgroup.  in_size is the
+// global memory).  A band gets one chunk (the work of one workgroup) per
+// `target` entries of its tiles, and its tiles are dealt to its chunks in
+// turn.  in_size is the
 // element size of the pixel-major batch (2 or 4 bytes).
 int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
   int rc = ensure_csr(m, mode);
@@ -321,16 +322,16 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
     int64_t ents = 0;         // staged entries
     bool chunk_start = false; // first tile of a workgroup's run
   };
-  // Chunks: runs of consecutive tiles of one band of about `target` cost
-  // (a few thousand workgroups for a 2048^2 map).  The cost of a tile
-  // counts its entries plus a fixed charge per bin and per tile: a bin of 2
-  // entries costs a wave about as much as one of 30.  A chunk ends only
-  // before an eta column that is a multiple of 8: the kernel writes whole
-  // 8-bin granules of a row, and a granule split between two workgroups
-  // costs one 4-byte store per bin.
+  // Chunks (one workgroup each): the band planner cuts a band's tiles into
+  // runs of about `target` cost (a few thousand for a 2048^2 map), each
+  // ending before an eta column that is a multiple of 8; the number of runs
+  // is the number of workgroups the band gets.  The cost of a tile counts
+  // its entries plus a fixed charge per bin and per tile: a bin of 2
+  // entries costs a wave about as much as one of 30.  The band's tiles are
+  // then dealt to its chunks in turn (below).
   constexpr int64_t kBinCost = 24, kTileCost = 256;
   const int64_t target =
-      std::max<int64_t>(4096, (c.nnz + kBinCost * nb + kTileCost * (nb / kTileBins + 1)) / 2048);
+      std::max<int64_t>(4096, (c.nnz + kBinCost * nb + kTileCost * (nb / kTileBins + 1)) / 4096);
   const int n_bands = (nR + H - 1) / H;
   // Columns added at a time: 4 where every row starts at a multiple of 4
   // bins (tiles then hold whole quads), else 1.
@@ -496,7 +497,31 @@ int ensure_tiles(Map *m, int mode, int in_size, Tiles **out, int width) {
       flush();
     }
   });
-  // Flatten, starting a chunk at each tile the band planner marked.
+  // Deal each band's tiles to its k chunks in turn: tile i (in eta order)
+  // goes to chunk i mod k.  Consecutive chunks run on one XCD at the same
+  // time (see k_batched), so the band's workgroups work on neighbouring
+  // tiles together: the 8-bin granules of a 128-byte cake line are stored
+  // about a tile's time apart and merge in that XCD's L2 before they are
+  // written back, and neighbouring footprints share their border pixels
+  // there.  (A workgroup that walks its own run of tiles leaves a line
+  // open for four tiles, and every granule goes to memory on its own.)
+  parallel_for(n_bands, [&](int64_t a, int64_t bq) {
+    for (int64_t gi = a; gi < bq; gi++) {
+      std::vector<TileH> &v = bands[gi];
+      size_t k = 0;
+      for (const TileH &T : v) k += T.chunk_start ? 1 : 0;
+      if (k < 2) continue;
+      std::vector<TileH> dealt;
+      dealt.reserve(v.size());
+      for (size_t j = 0; j < k; j++)
+        for (size_t i = j; i < v.size(); i += k) {
+          v[i].chunk_start = i == j;
+          dealt.push_back(std::move(v[i]));
+        }
+      v.swap(dealt);
+    }
+  });
+  // Flatten, starting a chunk at each tile marked above.
   std::vector<int64_t> chunk_cost;
   std::vector<int32_t> chunks, band_chunk(n_bands + 1, 0), tiles;
   std::vector<int4> qs;

@@ -328,8 +328,9 @@ __global__ __launch_bounds__(256) void k_profile_var(
 //
 // k_batched cuts the cake rows into bands of 16*G rows and a band into
 // tiles: rectangles of rows x eta columns whose pixel footprint fits the
-// LDS budget.  A workgroup of 16 waves takes one chunk -- a run of
-// consecutive tiles of one band -- and for each tile stages into LDS
+// LDS budget.  A workgroup of 16 waves takes one chunk -- every k-th tile
+// of one band, the band's k chunks running side by side on one XCD -- and
+// for each tile stages into LDS
 //   * its map entries (byte offset of the pixel row in the footprint,
 //     weight),
 //   * the footprint of the batch: each pixel's W frames contiguous --
@@ -614,7 +615,8 @@ template <> struct Lanes<float, 1> {
 template <int LG> struct GranVec;
 template <> struct GranVec<8> { typedef float type __attribute__((ext_vector_type(8))); };
 
-// Stores a row slot's granule for this lane's frame f (16 consecutive bins).
+// Stores a row slot's granule for this lane's frame f (8 consecutive bins;
+// mask: the bins this workgroup holds).
 template <int LG, typename VT>
 __device__ __forceinline__ void flush_granule(const VT &v, int idx, uint32_t mask, int f,
                                               float *__restrict__ cakes, int64_t nb,
@@ -630,6 +632,15 @@ __device__ __forceinline__ void flush_granule(const VT &v, int idx, uint32_t mas
       w.w = v[q + 3];
       *reinterpret_cast<f4v *>(dst + q) = w;
     }
+  } else if ((mask == 0x0fu || mask == 0xf0u) && vec_ok) {
+    // half a granule (a tile of 4 columns): one 16-byte store
+    const int q = mask == 0x0fu ? 0 : 4;
+    f4v w;
+    w.x = v[q];
+    w.y = v[q + 1];
+    w.z = v[q + 2];
+    w.w = v[q + 3];
+    *reinterpret_cast<f4v *>(dst + q) = w;
   } else {
 #pragma unroll
     for (int q = 0; q < LG; ++q)
