@@ -1167,28 +1167,60 @@ __global__ void k_quad_recs(const int4 *__restrict__ qs, const float2 *__restric
   out[i] = r;
 }
 
-// Lineouts from the per-chunk partial sums: one thread per (frame, row);
-// the chunks of band b are [band_chunk[b], band_chunk[b+1]).
-__global__ void k_lineout_finalize(const float *__restrict__ part,
-                                   const int32_t *__restrict__ band_chunk,
-                                   const float4 *__restrict__ row_static, int n_frames, int n_r,
-                                   int band_rows, int np, int mask_nan,
-                                   float *__restrict__ lineouts) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)n_frames * n_r) return;
-  // Consecutive threads take consecutive frames of one row: the partials
-  // of a (chunk, row) are frame-contiguous.
-  const int r = (int)(i / n_frames), f = (int)(i % n_frames);
-  const int band = r / band_rows, rib = r - band * band_rows;
-  float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int c = band_chunk[band]; c < band_chunk[band + 1]; ++c) {
-    const float *L = part + ((int64_t)c * band_rows + rib) * 4 * np;
-    s.x += L[f];
-    s.y += L[np + f];
-    s.z += L[2 * np + f];
-    s.w += L[3 * np + f];
+// Lineouts from the per-chunk partial sums; the chunks of band b are
+// [band_chunk[b], band_chunk[b+1]).  A block takes band blockIdx.x and
+// 256 / band_rows frames from blockIdx.y on: one thread per (row, frame),
+// consecutive threads on consecutive frames (the partials of a (chunk, row)
+// are frame-contiguous).  Up to 8 chunks' loads are in flight before the
+// first add (the sums keep chunk order); the three values of each thread
+// are staged in LDS and stored as runs of the band's rows of one frame's
+// lineout.
+__global__ __launch_bounds__(256) void k_lineout_finalize(
+    const float *__restrict__ part, const int32_t *__restrict__ band_chunk,
+    const float4 *__restrict__ row_static, int n_frames, int n_r, int band_rows, int np,
+    int mask_nan, float *__restrict__ lineouts) {
+  __shared__ float st[3 * 256];  // [frame in block][lineout][row in band]
+  const int fb = 256 / band_rows;  // frames per block
+  const int band = blockIdx.x, r0 = band * band_rows;
+  const int rows = min(band_rows, n_r - r0);
+  const int rib = threadIdx.x / fb, fl = threadIdx.x - rib * fb;
+  const int f0 = blockIdx.y * fb, f = f0 + fl;
+  const int nf = min(fb, n_frames - f0);
+  if (rib < rows && fl < nf) {
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int c1 = band_chunk[band + 1];
+    const int64_t cs = (int64_t)band_rows * 4 * np;  // floats from one chunk's partials to the next
+    for (int c = band_chunk[band]; c < c1; c += 8) {
+      const float *L = part + ((int64_t)c * band_rows + rib) * 4 * np + f;
+      const int n = c1 - c;
+      float4 v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < n)
+          v[k] = make_float4(__builtin_nontemporal_load(L + k * cs),
+                             __builtin_nontemporal_load(L + k * cs + np),
+                             __builtin_nontemporal_load(L + k * cs + 2 * np),
+                             __builtin_nontemporal_load(L + k * cs + 3 * np));
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < n) {
+          s.x += v[k].x;
+          s.y += v[k].y;
+          s.z += v[k].z;
+          s.w += v[k].w;
+        }
+    }
+    write_lineouts(s, row_static[r0 + rib], mask_nan, st + fl * 3 * band_rows + rib, band_rows,
+                   0);
   }
-  write_lineouts(s, row_static[r], mask_nan, lineouts + (int64_t)f * 3 * n_r, n_r, r);
+  __syncthreads();
+  for (int i = threadIdx.x; i < 3 * band_rows * nf; i += 256) {
+    const int fk = i / band_rows, j = i - fk * band_rows;  // (frame, lineout), row
+    if (j < rows) {
+      const int g = fk / 3, k = fk - 3 * g;
+      lineouts[((int64_t)(f0 + g) * 3 + k) * n_r + r0 + j] = st[i];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -1554,8 +1586,9 @@ static int launch_batched_g(hipStream_t s, const Map &m, const Tiles &t, int mas
   MC_HIP(hipGetLastError());
   // 3. lineouts
   if (lineouts) {
-    const int64_t n = (int64_t)n_frames * m.n_r;
-    hipLaunchKernelGGL(k_lineout_finalize, dim3(blocks_for(n, 256)), dim3(256), 0, s, part,
+    const int fb = 256 / t.rows;
+    hipLaunchKernelGGL(k_lineout_finalize, dim3((unsigned)t.n_bands, (n_frames + fb - 1) / fb),
+                       dim3(256), 0, s, part,
                        t.band_chunk.as<int32_t>(), m.d_row_static.as<float4>(), n_frames,
                        m.n_r, t.rows, XF, mask_nan, lineouts);
     MC_HIP(hipGetLastError());
