@@ -145,7 +145,7 @@ def fit_bench(g, P, frames, log, n_frames=32):
     return res
 
 
-def qmode_bench(log, steps=5, warmup=2):
+def qmode_bench(log, steps=5, warmup=2, serving=True):
     """The Q-mode workload (2880x2880 float32, 11500 x 720 Q x eta bins,
     SubPixelLevel 2, tilts, parallax, gradient mode): map build on the GPU,
     64-frame batched passes and the one-frame kernel, with the roofline of
@@ -190,7 +190,8 @@ def qmode_bench(log, steps=5, warmup=2):
                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                         "bytes_per_frame": bpf},
            "map_build_s": t_map, "layout_s": t_layout, "layout": layout}
-    out["serving"] = qmode_serving_bench(g, P, frames[:8].cpu().numpy(), log)
+    out["serving"] = (qmode_serving_bench(g, P, frames[:8].cpu().numpy(), log)
+                      if serving else None)
     g.close()
     del frames, cakes, lo
     torch.cuda.empty_cache()
@@ -295,6 +296,8 @@ def main():
     ap.add_argument("--serving-rates", default="100,500,1000,2000,5000")
     ap.add_argument("--no-fits", action="store_true")
     ap.add_argument("--no-qmode", action="store_true")
+    ap.add_argument("--no-qmode-serving", action="store_true",
+                    help="skip the Q-mode workload's streaming leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -414,7 +417,7 @@ def main():
     if not args.no_serving:
         rates = tuple(float(r) for r in args.serving_rates.split(",") if r)
         serving = serving_bench(rates, args.serving_frames, log)
-    qmode = None if args.no_qmode else qmode_bench(log)
+    qmode = None if args.no_qmode else qmode_bench(log, serving=not args.no_qmode_serving)
 
     layout = g.layout_info(args.mode)
     traffic = None
