@@ -114,3 +114,29 @@ def test_c2_batched_and_one_frame_match_oracle(c2, oracle_lib):
                 el1 = np.abs(got1[r] - want[r]) / np.maximum(np.abs(scale[r]), 1e-30)
                 assert el1.max() <= TOL, f"one-frame lineout {r}: {el1.max():.3g}"
     print(f"[c2] worst cake rel err over the checked lanes {worst:.3g}", flush=True)
+
+
+@pytest.mark.parametrize("batch", [16, 32])
+def test_c2_narrow_passes_match_oracle(c2, oracle_lib, batch):
+    """The 16- and 32-frame pass kernels (one and two frames per lane) on
+    the full 2048^2 layout: 32 frames in passes of `batch`, lanes checked
+    against the FP64 oracle at 1e-5 (cakes and lineouts)."""
+    P, frames, dark, g, pm, om = c2
+    saved = g.tuning()
+    g.set_tuning(batch=batch, rows_per_group=16, lds_bytes=saved[2])
+    try:
+        batch32 = np.concatenate([frames + np.uint16(k) for k in range(4)])
+        print(f"[c2] 32 frames in passes of {batch}", flush=True)
+        cakes, lo = ma.integrate_batch(batch32, g, mode="bilinear", mask_nan=True)
+    finally:
+        g.set_tuning(*saved)
+    ainb = oracle_lib.area_inbounds(om, P.NrPixelsY, P.NrPixelsZ)
+    for lane in (0, 9, 17, 31):
+        ref, sc = _oracle(oracle_lib, om, P, batch32[lane], dark)
+        e = rel_err(cakes[lane], ref, sc)
+        assert e.max() <= TOL, f"width {batch} lane {lane}: max rel err {e.max():.3g}"
+        want = oracle_lib.lineouts(ref, ainb)
+        scale = oracle_lib.lineouts(np.where(np.isnan(ref), np.nan, sc), ainb)
+        for r in range(3):
+            el = np.abs(lo[lane, r] - want[r]) / np.maximum(np.abs(scale[r]), 1e-30)
+            assert el.max() <= TOL, f"width {batch} lane {lane} lineout {r}: {el.max():.3g}"
