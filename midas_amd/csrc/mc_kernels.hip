@@ -807,14 +807,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
   const int t_beg = chunks[4 * c], t_end = chunks[4 * c + 1];
   const bool want_lo = part != nullptr;
   float lo[G][4];
-  VT ob[G];
-  int og[G];
-  uint32_t om[G];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    og[g] = -1;
-    om[g] = 0u;
-    ob[g] = 0.f;
 #pragma unroll
     for (int k = 0; k < 4; ++k) lo[g][k] = 0.f;
   }
@@ -851,6 +845,14 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
       const int jr = kWaves * g + wave - h.ra;  // row inside the tile
       if (jr < 0 || jr >= h.nrows) continue;
       const int q_row = h.q_beg + jr * h.nq;
+      // 64-frame passes: the granule being filled (this lane's frame, 8
+      // bins of the row) and its bins so far.  A chunk's tiles are every
+      // k-th tile of its band, so a granule never continues into the
+      // chunk's next tile: it is stored when the row's quads of this tile
+      // are done.
+      VT ob = 0.f;
+      int og = -1;
+      uint32_t om = 0u;
       // The pair this group reads next and its entry index: a quad whose
       // slots all hold staged entries reads on where the previous full quad
       // ended (the row's entries are one run), others read their first pair.
@@ -1001,30 +1003,30 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
           for (int s = 0; s < 4; ++s) o[s] = (o[s] - Q.dterm[s]) * Q.scale[s];  // NaN: masked
           const int gr = Q.b0 / LG;
           const bool hi = (Q.b0 & 4) != 0;
-          if (gr != og[g]) {
-            if (om[g] && live) flush_granule<LG>(ob[g], og[g], om[g], frame, cakes, nb, vec_ok);
-            og[g] = gr;
-            om[g] = 0u;
+          if (gr != og) {
+            if (om && live) flush_granule<LG>(ob, og, om, frame, cakes, nb, vec_ok);
+            og = gr;
+            om = 0u;
           }
-          om[g] |= hi ? pm << 4 : pm;
+          om |= hi ? pm << 4 : pm;
           if (pm == 15u) {
             if (hi) {
-              ob[g][4] = o[0];
-              ob[g][5] = o[1];
-              ob[g][6] = o[2];
-              ob[g][7] = o[3];
+              ob[4] = o[0];
+              ob[5] = o[1];
+              ob[6] = o[2];
+              ob[7] = o[3];
             } else {
-              ob[g][0] = o[0];
-              ob[g][1] = o[1];
-              ob[g][2] = o[2];
-              ob[g][3] = o[3];
+              ob[0] = o[0];
+              ob[1] = o[1];
+              ob[2] = o[2];
+              ob[3] = o[3];
             }
           } else {
 #pragma unroll
             for (int s = 0; s < 4; ++s)
               if ((pm >> s) & 1u) {
-                if (hi) ob[g][4 + s] = o[s];
-                else ob[g][s] = o[s];
+                if (hi) ob[4 + s] = o[s];
+                else ob[s] = o[s];
               }
           }
 #pragma unroll
@@ -1091,13 +1093,13 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(8, 
           lineout_add(lo[g], o, aa, af);
         }
       }
+      if constexpr (FPL == 4) {
+        if (om && live) flush_granule<LG>(ob, og, om, frame, cakes, nb, vec_ok);
+      }
     }
   }
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    if constexpr (FPL == 4) {
-      if (om[g] && live) flush_granule<LG>(ob[g], og[g], om[g], frame, cakes, nb, vec_ok);
-    }
     if (want_lo) {
       float v[4];
 #pragma unroll
